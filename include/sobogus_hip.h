@@ -1,0 +1,204 @@
+/*
+ * sobogus_hip.h — C ABI of the MI355X (gfx950) Coulomb-friction solve path.
+ *
+ * The ABI replaces the compute behind these reference interfaces
+ * (bogus 1.4.2 / So-bogus; paths relative to its src/ tree):
+ *   - SparseBlockMatrix storage and multiply<Transpose>(rhs, res, alpha, beta)
+ *       Core/Block/SparseBlockMatrix.hpp:18-135, Core/Block/BlockObjectBase.hpp:64-71,
+ *       Core/Block/SparseMatrixVectorProduct.impl.hpp:382-432
+ *   - the matrix-free product H * MInv * H^T (Product / Transpose expressions)
+ *       Core/Block/Expressions.hpp:18-161, SparseMatrixVectorProduct.impl.hpp:355-376, 457-462
+ *   - ProjectedGradient<WType>::solve(law, b, x)
+ *       Core/BlockSolvers/ProjectedGradient.hpp:36-50, ProjectedGradient.impl.hpp:355-382
+ *   - GaussSeidel<WType>::solve(law, b, x, tryZeroAsWell)
+ *       Core/BlockSolvers/GaussSeidel.hpp:78-79, GaussSeidel.impl.hpp:126-205
+ *   - Coloring::update(true, W)
+ *       Core/BlockSolvers/Coloring.hpp:28-30, Coloring.impl.hpp:20-76
+ *   - ConstrainedSolverBase::eval (the Fischer-Burmeister residual)
+ *       Core/BlockSolvers/ConstrainedSolverBase.impl.hpp:21-74
+ *   - DualFrictionProblem<3>::{computeFrom, solveWith, evalWith, applyPermutation}
+ *       Interfaces/FrictionProblem.hpp:127-214, Interfaces/FrictionProblem.cpp:30-114
+ *
+ * Conventions
+ *   - Return value: SB_OK (0), a positive warning (SB_NOT_CONVERGED) or a
+ *     negative SB_ERR_* code. sb_status_string() names it. Output residuals are
+ *     always written when the call gets far enough to compute one.
+ *   - Host arrays are caller-owned and copied; objects are opaque, owned by the
+ *     library, and freed by the matching *_destroy.
+ *   - Vectors r, b, u are 3 * n_contacts doubles, contact i at [3i, 3i+3) as
+ *     (normal, tangent, tangent). Blocks are row-major.
+ *   - One sb_ctx per host thread; a context owns one HIP stream on one device.
+ *   - Every call is synchronous with respect to the host unless its name ends in
+ *     _async (none yet).
+ */
+#ifndef SOBOGUS_HIP_H
+#define SOBOGUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sb_status {
+  SB_OK = 0,
+  SB_NOT_CONVERGED = 1,       /* warning: tolerance not reached within max_iters */
+  SB_ERR_INVALID = -1,        /* bad argument */
+  SB_ERR_OOM = -2,            /* device or host allocation failed */
+  SB_ERR_HIP = -3,            /* HIP runtime error */
+  SB_ERR_NO_DEVICE = -4,      /* no usable GPU */
+  SB_ERR_UNSUPPORTED = -5,    /* option not available on this path */
+  SB_ERR_BARRIER_TIMEOUT = -6,/* a persistent kernel's grid barrier timed out */
+  SB_ERR_NAN = -7             /* non-finite residual */
+};
+
+/* Solver variants (Core/BlockSolvers.fwd.hpp:29-46, projected_gradient::Variant) */
+enum sb_pg_variant { SB_PG_STANDARD = 0, SB_PG_DESCENT = 1, SB_PG_CONJUGATED = 2, SB_PG_APGD = 3, SB_PG_SPG = 4 };
+/* Local solver strategies (Extra/SecondOrder.fwd.hpp:29-55, local_soc_solver::Strategy) */
+enum sb_local_strategy { SB_PURE_NEWTON = 0, SB_PURE_ENUMERATIVE = 1, SB_HYBRID = 2, SB_REV_HYBRID = 3 };
+/* Laws: Coulomb friction (De Saxce change of variable, SOCLaw<3,double,true>) or
+   plain second-order-cone complementarity (SOCLaw<3,double,false>) */
+enum sb_law { SB_LAW_SOC = 0, SB_LAW_COULOMB = 1 };
+/* Operator used by the projected-gradient solver */
+enum sb_operator { SB_OP_ASSEMBLED = 0, SB_OP_MATRIX_FREE = 1 };
+/* Residual row scaling: reference semantics (max(1, tr(W_ii)/3) for an assembled
+   W, 1 for the matrix-free product) or max(1, tr(W_ii)/3) in both modes */
+enum sb_scaling { SB_SCALING_REFERENCE = 0, SB_SCALING_DIAGONAL = 1, SB_SCALING_NONE = 2 };
+/* Problem creation flags */
+enum sb_problem_flags { SB_PROBLEM_ASSEMBLE_W = 1 };
+/* BSR flags (bogus::flags) */
+enum sb_bsr_flags { SB_BSR_SYMMETRIC = 1 /* only blocks with col <= row are given */ };
+/* Coloring modes */
+enum sb_coloring_mode {
+  SB_COLORING_REFERENCE = 0, /* Coloring::compute as written (lower_bound test; many colors) */
+  SB_COLORING_GREEDY = 1     /* first-fit with an exact membership test (fewer colors) */
+};
+
+typedef struct sb_ctx sb_ctx;
+typedef struct sb_bsr sb_bsr;
+typedef struct sb_problem sb_problem;
+
+/* ProjectedGradient parameters (ProjectedGradient.hpp:57-95, BlockSolverBase.hpp:37-43) */
+typedef struct sb_pg_options {
+  int variant;            /* sb_pg_variant (APGD, DESCENT, SPG on the GPU) */
+  int law;                /* sb_law; DualFrictionProblem::solveWith(pg) defaults to SOC */
+  int op;                 /* sb_operator */
+  int scaling;            /* sb_scaling */
+  int inf_norm;           /* ConstrainedSolverBase::useInfinityNorm */
+  double tol;             /* 1e-6 */
+  int max_iters;          /* 300 */
+  int ls_iters;           /* 8 */
+  double ls_optimistic;   /* 1.25 */
+  double ls_pessimistic;  /* 0.5 */
+  double ls_armijo;       /* 0.5 (Standard/Conjugated only) */
+  int grid_blocks;        /* 0 = automatic; else persistent grid size (clamped) */
+} sb_pg_options;
+
+/* GaussSeidel parameters (GaussSeidelBase.hpp:39-91) */
+typedef struct sb_gs_options {
+  int law;                /* sb_law; DualFrictionProblem::solveWith(gs) defaults to Coulomb */
+  int strategy;           /* sb_local_strategy, RevHybrid by default */
+  int inf_norm;
+  int try_zero;           /* doTryZeroAsWell, 1 */
+  double tol;             /* 1e-6 */
+  int max_iters;          /* 250 */
+  int eval_every;         /* 25 */
+  double skip_tol;        /* 1e-6 */
+  int skip_iters;         /* 10 */
+  double auto_regularization; /* 0 */
+  int grid_blocks;        /* 0 = automatic */
+} sb_gs_options;
+
+typedef struct sb_stats {
+  int iterations;         /* PG iterations / GS sweeps */
+  int spmvs;              /* operator applications */
+  int ls_trials;          /* line-search trials (APGD/Descent) */
+  int nesterov_steps;
+  int converged;
+  int local_failures;     /* GS: local solves that did not reach the local tolerance */
+  double residual;        /* the value returned by the reference solve() */
+  double device_ms;       /* kernel time of the solve (HIP events) */
+  double* history;        /* optional caller buffer: residual per evaluation (callback stream) */
+  int history_capacity;
+  int history_length;     /* written: number of evaluations performed */
+} sb_stats;
+
+const char* sb_status_string(int status);
+void sb_pg_options_default(sb_pg_options* o);
+void sb_gs_options_default(sb_gs_options* o);
+
+/* ---------------------------------------------------------------- context */
+int sb_ctx_create(sb_ctx** ctx, int device);
+int sb_ctx_destroy(sb_ctx* ctx);
+/* Device name and compute-unit count (for logs). */
+int sb_ctx_info(sb_ctx* ctx, char* name, int name_len, int* n_cu);
+
+/* ---------------------------------------------------------------- BSR (SparseBlockMatrix) */
+/* Creates a device BSR matrix of nbrows x nbcols blocks of br x bc (br, bc <= 8). With
+   SB_BSR_SYMMETRIC, only the lower triangle (col <= row) is given, blocks square. */
+int sb_bsr_create(sb_ctx* ctx, sb_bsr** A, int nbrows, int nbcols, int br, int bc, int flags,
+                  const int32_t* rowptr, const int32_t* colidx, const double* vals);
+int sb_bsr_destroy(sb_bsr* A);
+/* res = alpha * op(A) * x + beta * res (SparseBlockMatrixBase::multiply<Transpose>).
+   x and res are host arrays when on_device == 0, device pointers otherwise. */
+int sb_bsr_multiply(sb_ctx* ctx, const sb_bsr* A, int transpose, double alpha, const double* x, double beta,
+                    double* res, int on_device);
+/* res += sum_{j != row} A_{row,j} x_j for a symmetric 3x3 matrix (splitRowMultiply), host arrays. */
+int sb_bsr_split_row_multiply(sb_ctx* ctx, const sb_bsr* A, int row, const double* x, double* res3);
+
+/* ---------------------------------------------------------------- friction problem */
+/* A dual problem u = W r + b with W = H Minv H^T, built from the primal data:
+   H: n_contacts x n_bodies BSR of 3x6 blocks (row-major), Minv: n_bodies 6x6 blocks
+   (explicit inverse mass, row-major), b: 3 n_contacts, mu: n_contacts.
+   With SB_PROBLEM_ASSEMBLE_W the library also assembles W (DualFrictionProblem::computeFrom). */
+int sb_problem_create(sb_ctx* ctx, sb_problem** P, int n_contacts, int n_bodies, const int32_t* h_rowptr,
+                      const int32_t* h_colidx, const double* h_vals, const double* minv, const double* b,
+                      const double* mu, int flags);
+/* Supplies an assembled W (lower triangle, 3x3 row-major blocks) instead of assembling it. */
+int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* w_rowptr, const int32_t* w_colidx,
+                     const double* w_vals);
+int sb_problem_destroy(sb_problem* P);
+int sb_problem_sizes(const sb_problem* P, int* n_contacts, int* n_bodies, int64_t* h_blocks, int64_t* w_blocks_lower);
+/* Copies out the assembled lower-triangle W (arrays sized by sb_problem_sizes). */
+int sb_problem_get_w(sb_ctx* ctx, const sb_problem* P, int32_t* w_rowptr, int32_t* w_colidx, double* w_vals);
+/* Contact permutation, new[i] = old[perm[i]] (DualFrictionProblem::applyPermutation). */
+int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm);
+/* Coloring of the contacts (Coloring::update(true, W)); perm has n_contacts entries and colors
+   n_colors + 1 offsets (allocate n_contacts + 1). Requires an assembled W. */
+int sb_problem_coloring(sb_ctx* ctx, const sb_problem* P, int mode, int64_t* perm, int64_t* colors, int* n_colors);
+
+/* y = W x (op = SB_OP_ASSEMBLED) or H (Minv (H^T x)) (SB_OP_MATRIX_FREE). Host or device arrays. */
+int sb_problem_apply_w(sb_ctx* ctx, const sb_problem* P, int op, const double* x, double* y, int on_device);
+/* Fischer-Burmeister residual of r (DualFrictionProblem::evalWith): u = W r + b, then
+   sum_i |FB(s_i r_i, u_i)|^2 / (n+1), or the max with inf_norm. */
+int sb_fb_residual(sb_ctx* ctx, const sb_problem* P, int law, int op, int scaling, int inf_norm, const double* r,
+                   double* residual, int on_device);
+
+/* ProjectedGradient::solve on the problem; r is warm start and result (host or device). */
+int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, double* r, int on_device,
+                double* residual, sb_stats* stats);
+/* GaussSeidel::solve on an assembled problem whose contacts are color-contiguous:
+   colors holds n_colors + 1 offsets (pass NULL / 0 for a single color). */
+int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
+                double* r, int on_device, double* residual, sb_stats* stats);
+
+/* ---------------------------------------------------------------- raw device memory helpers
+   (for callers that keep vectors resident in HBM between calls) */
+int sb_device_alloc(sb_ctx* ctx, void** ptr, size_t bytes);
+int sb_device_free(sb_ctx* ctx, void* ptr);
+int sb_memcpy_h2d(sb_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sb_memcpy_d2h(sb_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sb_synchronize(sb_ctx* ctx);
+
+/* ---------------------------------------------------------------- benchmarking hooks
+   Times `reps` back-to-back launches of one block-SpMV kernel of the problem on the
+   context's stream with HIP events; kind: 0 = assembled W x, 1 = H w, 2 = H^T r (+ Minv),
+   3 = matrix-free W x (H^T+Minv, then H). Returns the mean milliseconds per launch. */
+int sb_bench_spmv(sb_ctx* ctx, const sb_problem* P, int kind, int reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOBOGUS_HIP_H */

@@ -1,0 +1,1009 @@
+// extern "C" layer of the MI355X friction solve path (include/sobogus_hip.h).
+// Host setup (tiling, W assembly, coloring) happens here; every solve, product
+// and residual runs as HIP kernels on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sobogus_hip.h"
+#include "sb_host.hpp"
+#include "sb_problem.hpp"
+#include "sb_spmv.hpp"
+
+using sb::DevProblem;
+using sb::TileMat;
+
+namespace {
+
+constexpr int kThreads = 256;
+
+#define SB_HIP(expr)                                  \
+  do {                                                \
+    hipError_t e_ = (expr);                           \
+    if (e_ != hipSuccess) {                           \
+      last_error_string() = hipGetErrorString(e_);    \
+      return e_ == hipErrorOutOfMemory ? SB_ERR_OOM : SB_ERR_HIP; \
+    }                                                 \
+  } while (0)
+
+std::string& last_error_string() {
+  static thread_local std::string s;
+  return s;
+}
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, sizeof(T) * count);
+  }
+  hipError_t upload(const T* src, size_t count, hipStream_t s) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
+  }
+  hipError_t upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+};
+
+struct DevTiles {
+  DBuf<int> tstart, brow, bcol, tlong, long_row, long_t0, long_t1, rowptr;
+  DBuf<double> vals;
+  sbhost::Tiles host;  // index arrays kept (values dropped after upload)
+  hipError_t upload(sbhost::Tiles&& t, hipStream_t s) {
+    host = std::move(t);
+    hipError_t e;
+    if ((e = tstart.upload(host.tstart, s)) != hipSuccess) return e;
+    if ((e = brow.upload(host.brow, s)) != hipSuccess) return e;
+    if ((e = bcol.upload(host.bcol, s)) != hipSuccess) return e;
+    if ((e = tlong.upload(host.tlong, s)) != hipSuccess) return e;
+    if ((e = long_row.upload(host.long_row, s)) != hipSuccess) return e;
+    if ((e = long_t0.upload(host.long_t0, s)) != hipSuccess) return e;
+    if ((e = long_t1.upload(host.long_t1, s)) != hipSuccess) return e;
+    if ((e = rowptr.upload(host.rowptr, s)) != hipSuccess) return e;
+    if ((e = vals.upload(host.vals, s)) != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    std::vector<double>().swap(host.vals);
+    return e;
+  }
+  TileMat<double> view() const {
+    TileMat<double> m;
+    m.nrows = host.nrows;
+    m.ncols = host.ncols;
+    m.nblocks = host.nblocks;
+    m.ntiles = host.ntiles;
+    m.scan_steps = host.scan_steps;
+    m.tstart = tstart.p;
+    m.brow = brow.p;
+    m.bcol = bcol.p;
+    m.tlong = tlong.p;
+    m.vals = vals.p;
+    m.nlong = (int)host.long_row.size();
+    m.long_row = long_row.p;
+    m.long_t0 = long_t0.p;
+    m.long_t1 = long_t1.p;
+    return m;
+  }
+  size_t carryNeed(int br) const { return (size_t)host.ntiles * br; }
+};
+
+}  // namespace
+
+struct sb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int ncu = 0;
+  std::string name;
+  int coop = 0;
+  DBuf<unsigned> count, gen;
+  DBuf<int> error;
+  DBuf<double> partials, reduced, out;
+  int maxGrid = 0;  // capacity of partials (blocks)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct sb_bsr {
+  int nbr = 0, nbc = 0, br = 0, bc = 0, flags = 0;
+  DevTiles fwd, tr;  // product and transposed product (symmetric: both equal the expanded matrix)
+  DBuf<double> carry, xbuf, ybuf;
+  std::vector<int> lrp, lci;  // symmetric lower pattern (split-row multiply)
+  DBuf<int> d_lrp, d_lci;
+  DBuf<double> d_lvals;
+};
+
+struct sb_problem {
+  sb_ctx* ctx = nullptr;
+  int nc = 0, nb = 0;
+  std::vector<int> hrp, hci;
+  std::vector<double> hv, minv, b, mu, diag, scal;
+  bool hasW = false;
+  std::vector<int> wrp, wci;
+  std::vector<double> wv;
+  DevTiles H, HT, W;
+  DBuf<double> d_minv, d_b, d_mu, d_diag, d_scal, d_ones;
+  // workspace
+  DBuf<double> X[3], MX[3], G[2], xbest, wbody, carry, rio, hist, y, terms, local, reg, ndx;
+  DBuf<unsigned char> skip;
+  DBuf<int> colors;
+  DevProblem<double> view(bool withScaling) const {
+    DevProblem<double> v;
+    v.nc = nc;
+    v.nb = nb;
+    v.H = H.view();
+    v.HT = HT.view();
+    v.W = W.view();
+    v.minv = d_minv.p;
+    v.b = d_b.p;
+    v.mu = d_mu.p;
+    v.diag = d_diag.p;
+    v.scal = withScaling ? d_scal.p : d_ones.p;
+    v.hasW = hasW;
+    return v;
+  }
+};
+
+namespace {
+
+int check_ctx(sb_ctx* ctx) {
+  if (!ctx) return SB_ERR_INVALID;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) {
+    last_error_string() = hipGetErrorString(e);
+    return SB_ERR_HIP;
+  }
+  return SB_OK;
+}
+
+int ensure_grid_scratch(sb_ctx* ctx, int blocks) {
+  if (blocks <= ctx->maxGrid) return SB_OK;
+  SB_HIP(ctx->partials.alloc((size_t)blocks * sb::kMaxPartials));
+  ctx->maxGrid = blocks;
+  return SB_OK;
+}
+
+// Element-major 6x6 inverse-mass blocks.
+std::vector<double> minv_element_major(int nb, const std::vector<double>& minv) {
+  std::vector<double> em((size_t)36 * nb);
+  for (int k = 0; k < nb; ++k)
+    for (int e = 0; e < 36; ++e) em[(size_t)e * nb + k] = minv[(size_t)k * 36 + e];
+  return em;
+}
+
+int upload_problem(sb_problem* P) {
+  hipStream_t s = P->ctx->stream;
+  const int nc = P->nc, nb = P->nb;
+  // H contact-major, H^T body-major
+  SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
+  {
+    std::vector<int> trp, tci;
+    std::vector<double> tv;
+    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv);
+    SB_HIP(P->HT.upload(sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data()), s));
+  }
+  if (P->hasW) {
+    std::vector<int> frp, fci;
+    std::vector<double> fv;
+    sbhost::expandSymmetric(nc, 3, P->wrp.data(), P->wci.data(), P->wv.data(), frp, fci, fv);
+    SB_HIP(P->W.upload(sbhost::buildTiles(nc, nc, 3, 3, frp.data(), fci.data(), fv.data()), s));
+  }
+  SB_HIP(P->d_minv.upload(minv_element_major(nb, P->minv), s));
+  SB_HIP(P->d_b.upload(P->b, s));
+  SB_HIP(P->d_mu.upload(P->mu, s));
+  // diagonal blocks: from the assembled W when present, else from H Minv H^T
+  if (P->hasW) {
+    P->diag.assign((size_t)9 * nc, 0.0);
+    for (int i = 0; i < nc; ++i)
+      for (int p = P->wrp[i]; p < P->wrp[i + 1]; ++p)
+        if (P->wci[p] == i) std::memcpy(&P->diag[(size_t)9 * i], &P->wv[(size_t)9 * p], 9 * sizeof(double));
+  } else {
+    sbhost::diagBlocks(nc, P->hrp.data(), P->hci.data(), P->hv.data(), P->minv.data(), P->diag);
+  }
+  P->scal.resize(nc);
+  for (int i = 0; i < nc; ++i) {
+    const double* d = &P->diag[(size_t)9 * i];
+    P->scal[i] = std::max(1.0, (d[0] + d[4] + d[8]) / 3.0);  // estimate_block_scaling
+  }
+  SB_HIP(P->d_diag.upload(P->diag, s));
+  SB_HIP(P->d_scal.upload(P->scal, s));
+  std::vector<double> ones(nc, 1.0);
+  SB_HIP(P->d_ones.upload(ones, s));
+  const size_t n3 = 3 * (size_t)nc;
+  for (int k = 0; k < 3; ++k) {
+    SB_HIP(P->X[k].alloc(n3));
+    SB_HIP(P->MX[k].alloc(n3));
+  }
+  SB_HIP(P->G[0].alloc(n3));
+  SB_HIP(P->G[1].alloc(n3));
+  SB_HIP(P->xbest.alloc(n3));
+  SB_HIP(P->rio.alloc(n3));
+  SB_HIP(P->y.alloc(n3));
+  SB_HIP(P->terms.alloc(nc));
+  SB_HIP(P->wbody.alloc(6 * (size_t)nb));
+  size_t carry = std::max({P->H.carryNeed(3), P->HT.carryNeed(6), P->W.carryNeed(3), (size_t)8});
+  SB_HIP(P->carry.alloc(carry));
+  SB_HIP(hipStreamSynchronize(s));
+  return SB_OK;
+}
+
+int copy_in(sb_ctx* ctx, double* dst, const double* src, size_t n, int on_device) {
+  SB_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        ctx->stream));
+  return SB_OK;
+}
+int copy_out(sb_ctx* ctx, double* dst, const double* src, size_t n, int on_device) {
+  SB_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        ctx->stream));
+  return SB_OK;
+}
+
+int grid_for(sb_ctx* ctx, const void* func, int requested, int work_waves) {
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, func, kThreads, 0) != hipSuccess || perCU < 1) perCU = 1;
+  const int maxCoop = perCU * ctx->ncu;
+  int want = requested > 0 ? requested : (work_waves + (kThreads / 64) - 1) / (kThreads / 64);
+  want = std::max(1, std::min(want, maxCoop));
+  return want;
+}
+
+int read_status(sb_ctx* ctx, double* out, int nout) {
+  SB_HIP(hipMemcpyAsync(out, ctx->out.p, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
+  sb::GridSync g;
+  g.count = ctx->count.p;
+  g.gen = ctx->gen.p;
+  g.error = ctx->error.p;
+  g.partials = ctx->partials.p;
+  g.reduced = ctx->reduced.p;
+  g.nblocks = (unsigned)blocks;
+  g.timeout_ticks = 100ull * 1000 * 1000 * 20;  // 20 s at 100 MHz
+  return g;
+}
+
+// smallest eigenvalue of a symmetric 3x3 block (Jacobi rotations)
+double min_eig_sym3(const double* A) {
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = 0.5 * (A[3 * i + j] + A[3 * j + i]);
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    if (a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2] < 1e-300) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (a[p][q] == 0) continue;
+        const double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+        const double t = std::copysign(1.0, th) / (std::fabs(th) + std::sqrt(th * th + 1));
+        const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - s * akq;
+          a[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - s * aqk;
+          a[q][k] = s * apk + c * aqk;
+        }
+      }
+  }
+  return std::min(a[0][0], std::min(a[1][1], a[2][2]));
+}
+
+__global__ void k_split_row(const int* lrp, const int* lci, const double* lv, int n, int row, const double* x,
+                            double* res) {
+  // one thread: res += sum_{j != row} W_{row,j} x_j over the lower pattern (and its transpose)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double acc[3] = {res[0], res[1], res[2]};
+  for (int i = 0; i < n; ++i)
+    for (int p = lrp[i]; p < lrp[i + 1]; ++p) {
+      const int j = lci[p];
+      const double* B = lv + (size_t)9 * p;
+      if (i == row && j < row) {
+        for (int r = 0; r < 3; ++r) acc[r] += B[3 * r] * x[3 * j] + B[3 * r + 1] * x[3 * j + 1] + B[3 * r + 2] * x[3 * j + 2];
+      } else if (j == row && i > row) {
+        for (int r = 0; r < 3; ++r) acc[r] += B[r] * x[3 * i] + B[3 + r] * x[3 * i + 1] + B[6 + r] * x[3 * i + 2];
+      }
+    }
+  res[0] = acc[0];
+  res[1] = acc[1];
+  res[2] = acc[2];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sb_status_string(int status) {
+  switch (status) {
+    case SB_OK: return "ok";
+    case SB_NOT_CONVERGED: return "not converged (tolerance not reached within max_iters)";
+    case SB_ERR_INVALID: return "invalid argument";
+    case SB_ERR_OOM: return "out of memory";
+    case SB_ERR_HIP: return last_error_string().empty() ? "HIP runtime error" : last_error_string().c_str();
+    case SB_ERR_NO_DEVICE: return "no usable GPU device";
+    case SB_ERR_UNSUPPORTED: return "unsupported option on this path";
+    case SB_ERR_BARRIER_TIMEOUT: return "grid barrier timed out in a persistent kernel";
+    case SB_ERR_NAN: return "non-finite residual";
+    default: return "unknown status";
+  }
+}
+
+void sb_pg_options_default(sb_pg_options* o) {
+  if (!o) return;
+  o->variant = SB_PG_APGD;
+  o->law = SB_LAW_SOC;
+  o->op = SB_OP_ASSEMBLED;
+  o->scaling = SB_SCALING_REFERENCE;
+  o->inf_norm = 0;
+  o->tol = 1e-6;
+  o->max_iters = 300;
+  o->ls_iters = 8;
+  o->ls_optimistic = 1.25;
+  o->ls_pessimistic = 0.5;
+  o->ls_armijo = 0.5;
+  o->grid_blocks = 0;
+}
+
+void sb_gs_options_default(sb_gs_options* o) {
+  if (!o) return;
+  o->law = SB_LAW_COULOMB;
+  o->strategy = SB_REV_HYBRID;
+  o->inf_norm = 0;
+  o->try_zero = 1;
+  o->tol = 1e-6;
+  o->max_iters = 250;
+  o->eval_every = 25;
+  o->skip_tol = 1e-6;
+  o->skip_iters = 10;
+  o->auto_regularization = 0.0;
+  o->grid_blocks = 0;
+}
+
+int sb_ctx_create(sb_ctx** out, int device) {
+  if (!out) return SB_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SB_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return SB_ERR_INVALID;
+  std::unique_ptr<sb_ctx> c(new (std::nothrow) sb_ctx());
+  if (!c) return SB_ERR_OOM;
+  c->device = device;
+  SB_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  SB_HIP(hipGetDeviceProperties(&prop, device));
+  c->ncu = prop.multiProcessorCount;
+  c->name = prop.gcnArchName;
+  SB_HIP(hipDeviceGetAttribute(&c->coop, hipDeviceAttributeCooperativeLaunch, device));
+  SB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  SB_HIP(c->count.alloc(1));
+  SB_HIP(c->gen.alloc(1));
+  SB_HIP(c->error.alloc(1));
+  SB_HIP(c->reduced.alloc(sb::kMaxPartials));
+  SB_HIP(c->out.alloc(16));
+  SB_HIP(hipEventCreate(&c->ev0));
+  SB_HIP(hipEventCreate(&c->ev1));
+  int rc = ensure_grid_scratch(c.get(), std::max(1, c->ncu * 8));
+  if (rc != SB_OK) return rc;
+  *out = c.release();
+  return SB_OK;
+}
+
+int sb_ctx_destroy(sb_ctx* ctx) {
+  if (!ctx) return SB_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  hipStream_t s = ctx->stream;
+  delete ctx;
+  if (s) (void)hipStreamDestroy(s);
+  return SB_OK;
+}
+
+int sb_ctx_info(sb_ctx* ctx, char* name, int name_len, int* n_cu) {
+  if (!ctx) return SB_ERR_INVALID;
+  if (name && name_len > 0) {
+    std::strncpy(name, ctx->name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (n_cu) *n_cu = ctx->ncu;
+  return SB_OK;
+}
+
+// ------------------------------------------------------------------ BSR
+int sb_bsr_create(sb_ctx* ctx, sb_bsr** A, int nbr, int nbc, int br, int bc, int flags, const int32_t* rowptr,
+                  const int32_t* colidx, const double* vals) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!A || nbr < 0 || nbc < 0 || br < 1 || bc < 1 || br > 8 || bc > 8 || !rowptr) return SB_ERR_INVALID;
+  const bool sym = (flags & SB_BSR_SYMMETRIC) != 0;
+  if (sym && (br != bc || nbr != nbc)) return SB_ERR_INVALID;
+  const int nnz = rowptr[nbr];
+  if (nnz < 0 || (nnz > 0 && (!colidx || !vals))) return SB_ERR_INVALID;
+  for (int i = 0; i < nbr; ++i)
+    if (rowptr[i + 1] < rowptr[i]) return SB_ERR_INVALID;
+  for (int p = 0; p < nnz; ++p)
+    if (colidx[p] < 0 || colidx[p] >= nbc) return SB_ERR_INVALID;
+  std::unique_ptr<sb_bsr> M(new (std::nothrow) sb_bsr());
+  if (!M) return SB_ERR_OOM;
+  M->nbr = nbr;
+  M->nbc = nbc;
+  M->br = br;
+  M->bc = bc;
+  M->flags = flags;
+  hipStream_t s = ctx->stream;
+  if (sym) {
+    std::vector<int> frp, fci;
+    std::vector<double> fv;
+    sbhost::expandSymmetric(nbr, br, rowptr, colidx, vals, frp, fci, fv);
+    SB_HIP(M->fwd.upload(sbhost::buildTiles(nbr, nbc, br, bc, frp.data(), fci.data(), fv.data()), s));
+    if (br == 3) {
+      M->lrp.assign(rowptr, rowptr + nbr + 1);
+      M->lci.assign(colidx, colidx + nnz);
+      SB_HIP(M->d_lrp.upload(M->lrp, s));
+      SB_HIP(M->d_lci.upload(M->lci, s));
+      SB_HIP(M->d_lvals.upload(vals, (size_t)nnz * 9, s));
+    }
+  } else {
+    SB_HIP(M->fwd.upload(sbhost::buildTiles(nbr, nbc, br, bc, rowptr, colidx, vals), s));
+    std::vector<int> trp, tci;
+    std::vector<double> tv;
+    sbhost::transposeBSR(nbr, nbc, br, bc, rowptr, colidx, vals, trp, tci, tv);
+    SB_HIP(M->tr.upload(sbhost::buildTiles(nbc, nbr, bc, br, trp.data(), tci.data(), tv.data()), s));
+  }
+  const size_t carry = std::max({(size_t)M->fwd.host.ntiles, (size_t)M->tr.host.ntiles, (size_t)1}) * 8;
+  SB_HIP(M->carry.alloc(carry));
+  SB_HIP(hipStreamSynchronize(s));
+  *A = M.release();
+  return SB_OK;
+}
+
+int sb_bsr_destroy(sb_bsr* A) {
+  if (!A) return SB_ERR_INVALID;
+  delete A;
+  return SB_OK;
+}
+
+int sb_bsr_multiply(sb_ctx* ctx, const sb_bsr* Ac, int transpose, double alpha, const double* x, double beta,
+                    double* res, int on_device) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Ac || !x || !res) return SB_ERR_INVALID;
+  sb_bsr* A = const_cast<sb_bsr*>(Ac);
+  const bool sym = (A->flags & SB_BSR_SYMMETRIC) != 0;
+  const bool tr = transpose && !sym;
+  const DevTiles& T = tr ? A->tr : A->fwd;
+  const int outRows = tr ? A->nbc : A->nbr, outBs = tr ? A->bc : A->br;
+  const int inRows = tr ? A->nbr : A->nbc, inBs = tr ? A->br : A->bc;
+  const size_t nin = (size_t)inRows * inBs, nout = (size_t)outRows * outBs;
+  const double* dx = x;
+  double* dy = res;
+  hipStream_t s = ctx->stream;
+  if (!on_device) {
+    if (A->xbuf.n < nin) SB_HIP(A->xbuf.alloc(nin));
+    if (A->ybuf.n < nout) SB_HIP(A->ybuf.alloc(nout));
+    SB_HIP(hipMemcpyAsync(A->xbuf.p, x, nin * sizeof(double), hipMemcpyHostToDevice, s));
+    if (beta != 0.0) SB_HIP(hipMemcpyAsync(A->ybuf.p, res, nout * sizeof(double), hipMemcpyHostToDevice, s));
+    dx = A->xbuf.p;
+    dy = A->ybuf.p;
+  }
+  SB_HIP(sb::launch_bsr_generic(T.view(), outBs, inBs, alpha, dx, beta, dy, A->carry.p, s));
+  if (!on_device) SB_HIP(hipMemcpyAsync(res, dy, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  return SB_OK;
+}
+
+int sb_bsr_split_row_multiply(sb_ctx* ctx, const sb_bsr* A, int row, const double* x, double* res3) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!A || !x || !res3 || !(A->flags & SB_BSR_SYMMETRIC) || A->br != 3 || row < 0 || row >= A->nbr)
+    return SB_ERR_INVALID;
+  hipStream_t s = ctx->stream;
+  DBuf<double> dx, dr;
+  SB_HIP(dx.upload(x, (size_t)3 * A->nbr, s));
+  SB_HIP(dr.upload(res3, 3, s));
+  k_split_row<<<1, 64, 0, s>>>(A->d_lrp.p, A->d_lci.p, A->d_lvals.p, A->nbr, row, dx.p, dr.p);
+  SB_HIP(hipGetLastError());
+  SB_HIP(hipMemcpyAsync(res3, dr.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  return SB_OK;
+}
+
+// ------------------------------------------------------------------ problem
+int sb_problem_create(sb_ctx* ctx, sb_problem** out, int nc, int nb, const int32_t* rp, const int32_t* ci,
+                      const double* hv, const double* minv, const double* b, const double* mu, int flags) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!out || nc <= 0 || nb <= 0 || !rp || !ci || !hv || !minv || !b || !mu) return SB_ERR_INVALID;
+  if (rp[0] != 0) return SB_ERR_INVALID;
+  for (int i = 0; i < nc; ++i)
+    if (rp[i + 1] < rp[i]) return SB_ERR_INVALID;
+  const int nnz = rp[nc];
+  for (int p = 0; p < nnz; ++p)
+    if (ci[p] < 0 || ci[p] >= nb) return SB_ERR_INVALID;
+  std::unique_ptr<sb_problem> P(new (std::nothrow) sb_problem());
+  if (!P) return SB_ERR_OOM;
+  P->ctx = ctx;
+  P->nc = nc;
+  P->nb = nb;
+  P->hrp.assign(rp, rp + nc + 1);
+  P->hci.assign(ci, ci + nnz);
+  P->hv.assign(hv, hv + (size_t)nnz * 18);
+  P->minv.assign(minv, minv + (size_t)nb * 36);
+  P->b.assign(b, b + 3 * (size_t)nc);
+  P->mu.assign(mu, mu + nc);
+  if (flags & SB_PROBLEM_ASSEMBLE_W) {
+    sbhost::assembleW(nc, nb, P->hrp.data(), P->hci.data(), P->hv.data(), P->minv.data(), P->wrp, P->wci, P->wv);
+    P->hasW = true;
+  }
+  rc = upload_problem(P.get());
+  if (rc) return rc;
+  *out = P.release();
+  return SB_OK;
+}
+
+int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* wrp, const int32_t* wci, const double* wv) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!P || !wrp || !wci || !wv || wrp[0] != 0) return SB_ERR_INVALID;
+  const int nc = P->nc;
+  for (int i = 0; i < nc; ++i) {
+    if (wrp[i + 1] < wrp[i]) return SB_ERR_INVALID;
+    for (int p = wrp[i]; p < wrp[i + 1]; ++p)
+      if (wci[p] < 0 || wci[p] > i) return SB_ERR_INVALID;  // lower triangle only
+  }
+  P->wrp.assign(wrp, wrp + nc + 1);
+  P->wci.assign(wci, wci + wrp[nc]);
+  P->wv.assign(wv, wv + (size_t)wrp[nc] * 9);
+  P->hasW = true;
+  return upload_problem(P);
+}
+
+int sb_problem_destroy(sb_problem* P) {
+  if (!P) return SB_ERR_INVALID;
+  (void)hipSetDevice(P->ctx->device);
+  delete P;
+  return SB_OK;
+}
+
+int sb_problem_sizes(const sb_problem* P, int* nc, int* nb, int64_t* hblocks, int64_t* wblocks) {
+  if (!P) return SB_ERR_INVALID;
+  if (nc) *nc = P->nc;
+  if (nb) *nb = P->nb;
+  if (hblocks) *hblocks = (int64_t)P->hci.size();
+  if (wblocks) *wblocks = P->hasW ? (int64_t)P->wci.size() : 0;
+  return SB_OK;
+}
+
+int sb_problem_get_w(sb_ctx* ctx, const sb_problem* P, int32_t* wrp, int32_t* wci, double* wv) {
+  (void)ctx;
+  if (!P || !P->hasW) return SB_ERR_INVALID;
+  if (wrp) std::memcpy(wrp, P->wrp.data(), sizeof(int32_t) * P->wrp.size());
+  if (wci) std::memcpy(wci, P->wci.data(), sizeof(int32_t) * P->wci.size());
+  if (wv) std::memcpy(wv, P->wv.data(), sizeof(double) * P->wv.size());
+  return SB_OK;
+}
+
+int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!P || !perm) return SB_ERR_INVALID;
+  const int n = P->nc;
+  std::vector<int> inv(n, -1);
+  for (int i = 0; i < n; ++i) {
+    if (perm[i] < 0 || perm[i] >= n || inv[perm[i]] >= 0) return SB_ERR_INVALID;
+    inv[perm[i]] = i;
+  }
+  // H rows, b, mu: new i <- old perm[i]
+  std::vector<int> rp(n + 1, 0), ci;
+  std::vector<double> hv, b(3 * (size_t)n), mu(n);
+  for (int i = 0; i < n; ++i) rp[i + 1] = rp[i] + (P->hrp[perm[i] + 1] - P->hrp[perm[i]]);
+  ci.resize(rp[n]);
+  hv.resize((size_t)rp[n] * 18);
+  for (int i = 0; i < n; ++i) {
+    const int src = (int)perm[i];
+    int dst = rp[i];
+    for (int p = P->hrp[src]; p < P->hrp[src + 1]; ++p, ++dst) {
+      ci[dst] = P->hci[p];
+      std::memcpy(&hv[(size_t)dst * 18], &P->hv[(size_t)p * 18], 18 * sizeof(double));
+    }
+    for (int k = 0; k < 3; ++k) b[3 * i + k] = P->b[3 * src + k];
+    mu[i] = P->mu[src];
+  }
+  P->hrp.swap(rp);
+  P->hci.swap(ci);
+  P->hv.swap(hv);
+  P->b.swap(b);
+  P->mu.swap(mu);
+  if (P->hasW) {
+    // block (i, j) of the new W is old (perm[i], perm[j]); blocks landing above the
+    // diagonal are stored transposed (SparseBlockMatrixBase::applyPermutation)
+    std::vector<std::vector<std::pair<int, int>>> rows(n);
+    for (int oi = 0; oi < n; ++oi)
+      for (int p = P->wrp[oi]; p < P->wrp[oi + 1]; ++p) {
+        int ni = inv[oi], nj = inv[P->wci[p]];
+        int tag = p + 1;
+        if (nj > ni) {
+          std::swap(ni, nj);
+          tag = -(p + 1);
+        }
+        rows[ni].push_back({nj, tag});
+      }
+    std::vector<int> wrp(n + 1, 0), wci;
+    std::vector<double> wv;
+    for (int i = 0; i < n; ++i) {
+      std::sort(rows[i].begin(), rows[i].end());
+      wrp[i + 1] = wrp[i] + (int)rows[i].size();
+    }
+    wci.resize(wrp[n]);
+    wv.resize((size_t)wrp[n] * 9);
+    for (int i = 0; i < n; ++i) {
+      int dst = wrp[i];
+      for (auto& e : rows[i]) {
+        wci[dst] = e.first;
+        const double* src = &P->wv[(size_t)(std::abs(e.second) - 1) * 9];
+        double* d = &wv[(size_t)dst * 9];
+        if (e.second > 0)
+          std::memcpy(d, src, 9 * sizeof(double));
+        else
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) d[3 * r + c] = src[3 * c + r];
+        ++dst;
+      }
+    }
+    P->wrp.swap(wrp);
+    P->wci.swap(wci);
+    P->wv.swap(wv);
+  }
+  return upload_problem(P);
+}
+
+int sb_problem_coloring(sb_ctx* ctx, const sb_problem* P, int mode, int64_t* perm, int64_t* colors, int* ncolors) {
+  (void)ctx;
+  if (!P || !P->hasW || !perm || !colors || !ncolors || (mode != 0 && mode != 1)) return SB_ERR_INVALID;
+  std::vector<int64_t> pm, cl;
+  sbhost::coloring(P->nc, P->wrp.data(), P->wci.data(), mode, pm, cl);
+  std::memcpy(perm, pm.data(), sizeof(int64_t) * pm.size());
+  std::memcpy(colors, cl.data(), sizeof(int64_t) * cl.size());
+  *ncolors = (int)cl.size() - 1;
+  return SB_OK;
+}
+
+int sb_problem_apply_w(sb_ctx* ctx, const sb_problem* Pc, int op, const double* x, double* y, int on_device) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !x || !y) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  const size_t n3 = 3 * (size_t)P->nc;
+  hipStream_t s = ctx->stream;
+  const double* dx = x;
+  double* dy = y;
+  if (!on_device) {
+    if ((rc = copy_in(ctx, P->rio.p, x, n3, 0))) return rc;
+    dx = P->rio.p;
+    dy = P->y.p;
+  }
+  const DevProblem<double> V = P->view(true);
+  if (op == SB_OP_ASSEMBLED)
+    SB_HIP(sb::launch_w(V, dx, dy, P->carry.p, s));
+  else
+    SB_HIP(sb::launch_matfree(V, dx, dy, P->wbody.p, P->carry.p, s));
+  if (!on_device && (rc = copy_out(ctx, y, dy, n3, 0))) return rc;
+  SB_HIP(hipStreamSynchronize(s));
+  return SB_OK;
+}
+
+int sb_fb_residual(sb_ctx* ctx, const sb_problem* Pc, int law, int op, int scaling, int inf_norm, const double* r,
+                   double* residual, int on_device) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !r || !residual) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  const size_t n3 = 3 * (size_t)P->nc;
+  hipStream_t s = ctx->stream;
+  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
+  const DevProblem<double> V = P->view(true);
+  if (op == SB_OP_ASSEMBLED)
+    SB_HIP(sb::launch_w(V, P->rio.p, P->y.p, P->carry.p, s));
+  else
+    SB_HIP(sb::launch_matfree(V, P->rio.p, P->y.p, P->wbody.p, P->carry.p, s));
+  const bool useScal = (scaling == SB_SCALING_DIAGONAL) || (scaling == SB_SCALING_REFERENCE && op == SB_OP_ASSEMBLED);
+  SB_HIP(sb::launch_fb_terms(P->nc, law == SB_LAW_COULOMB, P->d_mu.p, useScal ? P->d_scal.p : nullptr, P->y.p, P->d_b.p,
+                             P->rio.p, P->terms.p, s));
+  std::vector<double> t(P->nc);
+  SB_HIP(hipMemcpyAsync(t.data(), P->terms.p, sizeof(double) * P->nc, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  double acc = 0.0;
+  if (inf_norm) {
+    for (double v : t) acc = std::max(acc, v);
+  } else {
+    for (double v : t) acc += v;
+    acc /= (1.0 + P->nc);
+  }
+  *residual = acc;
+  return std::isfinite(acc) ? SB_OK : SB_ERR_NAN;
+}
+
+int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, double* r, int on_device,
+                double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (o->variant != SB_PG_APGD && o->variant != SB_PG_DESCENT && o->variant != SB_PG_SPG) return SB_ERR_UNSUPPORTED;
+  if (o->op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  if (o->max_iters < 1 || o->ls_iters < 1 || !(o->tol >= 0)) return SB_ERR_INVALID;
+  if (!ctx->coop) return SB_ERR_UNSUPPORTED;
+  hipStream_t s = ctx->stream;
+  const size_t n3 = 3 * (size_t)P->nc;
+  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
+  const int histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
+  if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
+
+  sb::PGArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nc = P->nc;
+  a.nb = P->nb;
+  a.variant = o->variant;
+  a.coulomb = o->law == SB_LAW_COULOMB;
+  a.inf_norm = o->inf_norm;
+  const bool mf = (o->op == SB_OP_MATRIX_FREE);
+  const bool useScal = (o->scaling == SB_SCALING_DIAGONAL) || (o->scaling == SB_SCALING_REFERENCE && !mf);
+  a.use_scaling = useScal ? 1 : 0;
+  a.tol = o->tol;
+  a.max_iters = o->max_iters;
+  a.ls_iters = o->ls_iters;
+  a.ls_opt = o->ls_optimistic;
+  a.ls_pess = o->ls_pessimistic;
+  a.P = P->view(true);
+  for (int k = 0; k < 3; ++k) {
+    a.X[k] = P->X[k].p;
+    a.MX[k] = P->MX[k].p;
+  }
+  a.G[0] = P->G[0].p;
+  a.G[1] = P->G[1].p;
+  a.xbest = P->xbest.p;
+  a.wbody = P->wbody.p;
+  a.carry = P->carry.p;
+  a.x_io = P->rio.p;
+  a.hist = P->hist.p;
+  a.hist_cap = histCap;
+  a.out = ctx->out.p;
+  const void* fn = mf ? (const void*)&sb::pg_kernel<true> : (const void*)&sb::pg_kernel<false>;
+  const int workWaves = mf ? std::max(P->H.host.ntiles, P->HT.host.ntiles) : P->W.host.ntiles;
+  const int grid = grid_for(ctx, fn, o->grid_blocks, workWaves);
+  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
+  a.gs = make_sync(ctx, grid);
+  SB_HIP(sb::launch_reset_sync(ctx->count.p, ctx->gen.p, ctx->error.p, s));
+  void* args[] = {&a};
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kThreads), args, 0, s));
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  double out[8];
+  if ((rc = read_status(ctx, out, 8))) return rc;
+  if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
+  int hl = std::min((int)out[1] + 1, histCap);
+  if (st) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    st->iterations = (int)out[1];
+    st->spmvs = (int)out[2];
+    st->ls_trials = (int)out[3];
+    st->nesterov_steps = (int)out[4];
+    st->converged = (int)out[5];
+    st->local_failures = 0;
+    st->residual = out[0];
+    st->device_ms = ms;
+    st->history_length = (int)out[1] + ((int)out[5] ? 1 : 0);
+    if (histCap > 0) {
+      hl = std::min(st->history_length, histCap);
+      if (hl > 0) SB_HIP(hipMemcpyAsync(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost, s));
+    }
+  }
+  SB_HIP(hipStreamSynchronize(s));
+  if (residual) *residual = out[0];
+  if (out[6] != 0.0) return SB_ERR_BARRIER_TIMEOUT;
+  if (!std::isfinite(out[0])) return SB_ERR_NAN;
+  return out[5] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
+}
+
+int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const int64_t* colors, int ncolors,
+                double* r, int on_device, double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (!P->hasW) return SB_ERR_INVALID;
+  if (o->max_iters < 0 || o->eval_every < 1 || o->strategy < 0 || o->strategy > 3) return SB_ERR_INVALID;
+  if (!ctx->coop) return SB_ERR_UNSUPPORTED;
+  const int n = P->nc;
+  std::vector<int> col;
+  if (colors && ncolors > 0) {
+    if (colors[0] != 0 || colors[ncolors] != n) return SB_ERR_INVALID;
+    col.resize(ncolors + 1);
+    for (int c = 0; c <= ncolors; ++c) {
+      if (c > 0 && colors[c] < colors[c - 1]) return SB_ERR_INVALID;
+      col[c] = (int)colors[c];
+    }
+  } else {
+    col = {0, n};
+  }
+  hipStream_t s = ctx->stream;
+  // local matrices with optional auto-regularization (GaussSeidelBase::processLocalMatrices)
+  std::vector<double> local(P->diag), reg(n, 0.0);
+  if (o->auto_regularization > 0.0)
+    for (int i = 0; i < n; ++i) {
+      reg[i] = std::max(0.0, o->auto_regularization - min_eig_sym3(&local[(size_t)9 * i]));
+      local[(size_t)9 * i] += reg[i];
+      local[(size_t)9 * i + 4] += reg[i];
+      local[(size_t)9 * i + 8] += reg[i];
+    }
+  SB_HIP(P->local.upload(local, s));
+  SB_HIP(P->reg.upload(reg, s));
+  SB_HIP(P->colors.upload(col, s));
+  SB_HIP(P->skip.alloc(n));
+  SB_HIP(hipMemsetAsync(P->skip.p, 0, n, s));
+  const size_t n3 = 3 * (size_t)n;
+  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
+  const int histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
+  if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
+
+  sb::GSArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nc = n;
+  a.coulomb = o->law == SB_LAW_COULOMB;
+  a.strategy = o->strategy;
+  a.inf_norm = o->inf_norm;
+  a.try_zero = o->try_zero;
+  a.tol = o->tol;
+  a.max_iters = o->max_iters;
+  a.eval_every = o->eval_every;
+  a.skip_tol = o->skip_tol;
+  a.skip_iters = std::min(o->skip_iters, (int)std::sqrt((double)n));
+  a.local_tol = std::pow(2.220446049250313e-16, 0.75);  // SOCLaw::m_localTol
+  a.P = P->view(true);
+  a.local = P->local.p;
+  a.reg = P->reg.p;
+  a.colors = P->colors.p;
+  a.ncolors = (int)col.size() - 1;
+  a.x = P->rio.p;
+  a.y = P->y.p;
+  a.xbest = P->xbest.p;
+  a.skip = P->skip.p;
+  a.carry = P->carry.p;
+  a.hist = P->hist.p;
+  a.hist_cap = histCap;
+  a.out = ctx->out.p;
+  const void* fn = (const void*)&sb::gs_kernel;
+  int maxColor = 0;
+  for (size_t c = 0; c + 1 < col.size(); ++c) maxColor = std::max(maxColor, col[c + 1] - col[c]);
+  const int grid = grid_for(ctx, fn, o->grid_blocks, std::max(P->W.host.ntiles, (maxColor + 63) / 64));
+  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
+  a.gs = make_sync(ctx, grid);
+  SB_HIP(sb::launch_reset_sync(ctx->count.p, ctx->gen.p, ctx->error.p, s));
+  const int* wrowptr = P->W.rowptr.p;
+  void* args[] = {&a, &wrowptr};
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kThreads), args, 0, s));
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  double out[8];
+  if ((rc = read_status(ctx, out, 8))) return rc;
+  if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
+  if (st) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    st->iterations = (int)out[1];
+    st->spmvs = (int)out[4];
+    st->ls_trials = 0;
+    st->nesterov_steps = 0;
+    st->converged = (int)out[2];
+    st->local_failures = (int)out[3];
+    st->residual = out[0];
+    st->device_ms = ms;
+    st->history_length = (int)out[4];
+    if (histCap > 0) {
+      const int hl = std::min(st->history_length, histCap);
+      if (hl > 0) SB_HIP(hipMemcpyAsync(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost, s));
+    }
+  }
+  SB_HIP(hipStreamSynchronize(s));
+  if (residual) *residual = out[0];
+  if (out[5] != 0.0) return SB_ERR_BARRIER_TIMEOUT;
+  if (!std::isfinite(out[0])) return SB_ERR_NAN;
+  return out[2] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
+}
+
+// ------------------------------------------------------------------ memory helpers
+int sb_device_alloc(sb_ctx* ctx, void** ptr, size_t bytes) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!ptr) return SB_ERR_INVALID;
+  SB_HIP(hipMalloc(ptr, bytes));
+  return SB_OK;
+}
+int sb_device_free(sb_ctx* ctx, void* ptr) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  SB_HIP(hipFree(ptr));
+  return SB_OK;
+}
+int sb_memcpy_h2d(sb_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  SB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+int sb_memcpy_d2h(sb_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  SB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+int sb_synchronize(sb_ctx* ctx) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+int sb_bench_spmv(sb_ctx* ctx, const sb_problem* Pc, int kind, int reps, double* ms) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !ms || reps < 1 || kind < 0 || kind > 3) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (kind == 0 && !P->hasW) return SB_ERR_INVALID;
+  hipStream_t s = ctx->stream;
+  const DevProblem<double> V = P->view(true);
+  // deterministic non-trivial inputs
+  {
+    std::vector<double> x(3 * (size_t)P->nc), w(6 * (size_t)P->nb);
+    for (size_t k = 0; k < x.size(); ++k) x[k] = std::sin(0.37 * (double)k);
+    for (size_t k = 0; k < w.size(); ++k) w[k] = std::cos(0.11 * (double)k);
+    SB_HIP(hipMemcpyAsync(P->X[0].p, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, s));
+    SB_HIP(hipMemcpyAsync(P->wbody.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s));
+  }
+  auto launch = [&]() -> hipError_t {
+    switch (kind) {
+      case 0: return sb::launch_w(V, P->X[0].p, P->y.p, P->carry.p, s);
+      case 1: return sb::launch_h(V, P->wbody.p, P->y.p, P->carry.p, s);
+      case 2: return sb::launch_ht_minv(V, P->X[0].p, P->wbody.p, P->carry.p, s);
+      default: return sb::launch_matfree(V, P->X[0].p, P->y.p, P->wbody.p, P->carry.p, s);
+    }
+  };
+  SB_HIP(launch());  // warm-up
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  for (int k = 0; k < reps; ++k) SB_HIP(launch());
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  SB_HIP(hipEventSynchronize(ctx->ev1));
+  float t = 0.f;
+  SB_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  *ms = (double)t / reps;
+  return SB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// Persistent contact-colored projected block Gauss-Seidel.
+//
+// GaussSeidel::solveWithLinearConstraints / innerLoop
+// (src/Core/BlockSolvers/GaussSeidel.impl.hpp:62-124, 144-205) with
+// GaussSeidelBase::evalAndKeepBest / tryZero (GaussSeidelBase.impl.hpp:44-77).
+// Contacts are permuted so that every color is a contiguous range; contacts of
+// one color do not share a body, so they are solved in parallel, one lane per
+// contact: lb = b_i - reg_i x_i + sum_{j != i} W_ij x_j, then the FP64 local
+// solver, the damped update and the sleep heuristic. A grid barrier separates
+// colors. The reference step-size reference ndxRef is a racy running maximum;
+// here each color reads the value left by the previous colors and the maxima
+// of the color are merged at the color's barrier (deterministic).
+#include <hip/hip_runtime.h>
+
+#include "sb_law.hpp"
+#include "sb_local.hpp"
+#include "sb_problem.hpp"
+
+namespace sb {
+
+namespace {
+__device__ __forceinline__ void ld3(const double* p, int i, double v[3]) {
+  v[0] = p[3 * i];
+  v[1] = p[3 * i + 1];
+  v[2] = p[3 * i + 2];
+}
+}  // namespace
+
+struct GSRow {
+  const int* rowptr;  // W expanded row pointers
+};
+
+__global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict__ wrowptr) {
+  const int wpb = blockDim.x / SB_WAVE;
+  const int gw = blockIdx.x * wpb + wave_in_block();
+  const int nwaves = gridDim.x * wpb;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  const int n = a.nc, n3 = 3 * n;
+  const bool coulomb = a.coulomb != 0;
+  const unsigned resMax = a.inf_norm ? 1u : 0u;
+  const TileMat<double>& W = a.P.W;
+  const size_t nnz = (size_t)W.nblocks;
+
+  // y = W x + b ; err = eval(y, x) [and err0 = eval(b, 0) when asked]
+  auto eval_phase = [&](bool withZero, double& err, double& err0) -> bool {
+    double part[2] = {0.0, 0.0};
+    auto ld = [&](int j, double xc[3]) { ld3(a.x, j, xc); };
+    auto lead = [&](int i, double v[3]) {
+      double b[3], xi[3];
+      ld3(a.P.b, i, b);
+      ld3(a.x, i, xi);
+      const double s = a.P.scal[i], mu = a.P.mu[i];
+      const double y[3] = {v[0] + b[0], v[1] + b[1], v[2] + b[2]};
+      const double xs[3] = {s * xi[0], s * xi[1], s * xi[2]};
+      const double r = law_eval<double>(coulomb, mu, xs, y);
+      part[0] = resMax ? fmax(part[0], r) : part[0] + r;
+      if (withZero) {
+        const double z[3] = {0.0, 0.0, 0.0};
+        const double r0 = law_eval<double>(coulomb, mu, z, b);
+        part[1] = resMax ? fmax(part[1], r0) : part[1] + r0;
+      }
+    };
+    for (int t = gw; t < W.ntiles; t += nwaves) tile_gather<3, 3, double, double>(W, t, ld, lead, a.carry);
+    if (W.nlong > 0) {
+      if (!grid_reduce_sync(a.gs, 0, 0)) return false;
+      for (int k = gw; k < W.nlong; k += nwaves) {
+        double v[3];
+        long_row_sum<3, double, double>(W, k, a.carry, v);
+        if (lane_id() == 0) lead(W.long_row[k], v);
+      }
+    }
+    block_partials<2>(a.gs, part, resMax | (resMax << 1));
+    if (!grid_reduce_sync(a.gs, 2, resMax | (resMax << 1))) return false;
+    err = read_reduced(a.gs, 0);
+    err0 = read_reduced(a.gs, 1);
+    if (!a.inf_norm) {
+      err /= (1.0 + n);
+      err0 /= (1.0 + n);
+    }
+    return true;
+  };
+
+  double errBest = 1.7976931348623157e308;
+  double err = 0.0, err0 = 0.0;
+  bool ok = eval_phase(a.try_zero != 0, err, err0);
+  if (ok) {
+    if (err < errBest) {
+      errBest = err;
+      for (int k = tid; k < n3; k += nthreads) a.xbest[k] = a.x[k];
+    }
+    if (a.try_zero && err0 < errBest) {
+      errBest = err0;
+      for (int k = tid; k < n3; k += nthreads) {
+        a.xbest[k] = 0.0;
+        a.x[k] = 0.0;
+      }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.hist_cap > 0) a.hist[0] = errBest;
+    // x/xbest writes above are consumed by other blocks in the color phases
+    ok = grid_reduce_sync(a.gs, 0, 0);
+  }
+
+  const double absSkipTol = fmin(a.skip_tol, a.tol);
+  double ndxRef = 0.0;
+  int it = 1, nfail = 0, nEval = 1;
+  bool converged = false;
+  for (; ok && it <= a.max_iters; ++it) {
+    for (int col = 0; col < a.ncolors && ok; ++col) {
+      const int c0 = a.colors[col], c1 = a.colors[col + 1];
+      double part[1] = {0.0};
+      for (int i = c0 + tid; i < c1; i += nthreads) {
+        if (a.skip[i]) {
+          a.skip[i] = (unsigned char)(a.skip[i] - 1);
+          continue;
+        }
+        double lx[3], lb[3];
+        ld3(a.x, i, lx);
+        const double reg = a.reg[i];
+        ld3(a.P.b, i, lb);
+        lb[0] -= reg * lx[0];
+        lb[1] -= reg * lx[1];
+        lb[2] -= reg * lx[2];
+        for (int q = wrowptr[i]; q < wrowptr[i + 1]; ++q) {
+          const int j = W.bcol[q];
+          if (j == i) continue;
+          double xj[3];
+          ld3(a.x, j, xj);
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+            lb[r] += W.vals[(size_t)(3 * r + 0) * nnz + q] * xj[0] + W.vals[(size_t)(3 * r + 1) * nnz + q] * xj[1] +
+                     W.vals[(size_t)(3 * r + 2) * nnz + q] * xj[2];
+        }
+        double A[3][3];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) A[e / 3][e % 3] = a.local[(size_t)9 * i + e];
+        const double ldx0[3] = {-lx[0], -lx[1], -lx[2]};
+        const double s = a.P.scal[i];
+        const double lres = local::solve(coulomb, a.strategy, A, lb, lx, a.P.mu[i], a.local_tol, s);
+        const bool lok = a.local_tol > lres;
+        double ldx[3] = {ldx0[0] + lx[0], ldx0[1] + lx[1], ldx0[2] + lx[2]};
+        if (!lok) {
+          ldx[0] *= 0.5;
+          ldx[1] *= 0.5;
+          ldx[2] *= 0.5;
+          ++nfail;
+        }
+        a.x[3 * i] += ldx[0];
+        a.x[3 * i + 1] += ldx[1];
+        a.x[3 * i + 2] += ldx[2];
+        const double s2 = s * s;
+        const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
+        const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
+        part[0] = fmax(part[0], ndx2);
+        if (fmin(nx2, ndx2) < absSkipTol || ndx2 < a.skip_tol * fmin(nx2, ndxRef))
+          a.skip[i] = (unsigned char)a.skip_iters;
+      }
+      block_partials<1>(a.gs, part, 1u);
+      ok = grid_reduce_sync(a.gs, 1, 1u);
+      if (ok) ndxRef = fmax(ndxRef, read_reduced(a.gs, 0));
+    }
+    if (!ok) break;
+    if (it % a.eval_every == 0) {
+      double e0;
+      ok = eval_phase(false, err, e0);
+      if (!ok) break;
+      if (blockIdx.x == 0 && threadIdx.x == 0 && nEval < a.hist_cap) a.hist[nEval] = err;
+      ++nEval;
+      if (err < errBest) {
+        errBest = err;
+        for (int k = tid; k < n3; k += nthreads) a.xbest[k] = a.x[k];
+      }
+      if (err < a.tol) {
+        converged = true;
+        break;
+      }
+      ndxRef /= a.eval_every;
+      // xbest must be complete before the next sweep modifies x
+      ok = grid_reduce_sync(a.gs, 0, 0);
+    }
+  }
+  // maxIters exhausted: x = x_best (GaussSeidel.impl.hpp:202)
+  if (ok && !converged)
+    for (int k = tid; k < n3; k += nthreads) a.x[k] = a.xbest[k];
+  // count local failures over the grid
+  {
+    double part[1] = {(double)nfail};
+    block_partials<1>(a.gs, part, 0u);
+    const bool ok2 = grid_reduce_sync(a.gs, 1, 0u);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.out[0] = errBest;
+      a.out[1] = (double)(it > a.max_iters ? a.max_iters : it);
+      a.out[2] = converged ? 1.0 : 0.0;
+      a.out[3] = ok2 ? read_reduced(a.gs, 0) : -1.0;
+      a.out[4] = (double)nEval;
+      a.out[5] = (ok && ok2) ? 0.0 : 1.0;
+    }
+  }
+}
+
+}  // namespace sb

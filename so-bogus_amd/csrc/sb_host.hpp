@@ -1,0 +1,45 @@
+// Host-side setup of the device layout: tiling of block-sparse matrices,
+// assembly of W = H Minv H^T, coloring and permutations. Plain C++ (no HIP).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace sbhost {
+
+// Tiled lane-per-block layout (see sb_tiles.hpp). Values are element-major.
+struct Tiles {
+  int nrows = 0, ncols = 0, br = 0, bc = 0;
+  int nblocks = 0, ntiles = 0, scan_steps = 0;
+  std::vector<int> tstart, brow, bcol, tlong, long_row, long_t0, long_t1;
+  std::vector<int> rowptr;   // per-row block offsets in tile order
+  std::vector<double> vals;  // [br*bc][nblocks]
+};
+
+// Builds tiles from a row-major BSR (blocks in row order, vals row-major per
+// block). Empty rows receive one zero block so that every row has a leader.
+Tiles buildTiles(int nrows, int ncols, int br, int bc, const int* rowptr, const int* colidx, const double* vals,
+                 bool transposeBlocks = false);
+
+// Transposes a BSR (blocks become bc x br, rows <-> cols).
+void transposeBSR(int nrows, int ncols, int br, int bc, const int* rowptr, const int* colidx, const double* vals,
+                  std::vector<int>& trowptr, std::vector<int>& tcolidx, std::vector<double>& tvals);
+
+// Expands a symmetric lower-triangle BSR of square blocks into both triangles.
+void expandSymmetric(int n, int bs, const int* rowptr, const int* colidx, const double* vals, std::vector<int>& frowptr,
+                     std::vector<int>& fcolidx, std::vector<double>& fvals);
+
+// W = H Minv H^T, lower triangle, 3x3 blocks (DualFrictionProblem::computeFrom).
+void assembleW(int nc, int nb, const int* hrowptr, const int* hcolidx, const double* hvals, const double* minv,
+               std::vector<int>& wrowptr, std::vector<int>& wcolidx, std::vector<double>& wvals);
+
+// Diagonal blocks W_ii = sum_k H_ik Minv_k H_ik^T.
+void diagBlocks(int nc, const int* hrowptr, const int* hcolidx, const double* hvals, const double* minv,
+                std::vector<double>& diag);
+
+// Coloring of a symmetric lower-triangle block pattern.
+//  mode 0: Coloring::compute as written (conflict when lower_bound(color, j) != end);
+//  mode 1: first fit with an exact membership test.
+void coloring(int n, const int* wrowptr, const int* wcolidx, int mode, std::vector<int64_t>& perm,
+              std::vector<int64_t>& colors);
+
+}  // namespace sbhost

@@ -1,0 +1,98 @@
+// Per-contact second-order-cone / Coulomb law kernels (device, FP64 or FP32).
+//
+//  * project  : SOCLaw::projectOnConstraint      (src/Extra/SOC/SOCLaw.impl.hpp:46-72)
+//  * cov      : SOCLaw::dualityCOV (De Saxce)    (src/Extra/SOC/SOCLaw.hpp:92-99)
+//  * law_eval : SOCLaw::eval = |FB_mu(x, y [+ mu|y_T| e_N])|^2
+//               (src/Extra/SOC/SOCLaw.hpp:55-64, src/Extra/SOC/FischerBurmeister.impl.hpp:29-42, 69-101, 142-152)
+//
+// Vectors are (normal, tangent1, tangent2). mu < 0 disables the contact.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sb {
+
+template <typename T>
+struct LawConst;
+template <>
+struct LawConst<double> {
+  static constexpr double eps = 2.220446049250313e-16;  // DBL_EPSILON: NumTraits<double>::isZero threshold
+};
+template <>
+struct LawConst<float> {
+  static constexpr float eps = 1.1920928955078125e-07f;
+};
+
+template <typename T>
+__device__ __forceinline__ bool is_zero(T s) {
+  return fabs(s) < LawConst<T>::eps;
+}
+
+// Projection on K_mu = { |x_T| <= mu x_N }.
+template <typename T>
+__device__ __forceinline__ void project(T mu, T& x0, T& x1, T& x2) {
+  const T nxt = sqrt(x1 * x1 + x2 * x2);
+  if (mu < T(0)) {
+    x0 = x1 = x2 = T(0);
+    return;
+  }
+  if (nxt <= mu * x0) return;  // inside the cone
+  if (mu * nxt <= -x0) {       // inside the polar cone
+    x0 = x1 = x2 = T(0);
+    return;
+  }
+  const T proj = (x0 + mu * nxt) / (T(1) + mu * mu);
+  const T f = proj * mu / nxt;
+  x0 = proj;
+  x1 *= f;
+  x2 *= f;
+}
+
+// De Saxce change of variable s(y) = (mu |y_T|, 0, 0) (Coulomb) or 0 (SOC).
+template <typename T>
+__device__ __forceinline__ T cov_normal(bool coulomb, T mu, T y1, T y2) {
+  return coulomb ? mu * sqrt(y1 * y1 + y2 * y2) : T(0);
+}
+
+// Fischer-Burmeister on the canonical cone via the spectral decomposition of
+// z2 = x o x + y o y (Fukushima 2002; [Daviet et al 2011] App. A.1), returning |fb|^2.
+template <typename T>
+__device__ __forceinline__ T fb_canonical_sq(const T x[3], const T y[3]) {
+  const T z20 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+  const T z21 = x[0] * x[1] + y[0] * y[1];
+  const T z22 = x[0] * x[2] + y[0] * y[2];
+  const T nz2t = sqrt(z21 * z21 + z22 * z22);
+  T d1, d2;  // unit tangent direction of z2
+  if (is_zero(nz2t)) {
+    d1 = T(1);
+    d2 = T(0);
+  } else {
+    d1 = z21 / nz2t;
+    d2 = z22 / nz2t;
+  }
+  const T rl1 = sqrt(fmax(T(0), z20 - T(2) * nz2t));
+  const T rl2 = sqrt(z20 + T(2) * nz2t);
+  // z = rl1 * (1/2, -d/2) + rl2 * (1/2, d/2)
+  const T z0 = T(0.5) * (rl1 + rl2);
+  const T zt = T(0.5) * (rl2 - rl1);
+  const T f0 = x[0] + y[0] - z0;
+  const T f1 = x[1] + y[1] - zt * d1;
+  const T f2 = x[2] + y[2] - zt * d2;
+  return f0 * f0 + f1 * f1 + f2 * f2;
+}
+
+// SOCLaw<3,T,coulomb>::eval(x, y): x already scaled by the row scaling.
+template <typename T>
+__device__ __forceinline__ T law_eval(bool coulomb, T mu, const T x[3], const T y[3]) {
+  if (mu < T(0)) return y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+  T yt0 = y[0];
+  if (coulomb) yt0 += mu * sqrt(y[1] * y[1] + y[2] * y[2]);
+  if (is_zero(mu)) {
+    const T f0 = x[0] + yt0 - sqrt(x[0] * x[0] + yt0 * yt0);
+    return f0 * f0 + x[1] * x[1] + x[2] * x[2];
+  }
+  const T xh[3] = {mu * x[0], x[1], x[2]};
+  const T yh[3] = {yt0, mu * y[1], mu * y[2]};
+  return fb_canonical_sq(xh, yh);
+}
+
+}  // namespace sb

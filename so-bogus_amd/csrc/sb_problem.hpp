@@ -1,0 +1,82 @@
+// Device-resident problem layout and solver argument blocks (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sb_device.hpp"
+#include "sb_tiles.hpp"
+
+namespace sb {
+
+// A dual friction problem u = W r + b resident in HBM.
+//   H  : contact-major tiles, 3x6 blocks (H_ik)
+//   HT : body-major tiles, 6x3 blocks (H_ik^T), for H^T r
+//   W  : assembled Delassus operator, both triangles expanded, 3x3 blocks (optional)
+//   minv : inverse mass blocks, element-major [36][nb]
+//   diag : W_ii blocks [nc][9]; scal : max(1, tr(W_ii)/3)
+template <typename T>
+struct DevProblem {
+  int nc = 0, nb = 0;
+  TileMat<T> H, HT, W;
+  const T* minv = nullptr;
+  const double* b = nullptr;
+  const double* mu = nullptr;
+  const double* diag = nullptr;
+  const double* scal = nullptr;
+  bool hasW = false;
+};
+
+struct PGArgs {
+  int nc, nb;
+  int variant;   // 1 Descent, 3 APGD, 4 SPG
+  int coulomb;
+  int inf_norm;
+  int use_scaling;  // 1: residual row scaling from scal[], 0: all ones
+  double tol;
+  int max_iters;
+  int ls_iters;
+  double ls_opt, ls_pess;
+  DevProblem<double> P;
+  double* X[3];
+  double* MX[3];
+  double* G[2];
+  double* xbest;
+  double* wbody;  // [6 nb]
+  double* carry;
+  double* x_io;   // in: warm start, out: result
+  double* hist;
+  int hist_cap;
+  GridSync gs;
+  double* out;    // [0] residual [1] iterations [2] spmvs [3] ls trials [4] nesterov steps [5] converged
+};
+
+struct GSArgs {
+  int nc;
+  int coulomb;
+  int strategy;
+  int inf_norm;
+  int try_zero;
+  double tol;
+  int max_iters;
+  int eval_every;
+  double skip_tol;
+  int skip_iters;       // already min(skipIters, sqrt(n))
+  double local_tol;
+  DevProblem<double> P;
+  const double* local;  // [nc][9] local matrices (regularized diagonal blocks)
+  const double* reg;    // [nc]
+  const int* colors;    // [ncolors + 1] offsets in the (color-contiguous) contact order
+  int ncolors;
+  double* x;            // in/out
+  double* y;            // scratch [3 nc]
+  double* xbest;
+  unsigned char* skip;
+  double* ndx;          // [2] ndxRef of the previous color / running max of the current color
+  double* carry;
+  double* hist;
+  int hist_cap;
+  GridSync gs;
+  double* out;          // [0] residual [1] iterations [2] converged [3] local failures
+};
+
+}  // namespace sb

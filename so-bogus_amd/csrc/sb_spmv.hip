@@ -1,0 +1,206 @@
+// Stand-alone block SpMV kernels on the tiled layout (one wave per tile):
+//   * y = alpha op(A) x + beta y for a generic BSR (SparseBlockMatrixBase::multiply,
+//     src/Core/Block/SparseMatrixVectorProduct.impl.hpp:382-432), block sizes <= 8;
+//   * the friction operators: W x (assembled), H w, Minv H^T r, and the matrix-free
+//     W x = H (Minv (H^T x)) (ProductVector::multiply, :355-376).
+// Symmetric matrices are stored with both triangles expanded and transposed
+// products use a cached transposed copy, so every product is a pure gather: no
+// atomics, deterministic results.
+#include <hip/hip_runtime.h>
+
+#include "sb_law.hpp"
+#include "sb_problem.hpp"
+#include "sb_spmv.hpp"
+
+namespace sb {
+
+// ------------------------------------------------------------------ generic BSR
+template <int MAXB>
+__global__ void __launch_bounds__(256) k_bsr_generic(TileMat<double> A, int br, int bc, double alpha,
+                                                     const double* __restrict__ x, double beta, double* y,
+                                                     double* carry) {
+  const int wpb = blockDim.x / SB_WAVE;
+  const int t = blockIdx.x * wpb + wave_in_block();
+  if (t >= A.ntiles) return;
+  const int l = lane_id();
+  const int q0 = A.tstart[t], q1 = A.tstart[t + 1];
+  const int q = q0 + l;
+  const bool act = q < q1;
+  int row = -1;
+  double v[MAXB];
+  for (int r = 0; r < MAXB; ++r) v[r] = 0.0;
+  if (act) {
+    row = A.brow[q];
+    const int col = A.bcol[q];
+    for (int r = 0; r < br; ++r) {
+      double acc = 0.0;
+      for (int c = 0; c < bc; ++c) acc += A.vals[(size_t)(r * bc + c) * A.nblocks + q] * x[(size_t)col * bc + c];
+      v[r] = acc;
+    }
+  }
+  for (int s = 0; s < A.scan_steps; ++s) {
+    const int d = 1 << s;
+    const int ru = __shfl_up(row, d, SB_WAVE);
+    for (int r = 0; r < MAXB; ++r) {
+      const double vu = __shfl_up(v[r], d, SB_WAVE);
+      if (l >= d && ru == row) v[r] += vu;
+    }
+  }
+  const int rn = __shfl_down(row, 1, SB_WAVE);
+  if (act && (l == SB_WAVE - 1 || rn != row)) {
+    if (A.tlong[t] >= 0) {
+      for (int r = 0; r < br; ++r) carry[(size_t)t * MAXB + r] = v[r];
+    } else {
+      for (int r = 0; r < br; ++r) {
+        const size_t o = (size_t)row * br + r;
+        y[o] = alpha * v[r] + (beta == 0.0 ? 0.0 : beta * y[o]);
+      }
+    }
+  }
+}
+
+__global__ void k_bsr_generic_long(TileMat<double> A, int br, double alpha, double beta, double* y,
+                                   const double* carry) {
+  const int wpb = blockDim.x / SB_WAVE;
+  const int k = blockIdx.x * wpb + wave_in_block();
+  if (k >= A.nlong) return;
+  const int t0 = A.long_t0[k], t1 = A.long_t1[k], row = A.long_row[k];
+  for (int r = 0; r < br; ++r) {
+    double acc = 0.0;
+    for (int t = t0 + lane_id(); t < t1; t += SB_WAVE) acc += carry[(size_t)t * 8 + r];
+    acc = wave_sum(acc);
+    if (lane_id() == 0) {
+      const size_t o = (size_t)row * br + r;
+      y[o] = alpha * acc + (beta == 0.0 ? 0.0 : beta * y[o]);
+    }
+  }
+}
+
+hipError_t launch_bsr_generic(const TileMat<double>& A, int br, int bc, double alpha, const double* x, double beta,
+                              double* y, double* carry, hipStream_t s) {
+  const int threads = 256, wpb = threads / SB_WAVE;
+  if (A.ntiles > 0)
+    k_bsr_generic<8><<<(A.ntiles + wpb - 1) / wpb, threads, 0, s>>>(A, br, bc, alpha, x, beta, y, carry);
+  if (A.nlong > 0) k_bsr_generic_long<<<(A.nlong + wpb - 1) / wpb, threads, 0, s>>>(A, br, alpha, beta, y, carry);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ friction operators
+namespace {
+struct VecLoader3 {
+  const double* x;
+  __device__ void operator()(int j, double xc[3]) const {
+    xc[0] = x[3 * j];
+    xc[1] = x[3 * j + 1];
+    xc[2] = x[3 * j + 2];
+  }
+};
+struct VecLoader6 {
+  const double* x;
+  __device__ void operator()(int j, double xc[6]) const {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) xc[r] = x[6 * j + r];
+  }
+};
+struct Store3 {
+  double* y;
+  __device__ void operator()(int i, double v[3]) const {
+    y[3 * i] = v[0];
+    y[3 * i + 1] = v[1];
+    y[3 * i + 2] = v[2];
+  }
+};
+struct StoreMinv {  // w_k = Minv_k v
+  double* w;
+  const double* minv;
+  int nb;
+  __device__ void operator()(int k, double v[6]) const {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc += minv[(size_t)(r * 6 + c) * nb + k] * v[c];
+      w[6 * k + r] = acc;
+    }
+  }
+};
+}  // namespace
+
+template <int BR, int BC, typename LD, typename LE>
+__global__ void __launch_bounds__(256) k_tiles(TileMat<double> A, LD ld, LE le, double* carry) {
+  const int wpb = blockDim.x / SB_WAVE;
+  const int t = blockIdx.x * wpb + wave_in_block();
+  if (t >= A.ntiles) return;
+  tile_gather<BR, BC, double, double>(A, t, ld, le, carry);
+}
+
+template <int BR, typename LE>
+__global__ void k_long(TileMat<double> A, LE le, const double* carry) {
+  const int wpb = blockDim.x / SB_WAVE;
+  const int k = blockIdx.x * wpb + wave_in_block();
+  if (k >= A.nlong) return;
+  double v[BR];
+  long_row_sum<BR, double, double>(A, k, carry, v);
+  if (lane_id() == 0) le(A.long_row[k], v);
+}
+
+template <int BR, int BC, typename LD, typename LE>
+static void run_tiles(const TileMat<double>& A, LD ld, LE le, double* carry, hipStream_t s) {
+  const int threads = 256, wpb = threads / SB_WAVE;
+  if (A.ntiles > 0) k_tiles<BR, BC><<<(A.ntiles + wpb - 1) / wpb, threads, 0, s>>>(A, ld, le, carry);
+  if (A.nlong > 0) k_long<BR><<<(A.nlong + wpb - 1) / wpb, threads, 0, s>>>(A, le, carry);
+}
+
+hipError_t launch_w(const DevProblem<double>& P, const double* x, double* y, double* carry, hipStream_t s) {
+  run_tiles<3, 3>(P.W, VecLoader3{x}, Store3{y}, carry, s);
+  return hipGetLastError();
+}
+hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, double* carry, hipStream_t s) {
+  run_tiles<3, 6>(P.H, VecLoader6{w}, Store3{y}, carry, s);
+  return hipGetLastError();
+}
+hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* w, double* carry, hipStream_t s) {
+  run_tiles<6, 3>(P.HT, VecLoader3{r}, StoreMinv{w, P.minv, P.nb}, carry, s);
+  return hipGetLastError();
+}
+hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* y, double* wbody, double* carry,
+                          hipStream_t s) {
+  const hipError_t e = launch_ht_minv(P, x, wbody, carry, s);
+  if (e != hipSuccess) return e;
+  return launch_h(P, wbody, y, carry, s);
+}
+
+// ------------------------------------------------------------------ residual
+// u = (W r or matrix-free) + b is formed first; this kernel reduces the FB terms.
+__global__ void __launch_bounds__(256) k_fb_terms(int nc, int coulomb, const double* __restrict__ mu,
+                                                  const double* __restrict__ scal, const double* __restrict__ u,
+                                                  const double* __restrict__ b, const double* __restrict__ r,
+                                                  double* terms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const double s = scal ? scal[i] : 1.0;
+  const double x[3] = {s * r[3 * i], s * r[3 * i + 1], s * r[3 * i + 2]};
+  const double y[3] = {u[3 * i] + b[3 * i], u[3 * i + 1] + b[3 * i + 1], u[3 * i + 2] + b[3 * i + 2]};
+  terms[i] = law_eval<double>(coulomb != 0, mu[i], x, y);
+}
+
+hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* scal, const double* u,
+                           const double* b, const double* r, double* terms, hipStream_t s) {
+  if (nc > 0) k_fb_terms<<<(nc + 255) / 256, 256, 0, s>>>(nc, coulomb, mu, scal, u, b, r, terms);
+  return hipGetLastError();
+}
+
+__global__ void k_reset_sync(unsigned* count, unsigned* gen, int* error) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *count = 0u;
+    *gen = 0u;
+    *error = 0;
+  }
+}
+
+hipError_t launch_reset_sync(unsigned* count, unsigned* gen, int* error, hipStream_t s) {
+  k_reset_sync<<<1, 64, 0, s>>>(count, gen, error);
+  return hipGetLastError();
+}
+
+}  // namespace sb

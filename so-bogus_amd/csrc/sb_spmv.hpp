@@ -1,0 +1,24 @@
+// Launchers of the stand-alone block SpMV and helper kernels (sb_spmv.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sb_problem.hpp"
+
+namespace sb {
+
+hipError_t launch_bsr_generic(const TileMat<double>& A, int br, int bc, double alpha, const double* x, double beta,
+                              double* y, double* carry, hipStream_t s);
+hipError_t launch_w(const DevProblem<double>& P, const double* x, double* y, double* carry, hipStream_t s);
+hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, double* carry, hipStream_t s);
+hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* w, double* carry, hipStream_t s);
+hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* y, double* wbody, double* carry,
+                          hipStream_t s);
+hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* scal, const double* u,
+                           const double* b, const double* r, double* terms, hipStream_t s);
+hipError_t launch_reset_sync(unsigned* count, unsigned* gen, int* error, hipStream_t s);
+
+template <bool MF>
+__global__ void pg_kernel(PGArgs a);
+__global__ void gs_kernel(GSArgs a, const int* wrowptr);
+
+}  // namespace sb

@@ -1,0 +1,103 @@
+// Tiled block-sparse row gather ("lane per block") for wave64.
+//
+// A block-sparse matrix with BR x BC blocks is stored as:
+//   * blocks in row order (the reference's compressed major index,
+//     src/Core/Block/CompressedSparseBlockIndex.hpp:19-144), with the row and
+//     column of every block (brow, bcol);
+//   * values element-major: vals[e * nblocks + q] is element e (row-major in the
+//     block) of block q, so lane l of a wave reading block q0 + l reads
+//     consecutive addresses for every e (coalesced HBM traffic);
+//   * tiles of at most 64 consecutive blocks made of whole rows. A row longer
+//     than 64 blocks (a hub) gets tiles of its own ("long" tiles), whose partial
+//     sums go to a carry buffer and are combined by a fix-up pass.
+// One wave processes one tile: lane l multiplies block q0 + l with its column
+// vector, then a segmented inclusive scan over lanes (rows are contiguous)
+// leaves each row's sum in the row's last lane (the "leader").
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sb_device.hpp"
+
+namespace sb {
+
+template <typename T>
+struct TileMat {
+  int nrows = 0, ncols = 0, nblocks = 0, ntiles = 0;
+  int scan_steps = 0;          // ceil(log2(longest row inside a tile))
+  const int* tstart = nullptr; // [ntiles + 1] block offsets
+  const int* brow = nullptr;   // [nblocks]
+  const int* bcol = nullptr;   // [nblocks]
+  const int* tlong = nullptr;  // [ntiles] -1: whole rows; >= 0: tile holds a piece of that long row
+  const T* vals = nullptr;     // [BR*BC][nblocks]
+  int nlong = 0;               // long rows
+  const int* long_row = nullptr;  // [nlong]
+  const int* long_t0 = nullptr;   // [nlong] first tile
+  const int* long_t1 = nullptr;   // [nlong] one past last tile
+};
+
+// Processes tile t of A. load(col, xc[BC]) returns the column vector (in the
+// accumulation type A); lead(row, v[BR]) consumes complete row sums; long-tile
+// partial sums go to carry[t * BR + r].
+template <int BR, int BC, typename T, typename Acc, typename Loader, typename Leader>
+__device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& load, Leader& lead, Acc* carry) {
+  const int l = lane_id();
+  const int q0 = A.tstart[t], q1 = A.tstart[t + 1];
+  const int q = q0 + l;
+  const bool act = q < q1;
+  int row = -1;
+  Acc v[BR];
+#pragma unroll
+  for (int r = 0; r < BR; ++r) v[r] = Acc(0);
+  if (act) {
+    row = A.brow[q];
+    Acc xc[BC];
+    load(A.bcol[q], xc);
+    const T* vq = A.vals + q;
+    const size_t nb = (size_t)A.nblocks;
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+      Acc acc = Acc(0);
+#pragma unroll
+      for (int c = 0; c < BC; ++c) acc += Acc(vq[(size_t)(r * BC + c) * nb]) * xc[c];
+      v[r] = acc;
+    }
+  }
+  for (int s = 0; s < A.scan_steps; ++s) {
+    const int d = 1 << s;
+    const int ru = __shfl_up(row, d, SB_WAVE);
+    Acc vu[BR];
+#pragma unroll
+    for (int r = 0; r < BR; ++r) vu[r] = __shfl_up(v[r], d, SB_WAVE);
+    if (l >= d && ru == row) {
+#pragma unroll
+      for (int r = 0; r < BR; ++r) v[r] += vu[r];
+    }
+  }
+  const int rn = __shfl_down(row, 1, SB_WAVE);
+  const bool leader = act && (l == SB_WAVE - 1 || rn != row);
+  if (leader) {
+    if (A.tlong[t] >= 0) {
+#pragma unroll
+      for (int r = 0; r < BR; ++r) carry[(size_t)t * BR + r] = v[r];
+    } else {
+      lead(row, v);
+    }
+  }
+}
+
+// Sum of the carries of long row k (one wave), fixed order: lane-strided partial
+// sums then the xor tree. Result valid in every lane.
+template <int BR, typename T, typename Acc>
+__device__ __forceinline__ void long_row_sum(const TileMat<T>& A, int k, const Acc* carry, Acc v[BR]) {
+  const int t0 = A.long_t0[k], t1 = A.long_t1[k];
+#pragma unroll
+  for (int r = 0; r < BR; ++r) v[r] = Acc(0);
+  for (int t = t0 + lane_id(); t < t1; t += SB_WAVE) {
+#pragma unroll
+    for (int r = 0; r < BR; ++r) v[r] += carry[(size_t)t * BR + r];
+  }
+#pragma unroll
+  for (int r = 0; r < BR; ++r) v[r] = wave_sum(v[r]);
+}
+
+}  // namespace sb

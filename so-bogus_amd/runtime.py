@@ -1,0 +1,410 @@
+"""ctypes binding of libsbhip.so (the C ABI declared in include/sobogus_hip.h).
+
+Python-side mirror of the reference's solve-path API for tests and benchmarks:
+``SparseBlockMatrix`` (BSR multiply), ``DualFrictionProblem`` (solveWith /
+evalWith / applyPermutation / coloring), ``ProjectedGradient`` and
+``GaussSeidel`` parameter objects. Every product, residual and solve runs in the
+HIP library; if the library is missing or no GPU is usable, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libsbhip.so"
+HEADER = PKG_DIR.parent / "include" / "sobogus_hip.h"
+
+SB_OK, SB_NOT_CONVERGED = 0, 1
+SB_ERR = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "no device", -5: "unsupported",
+          -6: "barrier timeout", -7: "non-finite residual"}
+PG_STANDARD, PG_DESCENT, PG_CONJUGATED, PG_APGD, PG_SPG = range(5)
+PURE_NEWTON, PURE_ENUMERATIVE, HYBRID, REV_HYBRID = range(4)
+LAW_SOC, LAW_COULOMB = 0, 1
+OP_ASSEMBLED, OP_MATRIX_FREE = 0, 1
+SCALING_REFERENCE, SCALING_DIAGONAL, SCALING_NONE = 0, 1, 2
+COLORING_REFERENCE, COLORING_GREEDY = 0, 1
+BSR_SYMMETRIC = 1
+PROBLEM_ASSEMBLE_W = 1
+
+# Every extern "C" symbol of include/sobogus_hip.h
+EXPORTED = [
+    "sb_status_string", "sb_pg_options_default", "sb_gs_options_default", "sb_ctx_create", "sb_ctx_destroy",
+    "sb_ctx_info", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
+    "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
+    "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
+    "sb_gs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
+    "sb_bench_spmv",
+]
+
+
+class PGOptions(C.Structure):
+    _fields_ = [("variant", C.c_int), ("law", C.c_int), ("op", C.c_int), ("scaling", C.c_int),
+                ("inf_norm", C.c_int), ("tol", C.c_double), ("max_iters", C.c_int), ("ls_iters", C.c_int),
+                ("ls_optimistic", C.c_double), ("ls_pessimistic", C.c_double), ("ls_armijo", C.c_double),
+                ("grid_blocks", C.c_int)]
+
+
+class GSOptions(C.Structure):
+    _fields_ = [("law", C.c_int), ("strategy", C.c_int), ("inf_norm", C.c_int), ("try_zero", C.c_int),
+                ("tol", C.c_double), ("max_iters", C.c_int), ("eval_every", C.c_int), ("skip_tol", C.c_double),
+                ("skip_iters", C.c_int), ("auto_regularization", C.c_double), ("grid_blocks", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("spmvs", C.c_int), ("ls_trials", C.c_int), ("nesterov_steps", C.c_int),
+                ("converged", C.c_int), ("local_failures", C.c_int), ("residual", C.c_double),
+                ("device_ms", C.c_double), ("history", C.POINTER(C.c_double)), ("history_capacity", C.c_int),
+                ("history_length", C.c_int)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+_lib = None
+
+
+class SbError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: status {code} ({_status_string(code)})")
+        self.code = code
+
+
+def _status_string(code: int) -> str:
+    try:
+        return lib().sb_status_string(code).decode()
+    except Exception:  # pragma: no cover - library absent
+        return SB_ERR.get(code, "unknown")
+
+
+def lib():
+    """Load libsbhip.so; raises if it has not been built (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C so-bogus_amd)")
+    L = C.CDLL(str(LIB_PATH))
+    sig = {
+        "sb_status_string": (C.c_char_p, [C.c_int]),
+        "sb_pg_options_default": (None, [C.POINTER(PGOptions)]),
+        "sb_gs_options_default": (None, [C.POINTER(GSOptions)]),
+        "sb_ctx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+        "sb_ctx_destroy": (C.c_int, [_vp]),
+        "sb_ctx_info": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+        "sb_bsr_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
+        "sb_bsr_destroy": (C.c_int, [_vp]),
+        "sb_bsr_multiply": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int]),
+        "sb_bsr_split_row_multiply": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp]),
+        "sb_problem_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
+                                        C.c_int]),
+        "sb_problem_set_w": (C.c_int, [_vp, _vp, _ip, _ip, _dp]),
+        "sb_problem_destroy": (C.c_int, [_vp]),
+        "sb_problem_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _lp, _lp]),
+        "sb_problem_get_w": (C.c_int, [_vp, _vp, _ip, _ip, _dp]),
+        "sb_problem_apply_permutation": (C.c_int, [_vp, _vp, _lp]),
+        "sb_problem_coloring": (C.c_int, [_vp, _vp, C.c_int, _lp, _lp, C.POINTER(C.c_int)]),
+        "sb_problem_apply_w": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int]),
+        "sb_fb_residual": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _dp, C.c_int]),
+        "sb_pg_solve": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), _vp, C.c_int, _dp, C.POINTER(Stats)]),
+        "sb_gs_solve": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, _vp, C.c_int, _dp,
+                                  C.POINTER(Stats)]),
+        "sb_device_alloc": (C.c_int, [_vp, C.POINTER(_vp), C.c_size_t]),
+        "sb_device_free": (C.c_int, [_vp, _vp]),
+        "sb_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+        "sb_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+        "sb_synchronize": (C.c_int, [_vp]),
+        "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code: int, what: str, allow_not_converged: bool = False) -> int:
+    if code == SB_OK or (allow_not_converged and code == SB_NOT_CONVERGED):
+        return code
+    raise SbError(code, what)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _l(a):
+    return a.ctypes.data_as(_lp)
+
+
+class Context:
+    """One HIP device and stream (sb_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().sb_ctx_create(C.byref(h), device), "sb_ctx_create")
+        self.h = h
+        name = C.create_string_buffer(128)
+        ncu = C.c_int(0)
+        lib().sb_ctx_info(self.h, name, 128, C.byref(ncu))
+        self.arch = name.value.decode()
+        self.n_cu = ncu.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().sb_synchronize(self.h), "sb_synchronize")
+
+
+class DeviceArray:
+    """A float64 vector resident in HBM (for solves without host round trips)."""
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx, self.n = ctx, int(n)
+        p = _vp()
+        _check(lib().sb_device_alloc(ctx.h, C.byref(p), 8 * max(self.n, 1)), "sb_device_alloc")
+        self.ptr = p
+
+    def upload(self, a):
+        a = _f64(a)
+        assert a.size == self.n
+        _check(lib().sb_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), 8 * self.n), "h2d")
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n)
+        _check(lib().sb_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, 8 * self.n), "d2h")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().sb_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class SparseBlockMatrix:
+    """Device BSR matrix (bogus::SparseBlockMatrix) with multiply<Transpose>."""
+
+    def __init__(self, ctx: Context, nbr, nbc, br, bc, rowptr, colidx, vals, symmetric=False):
+        self.ctx, self.nbr, self.nbc, self.br, self.bc, self.symmetric = ctx, nbr, nbc, br, bc, symmetric
+        rp, ci, v = _i32(rowptr), _i32(colidx), _f64(vals).reshape(-1)
+        h = _vp()
+        _check(lib().sb_bsr_create(ctx.h, C.byref(h), nbr, nbc, br, bc, BSR_SYMMETRIC if symmetric else 0, _i(rp),
+                                   _i(ci), _d(v)), "sb_bsr_create")
+        self.h = h
+
+    def multiply(self, x, transpose=False, alpha=1.0, beta=0.0, res=None) -> np.ndarray:
+        x = _f64(x)
+        tr = transpose and not self.symmetric
+        n_out = self.nbc * self.bc if tr else self.nbr * self.br
+        out = np.zeros(n_out) if res is None else _f64(res).copy()
+        _check(lib().sb_bsr_multiply(self.ctx.h, self.h, int(transpose), alpha, _d(x), beta, _d(out), 0),
+               "sb_bsr_multiply")
+        return out
+
+    def split_row_multiply(self, row: int, x, res3=None) -> np.ndarray:
+        out = np.zeros(3) if res3 is None else _f64(res3).copy()
+        _check(lib().sb_bsr_split_row_multiply(self.ctx.h, self.h, row, _d(_f64(x)), _d(out)),
+               "sb_bsr_split_row_multiply")
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sb_bsr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def pg_options(**kw) -> PGOptions:
+    o = PGOptions()
+    lib().sb_pg_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def gs_options(**kw) -> GSOptions:
+    o = GSOptions()
+    lib().sb_gs_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class SolveResult:
+    status: int
+    residual: float
+    r: np.ndarray | None
+    iterations: int
+    spmvs: int
+    ls_trials: int
+    nesterov_steps: int
+    converged: bool
+    local_failures: int
+    device_ms: float
+    history: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+
+class DualFrictionProblem:
+    """u = W r + b with W = H Minv H^T on the device (sb_problem)."""
+
+    def __init__(self, ctx: Context, prob, assemble: bool = True):
+        self.ctx, self.nc, self.nb = ctx, prob.nc, prob.nb
+        rp, ci, hv, mi, b, mu = (_i32(prob.rowptr), _i32(prob.colidx), _f64(prob.hvals).reshape(-1),
+                                 _f64(prob.minv).reshape(-1), _f64(prob.b), _f64(prob.mu))
+        h = _vp()
+        _check(lib().sb_problem_create(ctx.h, C.byref(h), prob.nc, prob.nb, _i(rp), _i(ci), _d(hv), _d(mi), _d(b),
+                                       _d(mu), PROBLEM_ASSEMBLE_W if assemble else 0), "sb_problem_create")
+        self.h = h
+        self.assembled = assemble
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sb_problem_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def sizes(self):
+        nc, nb = C.c_int(), C.c_int()
+        hb, wb = C.c_int64(), C.c_int64()
+        _check(lib().sb_problem_sizes(self.h, C.byref(nc), C.byref(nb), C.byref(hb), C.byref(wb)), "sizes")
+        return nc.value, nb.value, hb.value, wb.value
+
+    def w_lower(self):
+        nc, _, _, wb = self.sizes()
+        rp, ci, v = np.zeros(nc + 1, np.int32), np.zeros(wb, np.int32), np.zeros(wb * 9)
+        _check(lib().sb_problem_get_w(self.ctx.h, self.h, _i(rp), _i(ci), _d(v)), "sb_problem_get_w")
+        return rp, ci, v.reshape(wb, 9)
+
+    def apply_w(self, x, op=OP_ASSEMBLED) -> np.ndarray:
+        x = _f64(x)
+        y = np.zeros(3 * self.nc)
+        _check(lib().sb_problem_apply_w(self.ctx.h, self.h, op, x.ctypes.data_as(_vp), y.ctypes.data_as(_vp), 0),
+               "sb_problem_apply_w")
+        return y
+
+    def eval_with(self, r, law=LAW_COULOMB, op=OP_ASSEMBLED, scaling=SCALING_REFERENCE, inf_norm=False) -> float:
+        r = _f64(r)
+        out = C.c_double()
+        _check(lib().sb_fb_residual(self.ctx.h, self.h, law, op, scaling, int(inf_norm), r.ctypes.data_as(_vp),
+                                    C.byref(out), 0), "sb_fb_residual")
+        return out.value
+
+    def coloring(self, mode=COLORING_GREEDY):
+        perm = np.zeros(self.nc, np.int64)
+        colors = np.zeros(self.nc + 1, np.int64)
+        n = C.c_int()
+        _check(lib().sb_problem_coloring(self.ctx.h, self.h, mode, _l(perm), _l(colors), C.byref(n)), "coloring")
+        return perm, colors[:n.value + 1].copy()
+
+    def apply_permutation(self, perm):
+        perm = _i64(perm)
+        _check(lib().sb_problem_apply_permutation(self.ctx.h, self.h, _l(perm)), "apply_permutation")
+
+    def _run(self, fn, r, on_device, hist_cap):
+        st = Stats()
+        hist = np.zeros(max(hist_cap, 1))
+        st.history = _d(hist)
+        st.history_capacity = hist_cap
+        res = C.c_double()
+        if on_device:
+            code = fn(r.ptr, 1, C.byref(res), C.byref(st))
+            rout = None
+        else:
+            rout = _f64(r).copy()
+            code = fn(rout.ctypes.data_as(_vp), 0, C.byref(res), C.byref(st))
+        _check(code, "solve", allow_not_converged=True)
+        return SolveResult(status=code, residual=res.value, r=rout, iterations=st.iterations, spmvs=st.spmvs,
+                           ls_trials=st.ls_trials, nesterov_steps=st.nesterov_steps, converged=bool(st.converged),
+                           local_failures=st.local_failures, device_ms=st.device_ms,
+                           history=hist[:min(st.history_length, hist_cap)].copy())
+
+    def solve_pg(self, r0, opts: PGOptions | None = None, on_device=False, hist_cap=0, **kw) -> SolveResult:
+        """ProjectedGradient::solve through DualFrictionProblem::solveWith(pg, r, staticProblem)."""
+        o = opts if opts is not None else pg_options(**kw)
+        fn = lambda rp, dev, res, st: lib().sb_pg_solve(self.ctx.h, self.h, C.byref(o), rp, dev, res, st)
+        return self._run(fn, r0, on_device, hist_cap)
+
+    def solve_gs(self, r0, opts: GSOptions | None = None, colors=None, on_device=False, hist_cap=0,
+                 **kw) -> SolveResult:
+        """GaussSeidel::solve on color-contiguous contacts (DualFrictionProblem::solveWith(gs, r))."""
+        o = opts if opts is not None else gs_options(**kw)
+        if colors is None:
+            cp, nco = None, 0
+        else:
+            colors = _i64(colors)
+            cp, nco = _l(colors), len(colors) - 1
+        fn = lambda rp, dev, res, st: lib().sb_gs_solve(self.ctx.h, self.h, C.byref(o), cp, nco, rp, dev, res, st)
+        return self._run(fn, r0, on_device, hist_cap)
+
+    def bench_spmv(self, kind: int, reps: int = 20) -> float:
+        ms = C.c_double()
+        _check(lib().sb_bench_spmv(self.ctx.h, self.h, kind, reps, C.byref(ms)), "sb_bench_spmv")
+        return ms.value
+
+
+def gpu_available() -> bool:
+    """True when the library loads and a device context can be created."""
+    try:
+        ctx = Context(0)
+        ctx.close()
+        return True
+    except Exception:
+        return False
+
+
+def exported_symbols() -> list[str]:
+    return list(EXPORTED)
+
+
+def header_symbols() -> list[str]:
+    import re
+    text = HEADER.read_text()
+    decl = re.findall(r"^\s*(?:int|void|const char\s*\*)\s*\*?\s*(sb_[a-z0-9_]+)\s*\(", text, flags=re.M)
+    return sorted(set(decl))
+
+
+os.environ.setdefault("HSA_ENABLE_IPC_MODE_LEGACY", "0")

@@ -1,0 +1,115 @@
+"""Projected-gradient solves on the GPU against the oracle.
+
+Parity bar (FP64 path): the GPU result's FB residual, recomputed by the oracle,
+is <= tol (x1.05 for the different summation order of the residual itself), and
+the result agrees with the oracle running the same algorithm from the same
+start: relative |r - r_ref| <= 1e-6 for small problems where the trajectories
+stay together. Iteration counts are compared loosely (line-search decisions can
+flip on rounding, after which the iterates legitimately differ).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(G):
+    return G.small_friction_pb()
+
+
+@pytest.mark.parametrize("variant", ["PG_APGD", "PG_DESCENT", "PG_SPG"])
+def test_small_pg_soc_gpu(R, ctx, small, variant):
+    """tests/ProjectedGradient.cpp:15-60 on the GPU path: residual < 1e-8."""
+    print(f"case: 2-contact PG {variant}", flush=True)
+    prob, sol = small
+    dp = R.DualFrictionProblem(ctx, prob)
+    res = dp.solve_pg(np.ones(6), variant=getattr(R, variant), law=R.LAW_SOC, tol=1e-8)
+    assert res.residual < 1e-8 and res.converged
+
+
+def test_small_pg_matrix_free_gpu(R, ctx, small):
+    """tests/ProjectedGradient.cpp:63-83: matrix-free H Minv H^T product, residual < 1e-8."""
+    print("case: 2-contact matrix-free APGD", flush=True)
+    prob, sol = small
+    dp = R.DualFrictionProblem(ctx, prob, assemble=False)
+    res = dp.solve_pg(np.ones(6), law=R.LAW_SOC, op=R.OP_MATRIX_FREE, tol=1e-8)
+    assert res.residual < 1e-8
+
+
+def test_small_pg_trajectory_matches_oracle(R, O, ctx, small):
+    """Same algorithm, same start: residual history and result match the oracle."""
+    print("case: 2-contact APGD trajectory", flush=True)
+    prob, sol = small
+    dp = R.DualFrictionProblem(ctx, prob)
+    op = O.OracleProblem(prob)
+    g = dp.solve_pg(np.ones(6), law=R.LAW_SOC, tol=1e-8, hist_cap=1000)
+    ores, ox, oinfo = op.pg_solve(np.ones(6), tol=1e-8, coulomb=False)
+    assert g.iterations == oinfo["iterations"]
+    assert np.allclose(g.history, oinfo["history"], rtol=1e-6, atol=1e-15)
+    assert np.allclose(g.r, ox, rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("variant", ["PG_APGD", "PG_SPG"])
+@pytest.mark.parametrize("law", ["LAW_COULOMB", "LAW_SOC"])
+def test_1k_parity(R, O, G, ctx, variant, law):
+    """1,000 contacts / 500 bodies, tol 1e-8, cold start."""
+    print(f"case: 1k {variant} {law}", flush=True)
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    op = O.OracleProblem(prob)
+    coul = law == "LAW_COULOMB"
+    g = dp.solve_pg(np.zeros(3 * prob.nc), variant=getattr(R, variant), law=getattr(R, law), tol=1e-8,
+                    max_iters=5000, hist_cap=5000)
+    assert g.converged
+    chk = op.eval(g.r, coulomb=coul)
+    assert chk <= 1.05e-8
+    ores, ox, oinfo = op.pg_solve(np.zeros(3 * prob.nc), variant=getattr(O, variant), tol=1e-8, max_iters=5000,
+                                  coulomb=coul)
+    # same algorithm: iteration counts within 25%, and the first iterations coincide
+    assert abs(g.iterations - oinfo["iterations"]) <= 0.25 * oinfo["iterations"] + 5
+    k = min(20, len(g.history), len(oinfo["history"]))
+    assert np.allclose(g.history[:k], oinfo["history"][:k], rtol=1e-6)
+    # the velocities u = W r + b agree to the tolerance scale
+    u_g = op.apply_w(g.r) + prob.b
+    u_o = op.apply_w(ox) + prob.b
+    assert np.linalg.norm(u_g - u_o) / np.linalg.norm(u_o) < 1e-2
+    print(f"   gpu iters {g.iterations} oracle {oinfo['iterations']} res {g.residual:.3e} check {chk:.3e} "
+          f"|r-rref|/|rref| {np.linalg.norm(g.r - ox) / np.linalg.norm(ox):.2e}", flush=True)
+
+
+def test_matrix_free_equals_assembled_operator(R, O, G, ctx):
+    """APGD with the matrix-free product and diagonal scaling reaches the same
+    tolerance as with the assembled W (both checked by the oracle)."""
+    print("case: 1k APGD matrix-free vs assembled", flush=True)
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    op = O.OracleProblem(prob)
+    g = dp.solve_pg(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL,
+                    tol=1e-8, max_iters=5000)
+    assert g.converged and op.eval(g.r, coulomb=True) <= 1.05e-8
+
+
+def test_warm_start_and_iteration_cap(R, O, G, ctx):
+    print("case: warm start / max_iters", flush=True)
+    prob = G.generate(nc=500, nb=250, seed=17)
+    dp = R.DualFrictionProblem(ctx, prob)
+    g = dp.solve_pg(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, tol=1e-8, max_iters=5000)
+    assert g.converged
+    again = dp.solve_pg(g.r, law=R.LAW_COULOMB, tol=1e-8, max_iters=5000)
+    assert again.converged and again.iterations == 0
+    capped = dp.solve_pg(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, tol=1e-14, max_iters=7)
+    assert capped.status == R.SB_NOT_CONVERGED and capped.iterations == 7
+    # the returned r is the best iterate: its residual is the returned value
+    assert dp.eval_with(capped.r, law=R.LAW_COULOMB) == pytest.approx(capped.residual, rel=1e-9)
+
+
+def test_device_resident_solve(R, O, G, ctx):
+    print("case: device-resident r", flush=True)
+    prob = G.generate(nc=800, nb=400, seed=23)
+    dp = R.DualFrictionProblem(ctx, prob)
+    r = R.DeviceArray(ctx, 3 * prob.nc).upload(np.zeros(3 * prob.nc))
+    g = dp.solve_pg(r, law=R.LAW_COULOMB, tol=1e-8, max_iters=5000, on_device=True)
+    host = dp.solve_pg(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, tol=1e-8, max_iters=5000)
+    assert g.converged
+    assert np.array_equal(r.download(), host.r)  # deterministic: same bits both ways
