@@ -436,7 +436,7 @@ int sb_bsr_create(sb_ctx* ctx, sb_bsr** A, int nbr, int nbc, int br, int bc, int
                   const int32_t* colidx, const double* vals) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
-  if (!A || nbr < 0 || nbc < 0 || br < 1 || bc < 1 || br > 8 || bc > 8 || !rowptr) return SB_ERR_INVALID;
+  if (!A || nbr < 1 || nbc < 1 || br < 1 || bc < 1 || br > 8 || bc > 8 || !rowptr) return SB_ERR_INVALID;
   const bool sym = (flags & SB_BSR_SYMMETRIC) != 0;
   if (sym && (br != bc || nbr != nbc)) return SB_ERR_INVALID;
   const int nnz = rowptr[nbr];
