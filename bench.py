@@ -158,6 +158,9 @@ def main() -> int:
         print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(prob, args.tol, threads)
     print(json.dumps(out), flush=True)
+    # release device objects before interpreter shutdown
+    del r, dp
+    ctx.close()
     return 0
 
 

@@ -93,6 +93,7 @@ typedef struct sb_pg_options {
   double ls_pessimistic;  /* 0.5 */
   double ls_armijo;       /* 0.5 (Standard/Conjugated only) */
   int grid_blocks;        /* 0 = automatic; else persistent grid size (clamped) */
+  int block_threads;      /* 0 = automatic; else 256 or 512 threads per block */
 } sb_pg_options;
 
 /* GaussSeidel parameters (GaussSeidelBase.hpp:39-91) */
@@ -108,6 +109,7 @@ typedef struct sb_gs_options {
   int skip_iters;         /* 10 */
   double auto_regularization; /* 0 */
   int grid_blocks;        /* 0 = automatic */
+  int block_threads;      /* 0 = automatic (256, the only size of the GS kernel) */
 } sb_gs_options;
 
 typedef struct sb_stats {
@@ -196,6 +198,10 @@ int sb_synchronize(sb_ctx* ctx);
    context's stream with HIP events; kind: 0 = assembled W x, 1 = H w, 2 = H^T r (+ Minv),
    3 = matrix-free W x (H^T+Minv, then H). Returns the mean milliseconds per launch. */
 int sb_bench_spmv(sb_ctx* ctx, const sb_problem* P, int kind, int reps, double* ms_per_launch);
+/* Phase-time breakdown of the last projected-gradient solve on this context, as seen
+   by block 0 of the persistent kernel (microseconds): operator-phase work, operator-phase
+   reduction + barrier wait, update-phase work, update-phase reduction + barrier wait. */
+int sb_last_phase_times(sb_ctx* ctx, double* us4);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,9 @@ using sb::TileMat;
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;     // default block size of the persistent kernels
+constexpr int kMaxThreads = 512;  // __launch_bounds__ of the persistent kernels
+constexpr size_t kSyncWords = 2 * sb::kSyncGroups * sb::kSyncStride + sb::kSyncStride;
 
 #define SB_HIP(expr)                                  \
   do {                                                \
@@ -114,10 +116,11 @@ struct sb_ctx {
   int ncu = 0;
   std::string name;
   int coop = 0;
-  DBuf<unsigned> count, gen;
+  DBuf<unsigned> sync;  // group counters, top counter, release words
   DBuf<int> error;
-  DBuf<double> partials, reduced, out;
+  DBuf<double> partials, out;
   int maxGrid = 0;  // capacity of partials (blocks)
+  double phase_us[4] = {0, 0, 0, 0};  // last PG solve: operator work/wait, update work/wait
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -175,7 +178,7 @@ int check_ctx(sb_ctx* ctx) {
 
 int ensure_grid_scratch(sb_ctx* ctx, int blocks) {
   if (blocks <= ctx->maxGrid) return SB_OK;
-  SB_HIP(ctx->partials.alloc((size_t)blocks * sb::kMaxPartials));
+  SB_HIP(ctx->partials.alloc((size_t)2 * blocks * sb::kMaxPartials));
   ctx->maxGrid = blocks;
   return SB_OK;
 }
@@ -255,13 +258,22 @@ int copy_out(sb_ctx* ctx, double* dst, const double* src, size_t n, int on_devic
   return SB_OK;
 }
 
-int grid_for(sb_ctx* ctx, const void* func, int requested, int work_waves) {
+// Persistent grid: at most the co-resident blocks (cooperative launch), by
+// default one block per CU (the barrier cost grows with the number of blocks),
+// and no more blocks than there are tiles of work.
+int grid_for(sb_ctx* ctx, const void* func, int threads, int requested, int work_waves) {
   int perCU = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, func, kThreads, 0) != hipSuccess || perCU < 1) perCU = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, func, threads, 0) != hipSuccess || perCU < 1) perCU = 1;
   const int maxCoop = perCU * ctx->ncu;
-  int want = requested > 0 ? requested : (work_waves + (kThreads / 64) - 1) / (kThreads / 64);
+  const int wpb = threads / 64;
+  int want = requested > 0 ? requested : std::min(ctx->ncu, (work_waves + wpb - 1) / wpb);
   want = std::max(1, std::min(want, maxCoop));
   return want;
+}
+
+int block_threads_for(int requested) {
+  if (requested == 0) return kThreads;
+  return (requested == 256 || requested == 512) ? requested : -1;
 }
 
 int read_status(sb_ctx* ctx, double* out, int nout) {
@@ -272,11 +284,11 @@ int read_status(sb_ctx* ctx, double* out, int nout) {
 
 sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
   sb::GridSync g;
-  g.count = ctx->count.p;
-  g.gen = ctx->gen.p;
+  g.group = ctx->sync.p;
+  g.release = ctx->sync.p + sb::kSyncGroups * sb::kSyncStride;
+  g.top = ctx->sync.p + 2 * sb::kSyncGroups * sb::kSyncStride;
   g.error = ctx->error.p;
   g.partials = ctx->partials.p;
-  g.reduced = ctx->reduced.p;
   g.nblocks = (unsigned)blocks;
   g.timeout_ticks = 100ull * 1000 * 1000 * 20;  // 20 s at 100 MHz
   return g;
@@ -363,6 +375,7 @@ void sb_pg_options_default(sb_pg_options* o) {
   o->ls_pessimistic = 0.5;
   o->ls_armijo = 0.5;
   o->grid_blocks = 0;
+  o->block_threads = 0;
 }
 
 void sb_gs_options_default(sb_gs_options* o) {
@@ -378,6 +391,7 @@ void sb_gs_options_default(sb_gs_options* o) {
   o->skip_iters = 10;
   o->auto_regularization = 0.0;
   o->grid_blocks = 0;
+  o->block_threads = 0;
 }
 
 int sb_ctx_create(sb_ctx** out, int device) {
@@ -396,10 +410,8 @@ int sb_ctx_create(sb_ctx** out, int device) {
   c->name = prop.gcnArchName;
   SB_HIP(hipDeviceGetAttribute(&c->coop, hipDeviceAttributeCooperativeLaunch, device));
   SB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  SB_HIP(c->count.alloc(1));
-  SB_HIP(c->gen.alloc(1));
+  SB_HIP(c->sync.alloc(kSyncWords));
   SB_HIP(c->error.alloc(1));
-  SB_HIP(c->reduced.alloc(sb::kMaxPartials));
   SB_HIP(c->out.alloc(16));
   SB_HIP(hipEventCreate(&c->ev0));
   SB_HIP(hipEventCreate(&c->ev1));
@@ -794,17 +806,20 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   a.out = ctx->out.p;
   const void* fn = mf ? (const void*)&sb::pg_kernel<true> : (const void*)&sb::pg_kernel<false>;
   const int workWaves = mf ? std::max(P->H.host.ntiles, P->HT.host.ntiles) : P->W.host.ntiles;
-  const int grid = grid_for(ctx, fn, o->grid_blocks, workWaves);
+  const int threads = block_threads_for(o->block_threads);
+  if (threads < 0) return SB_ERR_INVALID;
+  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, workWaves);
   if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
   a.gs = make_sync(ctx, grid);
-  SB_HIP(sb::launch_reset_sync(ctx->count.p, ctx->gen.p, ctx->error.p, s));
+  SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
   void* args[] = {&a};
   SB_HIP(hipEventRecord(ctx->ev0, s));
-  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kThreads), args, 0, s));
+  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
-  double out[8];
-  if ((rc = read_status(ctx, out, 8))) return rc;
+  double out[12];
+  if ((rc = read_status(ctx, out, 12))) return rc;
   if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
+  for (int k = 0; k < 4; ++k) ctx->phase_us[k] = out[7 + k];
   int hl = std::min((int)out[1] + 1, histCap);
   if (st) {
     float ms = 0.f;
@@ -900,14 +915,16 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   const void* fn = (const void*)&sb::gs_kernel;
   int maxColor = 0;
   for (size_t c = 0; c + 1 < col.size(); ++c) maxColor = std::max(maxColor, col[c + 1] - col[c]);
-  const int grid = grid_for(ctx, fn, o->grid_blocks, std::max(P->W.host.ntiles, (maxColor + 63) / 64));
+  const int threads = block_threads_for(o->block_threads);
+  if (threads != 256) return threads < 0 ? SB_ERR_INVALID : SB_ERR_UNSUPPORTED;  // GS kernel: 256 threads
+  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, std::max(P->W.host.ntiles, (maxColor + 63) / 64));
   if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
   a.gs = make_sync(ctx, grid);
-  SB_HIP(sb::launch_reset_sync(ctx->count.p, ctx->gen.p, ctx->error.p, s));
+  SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
   const int* wrowptr = P->W.rowptr.p;
   void* args[] = {&a, &wrowptr};
   SB_HIP(hipEventRecord(ctx->ev0, s));
-  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kThreads), args, 0, s));
+  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
   double out[8];
   if ((rc = read_status(ctx, out, 8))) return rc;
@@ -968,6 +985,12 @@ int sb_synchronize(sb_ctx* ctx) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+int sb_last_phase_times(sb_ctx* ctx, double* us4) {
+  if (!ctx || !us4) return SB_ERR_INVALID;
+  for (int k = 0; k < 4; ++k) us4[k] = ctx->phase_us[k];
   return SB_OK;
 }
 

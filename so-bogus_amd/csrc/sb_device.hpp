@@ -1,6 +1,6 @@
 // Device utilities for the CDNA4 (gfx950) solve path: wave64 shuffles and
-// reductions, a bounded grid barrier with a deterministic in-barrier reduction,
-// and the persistent-kernel bookkeeping shared by the solver kernels.
+// reductions, XCD-coherent loads/stores, and a bounded fence-free grid barrier
+// with a deterministic grid reduction for persistent kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -34,116 +34,159 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+// ------------------------------------------------------------------ coherent access
+// MI355X has one L2 per XCD and the L2s are not coherent with each other for
+// ordinary device memory. Values that one block writes and another block reads
+// later in the same persistent kernel are stored at agent scope (relaxed atomic
+// store: global_store with the sc1 policy bit, written through to memory), so no
+// dirty copy stays behind in a private L2 and no L2 write-back is needed at the
+// barrier. After each barrier every block invalidates its L1/L2 once (acquire
+// fence), and loads of such values in the next phase are then ordinary cached
+// loads. cld() (agent-scope load) is kept for reads that may not follow a
+// barrier of the reading block.
+__device__ __forceinline__ void cst(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double cld(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void cst(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float cld(const float* p) {
+  return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------------ barrier
-// Grid-wide barrier for persistent kernels. All blocks of the grid must be
-// co-resident (cooperative launch). The last block to arrive reduces the
-// per-block partial sums in a fixed order (so every run and every block sees
-// the same bits), publishes them, and opens the barrier. Waiting is bounded by
-// a real-time deadline: on timeout the error word is set and every block
-// leaves the solver loop, so a broken schedule ends instead of hanging.
+// Grid-wide barrier for persistent kernels whose blocks are all co-resident
+// (cooperative launch). Counters are never reset during a launch (barrier
+// number e completes when a counter reaches a multiple of e + 1), so there is
+// no reset race. Every wave first waits for its own memory operations
+// (s_waitcnt 0), so the coherent stores issued before the barrier have
+// completed when the block arrives. Waiting is bounded by a real-time
+// deadline: on timeout the error word is set and every block leaves the solver
+// loop, so a broken schedule ends instead of hanging.
 struct GridSync {
-  unsigned* count;     // arrivals in the current generation
-  unsigned* gen;       // generation counter
+  unsigned* group;     // [kSyncGroups][kSyncStride] monotonic arrival counters per group
+  unsigned* top;       // monotonic count of completed groups
+  unsigned* release;   // [kSyncGroups][kSyncStride] epoch released to each group
   int* error;          // non-zero after a timeout (sticky)
-  double* partials;    // [nblocks][kMaxPartials]
-  double* reduced;     // [kMaxPartials] published results
+  double* partials;    // [2][nblocks][kMaxPartials], alternating between reductions
   unsigned nblocks;
   uint64_t timeout_ticks;  // s_memrealtime ticks (100 MHz)
 };
+constexpr unsigned kSyncGroups = 16;  // blocks are grouped by blockIdx % kSyncGroups
+constexpr unsigned kSyncStride = 32;  // one 128-byte line per counter
 constexpr int kMaxPartials = 8;
 
 __device__ __forceinline__ uint64_t realtime() { return __builtin_amdgcn_s_memrealtime(); }
 
-// kind bit k set => partial k is reduced with max instead of sum.
-__device__ inline bool grid_reduce_sync(const GridSync& gs, int nvals, unsigned max_mask) {
-  __shared__ unsigned s_last;
-  __shared__ int s_err;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned g = __hip_atomic_load(gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned arrived = __hip_atomic_fetch_add(gs.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (arrived == gs.nblocks - 1) ? 1u : 0u;
-    s_err = 0;
-    if (!s_last) {
-      const uint64_t t0 = realtime();
-      while (__hip_atomic_load(gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(2);
-        if (__hip_atomic_load(gs.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (realtime() - t0 > gs.timeout_ticks) {
-          __hip_atomic_store(gs.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-    }
-    __threadfence();
-  }
-  __syncthreads();
-  if (s_last) {
-    // fixed-order reduction of the partials by the whole last block
-    __shared__ double s_red[kMaxPartials][SB_WAVE];
-    const int nthreads = blockDim.x;
-    for (int k = 0; k < nvals; ++k) {
-      const bool isMax = (max_mask >> k) & 1u;
-      double acc = 0.0;
-      for (unsigned b = threadIdx.x; b < gs.nblocks; b += nthreads) {
-        const double v = __hip_atomic_load(&gs.partials[(size_t)b * kMaxPartials + k], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-        acc = isMax ? (v > acc ? v : acc) : acc + v;
-      }
-      // combine across the block: first within waves (fixed xor tree), then waves in order
-      acc = isMax ? wave_max(acc) : wave_sum(acc);
-      if (lane_id() == 0) s_red[k][wave_in_block()] = acc;
-    }
-    __syncthreads();
-    if (threadIdx.x < (unsigned)nvals) {
-      const int k = threadIdx.x;
-      const bool isMax = (max_mask >> k) & 1u;
-      double acc = 0.0;
-      const int nw = (nthreads + SB_WAVE - 1) / SB_WAVE;
-      for (int w = 0; w < nw; ++w) acc = isMax ? (s_red[k][w] > acc ? s_red[k][w] : acc) : acc + s_red[k][w];
-      __hip_atomic_store(&gs.reduced[k], acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+struct Barrier {
+  GridSync gs;
+  unsigned epoch = 0;
+  unsigned nred = 0;         // reductions done (selects the partials buffer)
+  double red[kMaxPartials];  // results of the last reduction (identical in every block)
+
+  // Two-level arrival: a block increments its group's counter; the group's last
+  // block increments the top counter; the last group writes the new epoch into
+  // every group's release word, which that group's blocks poll. Contention on any
+  // one address is limited to a group (or the kSyncGroups group leaders).
+  __device__ bool sync() {
+    __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-      __threadfence();
-      __hip_atomic_store(gs.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(gs.gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned G = gs.nblocks < kSyncGroups ? gs.nblocks : kSyncGroups;
+      const unsigned g = blockIdx.x % G;
+      const unsigned gsize = gs.nblocks / G + ((g < gs.nblocks % G) ? 1u : 0u);
+      const unsigned old = __hip_atomic_fetch_add(&gs.group[g * kSyncStride], 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+      if (old + 1 == (epoch + 1) * gsize) {
+        const unsigned t = __hip_atomic_fetch_add(gs.top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t + 1 == (epoch + 1) * G)
+          for (unsigned k = 0; k < G; ++k)
+            __hip_atomic_store(&gs.release[k * kSyncStride], epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const uint64_t t0 = realtime();
+      int ok = 1;
+      for (;;) {
+        // the sticky error word is checked first: no block passes a barrier after a timeout
+        if (__hip_atomic_load(gs.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ok = 0;
+          break;
+        }
+        if (__hip_atomic_load(&gs.release[g * kSyncStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= epoch + 1)
+          break;
+        if (realtime() - t0 > gs.timeout_ticks) {
+          __hip_atomic_store(gs.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      s_ok = ok;
+      // Acquire: drop this CU's L1 and this XCD's L2 copies of lines written by
+      // other blocks before the barrier (their coherent stores already reached
+      // memory), so that plain cached loads in the next phase read fresh values.
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    __syncthreads();
+    ++epoch;
+    return s_ok != 0;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) s_err = __hip_atomic_load(gs.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  return s_err == 0;
-}
 
-// Block-level partial reduction: every thread contributes v[0..nvals); thread 0
-// of the block writes the block's partials (sum, or max for bits of max_mask).
-template <int NV>
-__device__ inline void block_partials(const GridSync& gs, const double (&v)[NV], unsigned max_mask) {
-  __shared__ double s_p[NV][16];
-  double w[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) w[k] = ((max_mask >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
-  if (lane_id() == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) s_p[k][wave_in_block()] = w[k];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + SB_WAVE - 1) / SB_WAVE;
+  // Reduces v[0..NV) over the whole grid (sum, or max for the bits of max_mask)
+  // and leaves the results in red[] of every block. Block partials are combined
+  // in the same fixed order in every block, so all blocks see the same bits.
+  template <int NV>
+  __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {
+    __shared__ double s_p[NV][16];
+    __shared__ double s_r[NV];
+    // Two partials buffers: a block can start writing the buffer of reduction n+2
+    // only after every block has passed the barrier of reduction n+1, i.e. after
+    // every block has finished reading the buffer of reduction n.
+    double* part = gs.partials + (size_t)(nred & 1u) * gs.nblocks * kMaxPartials;
+    ++nred;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
+      const double w = ((max_mask >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+      if (lane_id() == 0) s_p[k][wave_in_block()] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+      const int k = threadIdx.x;
       const bool isMax = (max_mask >> k) & 1u;
+      const int nw = (blockDim.x + SB_WAVE - 1) / SB_WAVE;
       double acc = 0.0;
       for (int q = 0; q < nw; ++q) acc = isMax ? (s_p[k][q] > acc ? s_p[k][q] : acc) : acc + s_p[k][q];
-      gs.partials[(size_t)blockIdx.x * kMaxPartials + k] = acc;
+      cst(&part[(size_t)blockIdx.x * kMaxPartials + k], acc);
     }
+    if (!sync()) return false;
+    // every wave reduces one partial index over all blocks: lane l takes blocks l, l+64,
+    // l+128, l+192 of each group of 256 (loads issued together), then the xor tree
+    const int nw = blockDim.x / SB_WAVE;
+    for (int k = wave_in_block(); k < NV; k += nw) {
+      const bool isMax = (max_mask >> k) & 1u;
+      double acc = 0.0;
+      for (unsigned b0 = 0; b0 < gs.nblocks; b0 += 4 * SB_WAVE) {
+        double x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned b = b0 + j * SB_WAVE + lane_id();
+          x[j] = (b < gs.nblocks) ? cld(&part[(size_t)b * kMaxPartials + k]) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc = isMax ? (x[j] > acc ? x[j] : acc) : acc + x[j];
+      }
+      acc = isMax ? wave_max(acc) : wave_sum(acc);
+      if (lane_id() == 0) s_r[k] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k] = s_r[k];
+    __syncthreads();
+    return true;
   }
-}
-
-__device__ __forceinline__ double read_reduced(const GridSync& gs, int k) {
-  return __hip_atomic_load(&gs.reduced[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+};
 
 }  // namespace sb

@@ -19,16 +19,17 @@
 namespace sb {
 
 namespace {
+// read-only inputs
 __device__ __forceinline__ void ld3(const double* p, int i, double v[3]) {
   v[0] = p[3 * i];
   v[1] = p[3 * i + 1];
   v[2] = p[3 * i + 2];
 }
+// x is updated inside the kernel: stores are coherent (cst) and every read of a
+// value written by another block follows a barrier (whose acquire dropped stale
+// cached copies), so reads are plain loads.
+__device__ __forceinline__ void cld3(const double* p, int i, double v[3]) { ld3(p, i, v); }
 }  // namespace
-
-struct GSRow {
-  const int* rowptr;  // W expanded row pointers
-};
 
 __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict__ wrowptr) {
   const int wpb = blockDim.x / SB_WAVE;
@@ -41,15 +42,18 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
   const unsigned resMax = a.inf_norm ? 1u : 0u;
   const TileMat<double>& W = a.P.W;
   const size_t nnz = (size_t)W.nblocks;
+  Barrier bar;
+  bar.gs = a.gs;
 
   // y = W x + b ; err = eval(y, x) [and err0 = eval(b, 0) when asked]
   auto eval_phase = [&](bool withZero, double& err, double& err0) -> bool {
     double part[2] = {0.0, 0.0};
-    auto ld = [&](int j, double xc[3]) { ld3(a.x, j, xc); };
-    auto lead = [&](int i, double v[3]) {
+    auto ld = [&](int j, double xc[3]) { cld3(a.x, j, xc); };
+    NoPre nopre;
+    auto lead = [&](int i, double v[3], NoPre::type) {
       double b[3], xi[3];
       ld3(a.P.b, i, b);
-      ld3(a.x, i, xi);
+      cld3(a.x, i, xi);
       const double s = a.P.scal[i], mu = a.P.mu[i];
       const double y[3] = {v[0] + b[0], v[1] + b[1], v[2] + b[2]};
       const double xs[3] = {s * xi[0], s * xi[1], s * xi[2]};
@@ -61,19 +65,18 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
         part[1] = resMax ? fmax(part[1], r0) : part[1] + r0;
       }
     };
-    for (int t = gw; t < W.ntiles; t += nwaves) tile_gather<3, 3, double, double>(W, t, ld, lead, a.carry);
+    for (int t = gw; t < W.ntiles; t += nwaves) tile_gather<3, 3, double, double>(W, t, ld, nopre, lead, a.carry);
     if (W.nlong > 0) {
-      if (!grid_reduce_sync(a.gs, 0, 0)) return false;
+      if (!bar.sync()) return false;
       for (int k = gw; k < W.nlong; k += nwaves) {
         double v[3];
         long_row_sum<3, double, double>(W, k, a.carry, v);
-        if (lane_id() == 0) lead(W.long_row[k], v);
+        if (lane_id() == 0) lead(W.long_row[k], v, NoPre::type{});
       }
     }
-    block_partials<2>(a.gs, part, resMax | (resMax << 1));
-    if (!grid_reduce_sync(a.gs, 2, resMax | (resMax << 1))) return false;
-    err = read_reduced(a.gs, 0);
-    err0 = read_reduced(a.gs, 1);
+    if (!bar.reduce<2>(part, resMax | (resMax << 1))) return false;
+    err = bar.red[0];
+    err0 = bar.red[1];
     if (!a.inf_norm) {
       err /= (1.0 + n);
       err0 /= (1.0 + n);
@@ -87,18 +90,18 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
   if (ok) {
     if (err < errBest) {
       errBest = err;
-      for (int k = tid; k < n3; k += nthreads) a.xbest[k] = a.x[k];
+      for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
     }
     if (a.try_zero && err0 < errBest) {
       errBest = err0;
       for (int k = tid; k < n3; k += nthreads) {
-        a.xbest[k] = 0.0;
-        a.x[k] = 0.0;
+        cst(&a.xbest[k], 0.0);
+        cst(&a.x[k], 0.0);
       }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.hist_cap > 0) a.hist[0] = errBest;
     // x/xbest writes above are consumed by other blocks in the color phases
-    ok = grid_reduce_sync(a.gs, 0, 0);
+    ok = bar.sync();
   }
 
   const double absSkipTol = fmin(a.skip_tol, a.tol);
@@ -114,8 +117,11 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
           a.skip[i] = (unsigned char)(a.skip[i] - 1);
           continue;
         }
-        double lx[3], lb[3];
-        ld3(a.x, i, lx);
+        double lx[3], lb[3], x0[3];
+        cld3(a.x, i, lx);
+        x0[0] = lx[0];
+        x0[1] = lx[1];
+        x0[2] = lx[2];
         const double reg = a.reg[i];
         ld3(a.P.b, i, lb);
         lb[0] -= reg * lx[0];
@@ -125,7 +131,7 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
           const int j = W.bcol[q];
           if (j == i) continue;
           double xj[3];
-          ld3(a.x, j, xj);
+          cld3(a.x, j, xj);
 #pragma unroll
           for (int r = 0; r < 3; ++r)
             lb[r] += W.vals[(size_t)(3 * r + 0) * nnz + q] * xj[0] + W.vals[(size_t)(3 * r + 1) * nnz + q] * xj[1] +
@@ -145,9 +151,10 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
           ldx[2] *= 0.5;
           ++nfail;
         }
-        a.x[3 * i] += ldx[0];
-        a.x[3 * i + 1] += ldx[1];
-        a.x[3 * i + 2] += ldx[2];
+        // x_i was read above (lx before the local solve); write x_i + ldx
+        cst(&a.x[3 * i], x0[0] + ldx[0]);
+        cst(&a.x[3 * i + 1], x0[1] + ldx[1]);
+        cst(&a.x[3 * i + 2], x0[2] + ldx[2]);
         const double s2 = s * s;
         const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
         const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
@@ -155,9 +162,8 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
         if (fmin(nx2, ndx2) < absSkipTol || ndx2 < a.skip_tol * fmin(nx2, ndxRef))
           a.skip[i] = (unsigned char)a.skip_iters;
       }
-      block_partials<1>(a.gs, part, 1u);
-      ok = grid_reduce_sync(a.gs, 1, 1u);
-      if (ok) ndxRef = fmax(ndxRef, read_reduced(a.gs, 0));
+      ok = bar.reduce<1>(part, 1u);
+      if (ok) ndxRef = fmax(ndxRef, bar.red[0]);
     }
     if (!ok) break;
     if (it % a.eval_every == 0) {
@@ -168,7 +174,7 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
       ++nEval;
       if (err < errBest) {
         errBest = err;
-        for (int k = tid; k < n3; k += nthreads) a.xbest[k] = a.x[k];
+        for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
       }
       if (err < a.tol) {
         converged = true;
@@ -176,22 +182,21 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
       }
       ndxRef /= a.eval_every;
       // xbest must be complete before the next sweep modifies x
-      ok = grid_reduce_sync(a.gs, 0, 0);
+      ok = bar.sync();
     }
   }
   // maxIters exhausted: x = x_best (GaussSeidel.impl.hpp:202)
   if (ok && !converged)
-    for (int k = tid; k < n3; k += nthreads) a.x[k] = a.xbest[k];
+    for (int k = tid; k < n3; k += nthreads) cst(&a.x[k], cld(&a.xbest[k]));
   // count local failures over the grid
   {
     double part[1] = {(double)nfail};
-    block_partials<1>(a.gs, part, 0u);
-    const bool ok2 = grid_reduce_sync(a.gs, 1, 0u);
+    const bool ok2 = ok && bar.reduce<1>(part, 0u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       a.out[0] = errBest;
       a.out[1] = (double)(it > a.max_iters ? a.max_iters : it);
       a.out[2] = converged ? 1.0 : 0.0;
-      a.out[3] = ok2 ? read_reduced(a.gs, 0) : -1.0;
+      a.out[3] = ok2 ? bar.red[0] : -1.0;
       a.out[4] = (double)nEval;
       a.out[5] = (ok && ok2) ? 0.0 : 1.0;
     }

@@ -104,7 +104,7 @@ struct VecLoader6 {
 };
 struct Store3 {
   double* y;
-  __device__ void operator()(int i, double v[3]) const {
+  __device__ void operator()(int i, double v[3], NoPre::type = {}) const {
     y[3 * i] = v[0];
     y[3 * i + 1] = v[1];
     y[3 * i + 2] = v[2];
@@ -114,7 +114,7 @@ struct StoreMinv {  // w_k = Minv_k v
   double* w;
   const double* minv;
   int nb;
-  __device__ void operator()(int k, double v[6]) const {
+  __device__ void operator()(int k, double v[6], NoPre::type = {}) const {
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       double acc = 0.0;
@@ -131,7 +131,8 @@ __global__ void __launch_bounds__(256) k_tiles(TileMat<double> A, LD ld, LE le, 
   const int wpb = blockDim.x / SB_WAVE;
   const int t = blockIdx.x * wpb + wave_in_block();
   if (t >= A.ntiles) return;
-  tile_gather<BR, BC, double, double>(A, t, ld, le, carry);
+  NoPre pre;
+  tile_gather<BR, BC, double, double>(A, t, ld, pre, le, carry);
 }
 
 template <int BR, typename LE>
@@ -190,16 +191,15 @@ hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* 
   return hipGetLastError();
 }
 
-__global__ void k_reset_sync(unsigned* count, unsigned* gen, int* error) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    *count = 0u;
-    *gen = 0u;
-    *error = 0;
-  }
+// Zeroes the barrier words before a persistent launch (a kernel, not a memset, so
+// that the sequence can be captured in a graph).
+__global__ void k_reset_sync(unsigned* words, int n, int* error) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) words[k] = 0u;
+  if (threadIdx.x == 0) *error = 0;
 }
 
-hipError_t launch_reset_sync(unsigned* count, unsigned* gen, int* error, hipStream_t s) {
-  k_reset_sync<<<1, 64, 0, s>>>(count, gen, error);
+hipError_t launch_reset_sync(unsigned* words, int n, int* error, hipStream_t s) {
+  k_reset_sync<<<1, 256, 0, s>>>(words, n, error);
   return hipGetLastError();
 }
 

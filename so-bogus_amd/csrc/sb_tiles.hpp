@@ -35,11 +35,19 @@ struct TileMat {
   const int* long_t1 = nullptr;   // [nlong] one past last tile
 };
 
+struct NoPre {  // row prefetch that loads nothing
+  struct type {};
+  __device__ type operator()(int) const { return type{}; }
+};
+
 // Processes tile t of A. load(col, xc[BC]) returns the column vector (in the
-// accumulation type A); lead(row, v[BR]) consumes complete row sums; long-tile
-// partial sums go to carry[t * BR + r].
-template <int BR, int BC, typename T, typename Acc, typename Loader, typename Leader>
-__device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& load, Leader& lead, Acc* carry) {
+// accumulation type Acc). pre(row) is evaluated by every active lane as soon as
+// its row is known (so the row's own data is fetched while the gathers are in
+// flight); lead(row, v[BR], pre) consumes complete row sums in the row's last
+// lane. Partial sums of long tiles go to carry[t * BR + r].
+template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
+__device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& load, Pre& pre, Leader& lead,
+                                            Acc* carry) {
   const int l = lane_id();
   const int q0 = A.tstart[t], q1 = A.tstart[t + 1];
   const int q = q0 + l;
@@ -48,10 +56,13 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& 
   Acc v[BR];
 #pragma unroll
   for (int r = 0; r < BR; ++r) v[r] = Acc(0);
+  decltype(pre(0)) rp{};
   if (act) {
     row = A.brow[q];
+    const int col = A.bcol[q];
+    rp = pre(row);
     Acc xc[BC];
-    load(A.bcol[q], xc);
+    load(col, xc);
     const T* vq = A.vals + q;
     const size_t nb = (size_t)A.nblocks;
 #pragma unroll
@@ -78,9 +89,9 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& 
   if (leader) {
     if (A.tlong[t] >= 0) {
 #pragma unroll
-      for (int r = 0; r < BR; ++r) carry[(size_t)t * BR + r] = v[r];
+      for (int r = 0; r < BR; ++r) cst(&carry[(size_t)t * BR + r], v[r]);
     } else {
-      lead(row, v);
+      lead(row, v, rp);
     }
   }
 }
@@ -94,7 +105,7 @@ __device__ __forceinline__ void long_row_sum(const TileMat<T>& A, int k, const A
   for (int r = 0; r < BR; ++r) v[r] = Acc(0);
   for (int t = t0 + lane_id(); t < t1; t += SB_WAVE) {
 #pragma unroll
-    for (int r = 0; r < BR; ++r) v[r] += carry[(size_t)t * BR + r];
+    for (int r = 0; r < BR; ++r) v[r] += cld(&carry[(size_t)t * BR + r]);
   }
 #pragma unroll
   for (int r = 0; r < BR; ++r) v[r] = wave_sum(v[r]);

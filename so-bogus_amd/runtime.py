@@ -16,7 +16,7 @@ from pathlib import Path
 import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
-LIB_PATH = PKG_DIR / "lib" / "libsbhip.so"
+LIB_PATH = Path(os.environ.get("SOBOGUS_LIB", PKG_DIR / "lib" / "libsbhip.so"))
 HEADER = PKG_DIR.parent / "include" / "sobogus_hip.h"
 
 SB_OK, SB_NOT_CONVERGED = 0, 1
@@ -38,7 +38,7 @@ EXPORTED = [
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_gs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
-    "sb_bench_spmv",
+    "sb_bench_spmv", "sb_last_phase_times",
 ]
 
 
@@ -46,13 +46,14 @@ class PGOptions(C.Structure):
     _fields_ = [("variant", C.c_int), ("law", C.c_int), ("op", C.c_int), ("scaling", C.c_int),
                 ("inf_norm", C.c_int), ("tol", C.c_double), ("max_iters", C.c_int), ("ls_iters", C.c_int),
                 ("ls_optimistic", C.c_double), ("ls_pessimistic", C.c_double), ("ls_armijo", C.c_double),
-                ("grid_blocks", C.c_int)]
+                ("grid_blocks", C.c_int), ("block_threads", C.c_int)]
 
 
 class GSOptions(C.Structure):
     _fields_ = [("law", C.c_int), ("strategy", C.c_int), ("inf_norm", C.c_int), ("try_zero", C.c_int),
                 ("tol", C.c_double), ("max_iters", C.c_int), ("eval_every", C.c_int), ("skip_tol", C.c_double),
-                ("skip_iters", C.c_int), ("auto_regularization", C.c_double), ("grid_blocks", C.c_int)]
+                ("skip_iters", C.c_int), ("auto_regularization", C.c_double), ("grid_blocks", C.c_int),
+                ("block_threads", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -121,6 +122,7 @@ def lib():
         "sb_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
         "sb_synchronize": (C.c_int, [_vp]),
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
+        "sb_last_phase_times": (C.c_int, [_vp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -186,6 +188,12 @@ class Context:
 
     def synchronize(self):
         _check(lib().sb_synchronize(self.h), "sb_synchronize")
+
+    def last_phase_times(self) -> dict:
+        """Phase-time breakdown (us) of the last projected-gradient solve."""
+        out = np.zeros(4)
+        _check(lib().sb_last_phase_times(self.h, _d(out)), "sb_last_phase_times")
+        return dict(op_work=out[0], op_wait=out[1], update_work=out[2], update_wait=out[3])
 
 
 class DeviceArray:
