@@ -1,9 +1,11 @@
 """Benchmark: contact-solves/s of the FP64 APGD Coulomb-friction solve on one MI355X.
 
 Workload: 50,000 contacts between 20,000 rigid bodies (synthetic generator, seed 2),
-assembled Delassus operator W = H Minv H^T, Coulomb law (De Saxce), cold start
-r0 = 0, solved to FB residual 1e-6. One step = one complete solve with the
-problem and r resident in HBM. value = n_contacts * steps / wall time.
+Delassus operator W = H Minv H^T applied matrix-free (H^T, Minv and H on the
+GPU; the residual uses the assembled-W row scaling max(1, tr(W_ii)/3)), Coulomb
+law (De Saxce), cold start r0 = 0, solved to FB residual 1e-6. One step = one
+complete solve with the problem and r resident in HBM.
+value = n_contacts * steps / wall time.
 
 Also reported: the roofline of the dominant kernel (the persistent solver
 kernel) from its HIP-event time and its algorithmic bytes, the stand-alone
@@ -25,6 +27,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+COPY_BYTES = 1 << 30   # calibration copy: 1 GiB read + 1 GiB written, beyond the 256 MB MALL
 
 
 def _load_package():
@@ -72,6 +75,9 @@ def main() -> int:
     ap.add_argument("--workload", default="contacts-50k")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--operator", choices=["matrix-free", "assembled"], default="matrix-free")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for rocprofv3: 1 warm-up + 2 solves + 2 calibration copies, no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -91,7 +97,9 @@ def main() -> int:
     nc, nb, hblocks, wblocks = dp.sizes()
     print(f"[bench] setup (W assembly + upload) {time.perf_counter() - t0:.2f} s, W lower blocks {wblocks}",
           file=sys.stderr, flush=True)
-    opts = R.pg_options(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_ASSEMBLED, tol=args.tol, max_iters=100000)
+    op = R.OP_ASSEMBLED if args.operator == "assembled" else R.OP_MATRIX_FREE
+    opts = R.pg_options(variant=R.PG_APGD, law=R.LAW_COULOMB, op=op, scaling=R.SCALING_DIAGONAL, tol=args.tol,
+                        max_iters=100000)
     r = R.DeviceArray(ctx, 3 * nc)
     zero = np.zeros(3 * nc)
 
@@ -99,6 +107,14 @@ def main() -> int:
         r.upload(zero)
         return dp.solve_pg(r, opts, on_device=True)
 
+    if args.profile_run:
+        for _ in range(3):
+            step()
+        ctx.synchronize()
+        ctx.bench_copy(COPY_BYTES, reps=2)
+        del r, dp
+        ctx.close()
+        return 0
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
@@ -111,7 +127,8 @@ def main() -> int:
     ctx.synchronize()
     wall = time.perf_counter() - t_start
     last = results[-1]
-    final_res = dp.eval_with(r.download(), law=R.LAW_COULOMB)
+    # FB residual of the returned r with the assembled W and its row scaling
+    final_res = dp.eval_with(r.download(), law=R.LAW_COULOMB, op=R.OP_ASSEMBLED)
     ms_per_step = 1000.0 * wall / args.steps
     value = nc * args.steps / wall
     print(f"[bench] {args.steps} solves: {ms_per_step:.2f} ms/solve, {last.iterations} iterations, "
@@ -119,12 +136,16 @@ def main() -> int:
 
     # roofline of the dominant kernel: the persistent solver (one launch per solve)
     k_ms = float(np.mean(kernel_ms))
+    phases = ctx.last_phase_times()
     bytes_wx = wx_assembled_bytes(wblocks, nc)
-    algo_bytes = last.spmvs * bytes_wx
+    bytes_op = bytes_wx if args.operator == "assembled" else wx_matfree_bytes(nc, nb)
+    algo_bytes = last.spmvs * bytes_op
     achieved = algo_bytes / (k_ms * 1e-3) / 1e9
     # stand-alone block SpMV kernels (W x assembled, matrix-free H (Minv H^T x))
     ms_w = dp.bench_spmv(0, reps=50)
     ms_mf = dp.bench_spmv(3, reps=50)
+    ms_copy = ctx.bench_copy(COPY_BYTES, reps=10)
+    copy_gbs = 2 * COPY_BYTES / (ms_copy * 1e-3) / 1e9
     spmv = {
         "assembled_wx": {"ms": ms_w, "bytes": bytes_wx, "GBs": bytes_wx / (ms_w * 1e-3) / 1e9},
         "matrix_free_wx": {"ms": ms_mf, "bytes": wx_matfree_bytes(nc, nb),
@@ -144,14 +165,17 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_contacts": nc, "n_bodies": nb, "solver": "APGD",
-                   "law": "coulomb", "operator": "assembled", "tol": args.tol, "precision": "fp64"},
+                   "law": "coulomb", "operator": args.operator, "tol": args.tol, "precision": "fp64"},
         "solve": {"iterations": last.iterations, "spmvs": last.spmvs, "ls_trials": last.ls_trials,
-                  "residual": last.residual, "residual_recomputed": final_res, "kernel_ms": k_ms},
+                  "nesterov_steps": last.nesterov_steps, "residual": last.residual,
+                  "residual_recomputed": final_res, "kernel_ms": k_ms, "phase_us": phases},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "pg_kernel<false> (persistent APGD)",
-                     "algorithmic_bytes_per_launch": algo_bytes},
+                     "kernel": f"pg_kernel<{'false' if args.operator == 'assembled' else 'true'}> (persistent APGD)",
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "model": "operator applications x compulsory bytes of one W x"},
         "spmv": spmv,
+        "stream_copy_GBs": copy_gbs,
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))

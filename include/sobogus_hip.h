@@ -93,7 +93,7 @@ typedef struct sb_pg_options {
   double ls_pessimistic;  /* 0.5 */
   double ls_armijo;       /* 0.5 (Standard/Conjugated only) */
   int grid_blocks;        /* 0 = automatic; else persistent grid size (clamped) */
-  int block_threads;      /* 0 = automatic; else 256 or 512 threads per block */
+  int block_threads;      /* 0 = automatic (512); else 256 or 512 threads per block */
 } sb_pg_options;
 
 /* GaussSeidel parameters (GaussSeidelBase.hpp:39-91) */
@@ -198,6 +198,9 @@ int sb_synchronize(sb_ctx* ctx);
    context's stream with HIP events; kind: 0 = assembled W x, 1 = H w, 2 = H^T r (+ Minv),
    3 = matrix-free W x (H^T+Minv, then H). Returns the mean milliseconds per launch. */
 int sb_bench_spmv(sb_ctx* ctx, const sb_problem* P, int kind, int reps, double* ms_per_launch);
+/* Times `reps` launches of a plain device-to-device copy of `bytes` (read + write
+   traffic 2 * bytes per launch): achievable-bandwidth and counter calibration. */
+int sb_bench_copy(sb_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
 /* Phase-time breakdown of the last projected-gradient solve on this context, as seen
    by block 0 of the persistent kernel (microseconds): operator-phase work, operator-phase
    reduction + barrier wait, update-phase work, update-phase reduction + barrier wait. */

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -115,7 +116,8 @@ struct sb_ctx {
   hipStream_t stream = nullptr;
   int ncu = 0;
   std::string name;
-  int coop = 0;
+  int coop = 0;          // device supports cooperative launches
+  bool coopLaunch = true; // launch persistent kernels cooperatively (SOBOGUS_LAUNCH=regular: plain launch)
   DBuf<unsigned> sync;  // group counters, top counter, release words
   DBuf<int> error;
   DBuf<double> partials, out;
@@ -271,8 +273,8 @@ int grid_for(sb_ctx* ctx, const void* func, int threads, int requested, int work
   return want;
 }
 
-int block_threads_for(int requested) {
-  if (requested == 0) return kThreads;
+int block_threads_for(int requested, int dflt) {
+  if (requested == 0) return dflt;
   return (requested == 256 || requested == 512) ? requested : -1;
 }
 
@@ -280,6 +282,11 @@ int read_status(sb_ctx* ctx, double* out, int nout) {
   SB_HIP(hipMemcpyAsync(out, ctx->out.p, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
   SB_HIP(hipStreamSynchronize(ctx->stream));
   return SB_OK;
+}
+
+hipError_t launch_persistent(sb_ctx* ctx, const void* fn, int grid, int threads, void** args, hipStream_t s) {
+  if (ctx->coopLaunch) return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+  return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s);
 }
 
 sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
@@ -409,6 +416,10 @@ int sb_ctx_create(sb_ctx** out, int device) {
   c->ncu = prop.multiProcessorCount;
   c->name = prop.gcnArchName;
   SB_HIP(hipDeviceGetAttribute(&c->coop, hipDeviceAttributeCooperativeLaunch, device));
+  // A plain launch of the same (co-resident sized) grid serves profilers that do
+  // not handle cooperative launches; the barriers' bounded waits report a grid
+  // that is not fully resident as SB_ERR_BARRIER_TIMEOUT instead of hanging.
+  if (const char* m = std::getenv("SOBOGUS_LAUNCH")) c->coopLaunch = std::strcmp(m, "regular") != 0;
   SB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   SB_HIP(c->sync.alloc(kSyncWords));
   SB_HIP(c->error.alloc(1));
@@ -768,7 +779,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   if (o->variant != SB_PG_APGD && o->variant != SB_PG_DESCENT && o->variant != SB_PG_SPG) return SB_ERR_UNSUPPORTED;
   if (o->op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
   if (o->max_iters < 1 || o->ls_iters < 1 || !(o->tol >= 0)) return SB_ERR_INVALID;
-  if (!ctx->coop) return SB_ERR_UNSUPPORTED;
+  if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   hipStream_t s = ctx->stream;
   const size_t n3 = 3 * (size_t)P->nc;
   if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
@@ -806,7 +817,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   a.out = ctx->out.p;
   const void* fn = mf ? (const void*)&sb::pg_kernel<true> : (const void*)&sb::pg_kernel<false>;
   const int workWaves = mf ? std::max(P->H.host.ntiles, P->HT.host.ntiles) : P->W.host.ntiles;
-  const int threads = block_threads_for(o->block_threads);
+  const int threads = block_threads_for(o->block_threads, kMaxThreads);
   if (threads < 0) return SB_ERR_INVALID;
   const int grid = grid_for(ctx, fn, threads, o->grid_blocks, workWaves);
   if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
@@ -814,7 +825,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
   void* args[] = {&a};
   SB_HIP(hipEventRecord(ctx->ev0, s));
-  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s));
+  SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
   double out[12];
   if ((rc = read_status(ctx, out, 12))) return rc;
@@ -853,7 +864,7 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (!P->hasW) return SB_ERR_INVALID;
   if (o->max_iters < 0 || o->eval_every < 1 || o->strategy < 0 || o->strategy > 3) return SB_ERR_INVALID;
-  if (!ctx->coop) return SB_ERR_UNSUPPORTED;
+  if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   const int n = P->nc;
   std::vector<int> col;
   if (colors && ncolors > 0) {
@@ -915,7 +926,7 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   const void* fn = (const void*)&sb::gs_kernel;
   int maxColor = 0;
   for (size_t c = 0; c + 1 < col.size(); ++c) maxColor = std::max(maxColor, col[c + 1] - col[c]);
-  const int threads = block_threads_for(o->block_threads);
+  const int threads = block_threads_for(o->block_threads, kThreads);
   if (threads != 256) return threads < 0 ? SB_ERR_INVALID : SB_ERR_UNSUPPORTED;  // GS kernel: 256 threads
   const int grid = grid_for(ctx, fn, threads, o->grid_blocks, std::max(P->W.host.ntiles, (maxColor + 63) / 64));
   if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
@@ -924,7 +935,7 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   const int* wrowptr = P->W.rowptr.p;
   void* args[] = {&a, &wrowptr};
   SB_HIP(hipEventRecord(ctx->ev0, s));
-  SB_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s));
+  SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
   double out[8];
   if ((rc = read_status(ctx, out, 8))) return rc;
@@ -985,6 +996,27 @@ int sb_synchronize(sb_ctx* ctx) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+int sb_bench_copy(sb_ctx* ctx, size_t bytes, int reps, double* ms) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!ms || reps < 1 || bytes < 32) return SB_ERR_INVALID;
+  bytes &= ~(size_t)31;
+  DBuf<char> a, b;
+  SB_HIP(a.alloc(bytes));
+  SB_HIP(b.alloc(bytes));
+  hipStream_t s = ctx->stream;
+  SB_HIP(hipMemsetAsync(a.p, 0, bytes, s));
+  SB_HIP(sb::launch_copy(a.p, b.p, bytes, s));  // warm-up
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  for (int k = 0; k < reps; ++k) SB_HIP(sb::launch_copy(a.p, b.p, bytes, s));
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  SB_HIP(hipEventSynchronize(ctx->ev1));
+  float t = 0.f;
+  SB_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  *ms = (double)t / reps;
   return SB_OK;
 }
 

@@ -7,6 +7,14 @@
 
 #define SB_WAVE 64
 
+// 1: values written by other blocks are read with agent-scope loads (sc1) and the
+//    barrier does not invalidate caches, so read-only matrix data stays in L2
+//    across phases. 0: the barrier invalidates L1/L2 (acquire) and such values
+//    are read with plain cached loads.
+#ifndef SB_COHERENT_GATHER
+#define SB_COHERENT_GATHER 0
+#endif
+
 namespace sb {
 
 // ------------------------------------------------------------------ shuffles
@@ -55,6 +63,14 @@ __device__ __forceinline__ void cst(float* p, float v) {
 }
 __device__ __forceinline__ float cld(const float* p) {
   return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Load of a value another block wrote in an earlier phase of the same kernel.
+__device__ __forceinline__ double mld(const double* p) {
+#if SB_COHERENT_GATHER
+  return cld(p);
+#else
+  return *p;
+#endif
 }
 
 // ------------------------------------------------------------------ barrier
@@ -125,10 +141,12 @@ struct Barrier {
         __builtin_amdgcn_s_sleep(1);
       }
       s_ok = ok;
+#if !SB_COHERENT_GATHER
       // Acquire: drop this CU's L1 and this XCD's L2 copies of lines written by
       // other blocks before the barrier (their coherent stores already reached
       // memory), so that plain cached loads in the next phase read fresh values.
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#endif
     }
     __syncthreads();
     ++epoch;

@@ -28,7 +28,11 @@ __device__ __forceinline__ void ld3(const double* p, int i, double v[3]) {
 // x is updated inside the kernel: stores are coherent (cst) and every read of a
 // value written by another block follows a barrier (whose acquire dropped stale
 // cached copies), so reads are plain loads.
-__device__ __forceinline__ void cld3(const double* p, int i, double v[3]) { ld3(p, i, v); }
+__device__ __forceinline__ void cld3(const double* p, int i, double v[3]) {
+  v[0] = mld(p + 3 * i);
+  v[1] = mld(p + 3 * i + 1);
+  v[2] = mld(p + 3 * i + 2);
+}
 }  // namespace
 
 __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict__ wrowptr) {

@@ -33,12 +33,17 @@ struct Ctl {  // grid geometry of this thread
   int gw, nwaves, tid, nthreads;
 };
 
-// Plain loads: read-only inputs, and kernel-written vectors after the barrier
-// that follows their write (the barrier's acquire invalidated stale copies).
+// Read-only inputs (b, x_io): plain loads.
 __device__ __forceinline__ void ld3(const double* p, int i, double v[3]) {
   v[0] = p[3 * i];
   v[1] = p[3 * i + 1];
   v[2] = p[3 * i + 2];
+}
+// Vectors written by other blocks in earlier phases.
+__device__ __forceinline__ void mld3(const double* p, int i, double v[3]) {
+  v[0] = mld(p + 3 * i);
+  v[1] = mld(p + 3 * i + 1);
+  v[2] = mld(p + 3 * i + 2);
 }
 // Coherent stores of vectors that other blocks read in later phases.
 __device__ __forceinline__ void cst3(double* p, int i, const double v[3]) {
@@ -101,7 +106,7 @@ __device__ __forceinline__ bool apply_op(const PGArgs& a, const Ctl& c, Barrier&
   }
   auto ldB = [&](int k, double xc[6]) {
 #pragma unroll
-    for (int r = 0; r < 6; ++r) xc[r] = a.wbody[6 * k + r];
+    for (int r = 0; r < 6; ++r) xc[r] = mld(&a.wbody[6 * k + r]);
   };
   for (int t = c.gw; t < a.P.H.ntiles; t += c.nwaves)
     tile_gather<3, 6, double, double>(a.P.H, t, ldB, pre, lead, a.carry);
@@ -257,10 +262,10 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
       double Js = 0, decr = 0, nd = 0, decr2 = 0, resN = 0, JN = 0;
       auto pre = [&](int i) {
         TrialRow r;
-        ld3(Xx, i, r.x);
-        ld3(Gx, i, r.g);
-        ld3(a.X[rprev], i, r.p);
-        ld3(a.MX[rx], i, r.m);
+        mld3(Xx, i, r.x);
+        mld3(Gx, i, r.g);
+        mld3(a.X[rprev], i, r.p);
+        mld3(a.MX[rx], i, r.m);
         ld3(a.P.b, i, r.b);
         r.mu = a.P.mu[i];
         r.s = rowScale(i);
@@ -271,8 +276,8 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
         const double al = alpha;
         auto ld = [&](int j, double xc[3]) {
           double xv[3], gv[3];
-          ld3(Xx, j, xv);
-          ld3(Gx, j, gv);
+          mld3(Xx, j, xv);
+          mld3(Gx, j, gv);
           proj_step(a.P.mu[j], al, xv, gv, xc);
         };
         auto lead = [&](int i, double v[3], const TrialRow& r) {
@@ -321,10 +326,10 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
         double part[2] = {0.0, 0.0};
         for (int i = c.tid; i < a.nc; i += c.nthreads) {
           double xs[3], pv[3], Ms[3], Mp[3], xn[3], Mn[3], b[3];
-          ld3(a.X[rxs], i, xs);
-          ld3(a.X[rprev], i, pv);
-          ld3(a.MX[rxs], i, Ms);
-          ld3(a.MX[rprev], i, Mp);
+          mld3(a.X[rxs], i, xs);
+          mld3(a.X[rprev], i, pv);
+          mld3(a.MX[rxs], i, Ms);
+          mld3(a.MX[rprev], i, Mp);
           ld3(a.P.b, i, b);
           for (int k = 0; k < 3; ++k) {
             xn[k] = xs[k] + beta * (xs[k] - pv[k]);
@@ -363,11 +368,11 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
       const double al = alpha;
       auto pre = [&](int i) {
         SpgRow r;
-        ld3(Xx, i, r.x);
-        ld3(Gx, i, r.g);
-        ld3(a.X[rprev], i, r.p);
-        ld3(a.MX[rx], i, r.m);
-        ld3(a.MX[rprev], i, r.mp);
+        mld3(Xx, i, r.x);
+        mld3(Gx, i, r.g);
+        mld3(a.X[rprev], i, r.p);
+        mld3(a.MX[rx], i, r.m);
+        mld3(a.MX[rprev], i, r.mp);
         ld3(a.P.b, i, r.b);
         r.mu = a.P.mu[i];
         r.s = rowScale(i);
@@ -375,8 +380,8 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
       };
       auto ld = [&](int j, double xc[3]) {
         double xv[3], gv[3];
-        ld3(Xx, j, xv);
-        ld3(Gx, j, gv);
+        mld3(Xx, j, xv);
+        mld3(Gx, j, gv);
         proj_step(a.P.mu[j], al, xv, gv, xc);
       };
       auto lead = [&](int i, double v[3], const SpgRow& r) {
@@ -429,10 +434,10 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
         double p2[1] = {0.0};
         for (int i = c.tid; i < a.nc; i += c.nthreads) {
           double xs[3], pv[3], Ms[3], Mp[3], xn[3], Mn[3], b[3];
-          ld3(a.X[rxs], i, xs);
-          ld3(a.X[rprev], i, pv);
-          ld3(a.MX[rxs], i, Ms);
-          ld3(a.MX[rprev], i, Mp);
+          mld3(a.X[rxs], i, xs);
+          mld3(a.X[rprev], i, pv);
+          mld3(a.MX[rxs], i, Ms);
+          mld3(a.MX[rprev], i, Mp);
           ld3(a.P.b, i, b);
           for (int k = 0; k < 3; ++k) {
             xn[k] = xs[k] + beta * (xs[k] - pv[k]);

@@ -191,6 +191,18 @@ hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* 
   return hipGetLastError();
 }
 
+// Plain streaming copy (bandwidth and counter calibration).
+__global__ void __launch_bounds__(256) k_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / sizeof(double4);
+  k_copy<<<4096, 256, 0, s>>>((const double4*)src, (double4*)dst, n);
+  return hipGetLastError();
+}
+
 // Zeroes the barrier words before a persistent launch (a kernel, not a memset, so
 // that the sequence can be captured in a graph).
 __global__ void k_reset_sync(unsigned* words, int n, int* error) {

@@ -15,6 +15,7 @@ hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* 
                           hipStream_t s);
 hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* scal, const double* u,
                            const double* b, const double* r, double* terms, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_reset_sync(unsigned* words, int n, int* error, hipStream_t s);
 
 template <bool MF>

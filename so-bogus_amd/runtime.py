@@ -38,7 +38,7 @@ EXPORTED = [
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_gs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
-    "sb_bench_spmv", "sb_last_phase_times",
+    "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times",
 ]
 
 
@@ -123,6 +123,7 @@ def lib():
         "sb_synchronize": (C.c_int, [_vp]),
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
         "sb_last_phase_times": (C.c_int, [_vp, _dp]),
+        "sb_bench_copy": (C.c_int, [_vp, C.c_size_t, C.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -188,6 +189,12 @@ class Context:
 
     def synchronize(self):
         _check(lib().sb_synchronize(self.h), "sb_synchronize")
+
+    def bench_copy(self, nbytes: int, reps: int = 10) -> float:
+        """ms per launch of a plain device copy of nbytes (2 * nbytes of traffic)."""
+        ms = C.c_double()
+        _check(lib().sb_bench_copy(self.h, int(nbytes), reps, C.byref(ms)), "sb_bench_copy")
+        return ms.value
 
     def last_phase_times(self) -> dict:
         """Phase-time breakdown (us) of the last projected-gradient solve."""
