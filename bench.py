@@ -53,6 +53,20 @@ def wx_matfree_bytes(nc: int, nb: int, s: int = 8) -> int:
     return 2 * hx + minv
 
 
+def measured_traffic(key: str, spmvs: int):
+    """HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes.
+
+    profiles/traffic.json holds, per kernel and workload, the calibrated bytes of
+    one profiled launch and its operator-application count; the figure is scaled
+    to this run's count. None when no pass was recorded for this configuration.
+    """
+    try:
+        rec = json.loads((ROOT / "profiles" / "traffic.json").read_text())[key]
+    except (OSError, KeyError, ValueError):
+        return None, None
+    return rec["traffic_bytes"] * spmvs / rec["spmvs"], rec["summary"]
+
+
 def cpu_baseline(prob, tol: float, threads: int) -> dict:
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
@@ -109,7 +123,9 @@ def main() -> int:
 
     if args.profile_run:
         for _ in range(3):
-            step()
+            res = step()
+            print(f"[bench] profile solve: {res.iterations} iterations, {res.spmvs} W x, "
+                  f"residual {res.residual:.3e}, {res.device_ms:.3f} ms", file=sys.stderr, flush=True)
         ctx.synchronize()
         ctx.bench_copy(COPY_BYTES, reps=2)
         del r, dp
@@ -141,6 +157,8 @@ def main() -> int:
     bytes_op = bytes_wx if args.operator == "assembled" else wx_matfree_bytes(nc, nb)
     algo_bytes = last.spmvs * bytes_op
     achieved = algo_bytes / (k_ms * 1e-3) / 1e9
+    kname = f"pg_kernel<{'false' if args.operator == 'assembled' else 'true'}>"
+    traffic, traffic_src = measured_traffic(f"{kname}|{args.workload}|{args.operator}|fp64", last.spmvs)
     # stand-alone block SpMV kernels (W x assembled, matrix-free H (Minv H^T x))
     ms_w = dp.bench_spmv(0, reps=50)
     ms_mf = dp.bench_spmv(3, reps=50)
@@ -170,8 +188,9 @@ def main() -> int:
                   "nesterov_steps": last.nesterov_steps, "residual": last.residual,
                   "residual_recomputed": final_res, "kernel_ms": k_ms, "phase_us": phases},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"pg_kernel<{'false' if args.operator == 'assembled' else 'true'}> (persistent APGD)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": f"{kname} (persistent APGD)",
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "model": "operator applications x compulsory bytes of one W x"},
         "spmv": spmv,
