@@ -65,6 +65,13 @@ enum sb_operator { SB_OP_ASSEMBLED = 0, SB_OP_MATRIX_FREE = 1 };
 /* Residual row scaling: reference semantics (max(1, tr(W_ii)/3) for an assembled
    W, 1 for the matrix-free product) or max(1, tr(W_ii)/3) in both modes */
 enum sb_scaling { SB_SCALING_REFERENCE = 0, SB_SCALING_DIAGONAL = 1, SB_SCALING_NONE = 2 };
+/* Arithmetic of the projected-gradient solvers */
+enum sb_precision {
+  SB_PRECISION_FP64 = 0,  /* everything FP64 (the reference's arithmetic) */
+  SB_PRECISION_MIXED = 1  /* FP32 matrices, iterates, products and projection; FP64 residual,
+                             dot products and decisions; then an FP64 solve warm-started from
+                             the FP32 result verifies the tolerance and refines if needed */
+};
 /* Problem creation flags */
 enum sb_problem_flags { SB_PROBLEM_ASSEMBLE_W = 1 };
 /* BSR flags (bogus::flags) */
@@ -94,6 +101,8 @@ typedef struct sb_pg_options {
   double ls_armijo;       /* 0.5 (Standard/Conjugated only) */
   int grid_blocks;        /* 0 = automatic; else persistent grid size (clamped) */
   int block_threads;      /* 0 = automatic (512); else 256 or 512 threads per block */
+  int precision;          /* sb_precision, FP64 by default */
+  double mixed_tol_factor;/* mixed: the FP32 stage stops at tol * factor (0.8) */
 } sb_pg_options;
 
 /* GaussSeidel parameters (GaussSeidelBase.hpp:39-91) */
@@ -124,6 +133,9 @@ typedef struct sb_stats {
   double* history;        /* optional caller buffer: residual per evaluation (callback stream) */
   int history_capacity;
   int history_length;     /* written: number of evaluations performed */
+  int refine_iterations;  /* mixed precision: iterations of the FP64 stage (0: verified at once) */
+  int refine_spmvs;       /* mixed precision: FP64 operator applications (verification included) */
+  double mixed_residual;  /* mixed precision: residual reached by the FP32 stage */
 } sb_stats;
 
 const char* sb_status_string(int status);
@@ -184,6 +196,47 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, dou
    colors holds n_colors + 1 offsets (pass NULL / 0 for a single color). */
 int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
                 double* r, int on_device, double* residual, sb_stats* stats);
+
+/* ---------------------------------------------------------------- batches
+   Many independent dual friction problems solved in one persistent launch: each
+   workgroup takes whole problems from a queue, largest first, and solves them
+   alone (APGD / Descent / SPG, DualFrictionProblem::solveWith(pg, r) semantics
+   per problem). The launch sequence is captured once in a hipGraph and replayed.
+   The reference has no batched entry point; this is the device-side equivalent
+   of a loop of solveWith calls over independent problems. */
+typedef struct sb_batch sb_batch;
+typedef struct sb_batch_config {
+  int block_threads;      /* 256, 512 or 1024 threads per workgroup; 0: tuned or default */
+  int blocks_per_cu;      /* workgroups launched per CU (1..8); 0: tuned or default */
+} sb_batch_config;
+typedef struct sb_batch_stats {
+  int n_converged;        /* problems whose FP64 residual is below tol */
+  int total_iterations;   /* summed over problems and stages */
+  int max_iterations;     /* largest per-problem iteration count */
+  int refined_problems;   /* mixed precision: problems whose FP64 stage had to iterate */
+  int64_t total_spmvs;    /* operator applications summed over problems and stages */
+  double device_ms;       /* graph execution time (HIP events) */
+  sb_batch_config config; /* geometry of the main stage */
+  int* iterations;        /* optional caller array [n_problems] */
+  double* residuals;      /* optional caller array [n_problems]: FP64 residual of each result */
+  int* converged;         /* optional caller array [n_problems] */
+} sb_batch_stats;
+
+/* Problems k = 0..n_problems-1 as in sb_problem_create (flags: SB_PROBLEM_ASSEMBLE_W). */
+int sb_batch_create(sb_ctx* ctx, sb_batch** B, int n_problems, const int* n_contacts, const int* n_bodies,
+                    const int32_t* const* h_rowptr, const int32_t* const* h_colidx, const double* const* h_vals,
+                    const double* const* minv, const double* const* b, const double* const* mu, int flags);
+int sb_batch_destroy(sb_batch* B);
+int sb_batch_sizes(const sb_batch* B, int* n_problems, int64_t* total_contacts);
+/* r: the problems' 3*n_contacts vectors concatenated in problem order (warm start in,
+   result out); host or device (on_device) memory. cfg may be NULL. */
+int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, const sb_batch_config* cfg, double* r,
+                   int on_device, sb_batch_stats* stats);
+/* Times every launch geometry the kernels allow on this batch (each capped at
+   probe_iters iterations per problem, r0 = 0), keeps the fastest for later solves
+   of this batch and returns it. */
+int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, int probe_iters, sb_batch_config* best,
+                      double* best_ms);
 
 /* ---------------------------------------------------------------- raw device memory helpers
    (for callers that keep vectors resident in HBM between calls) */

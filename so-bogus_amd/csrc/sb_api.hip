@@ -27,13 +27,14 @@ constexpr int kThreads = 256;     // default block size of the persistent kernel
 constexpr int kMaxThreads = 512;  // __launch_bounds__ of the persistent kernels
 constexpr size_t kSyncWords = 2 * sb::kSyncGroups * sb::kSyncStride + sb::kSyncStride;
 
-#define SB_HIP(expr)                                  \
-  do {                                                \
-    hipError_t e_ = (expr);                           \
-    if (e_ != hipSuccess) {                           \
-      last_error_string() = hipGetErrorString(e_);    \
-      return e_ == hipErrorOutOfMemory ? SB_ERR_OOM : SB_ERR_HIP; \
-    }                                                 \
+#define SB_HIP(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      last_error_string() = std::string(hipGetErrorString(e_)) + " (" #expr ", line " +   \
+                            std::to_string(__LINE__) + ")";                               \
+      return e_ == hipErrorOutOfMemory ? SB_ERR_OOM : SB_ERR_HIP;                         \
+    }                                                                                     \
   } while (0)
 
 std::string& last_error_string() {
@@ -68,9 +69,16 @@ struct DBuf {
   hipError_t upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
 };
 
+std::vector<float> to_float(const std::vector<double>& v) {
+  std::vector<float> f(v.size());
+  for (size_t k = 0; k < v.size(); ++k) f[k] = (float)v[k];
+  return f;
+}
+
 struct DevTiles {
   DBuf<int> tstart, brow, bcol, tlong, long_row, long_t0, long_t1, rowptr;
   DBuf<double> vals;
+  DBuf<float> vals32;  // FP32 copy for the mixed-precision solvers
   sbhost::Tiles host;  // index arrays kept (values dropped after upload)
   hipError_t upload(sbhost::Tiles&& t, hipStream_t s) {
     host = std::move(t);
@@ -84,12 +92,18 @@ struct DevTiles {
     if ((e = long_t1.upload(host.long_t1, s)) != hipSuccess) return e;
     if ((e = rowptr.upload(host.rowptr, s)) != hipSuccess) return e;
     if ((e = vals.upload(host.vals, s)) != hipSuccess) return e;
+    {
+      const std::vector<float> f = to_float(host.vals);
+      if ((e = vals32.upload(f, s)) != hipSuccess) return e;
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    }
     e = hipStreamSynchronize(s);
     std::vector<double>().swap(host.vals);
     return e;
   }
-  TileMat<double> view() const {
-    TileMat<double> m;
+  template <typename T>
+  TileMat<T> viewT(const T* v) const {
+    TileMat<T> m;
     m.nrows = host.nrows;
     m.ncols = host.ncols;
     m.nblocks = host.nblocks;
@@ -99,13 +113,15 @@ struct DevTiles {
     m.brow = brow.p;
     m.bcol = bcol.p;
     m.tlong = tlong.p;
-    m.vals = vals.p;
+    m.vals = v;
     m.nlong = (int)host.long_row.size();
     m.long_row = long_row.p;
     m.long_t0 = long_t0.p;
     m.long_t1 = long_t1.p;
     return m;
   }
+  TileMat<double> view() const { return viewT<double>(vals.p); }
+  TileMat<float> view32() const { return viewT<float>(vals32.p); }
   size_t carryNeed(int br) const { return (size_t)host.ntiles * br; }
 };
 
@@ -145,10 +161,14 @@ struct sb_problem {
   std::vector<double> wv;
   DevTiles H, HT, W;
   DBuf<double> d_minv, d_b, d_mu, d_diag, d_scal, d_ones;
-  // workspace
+  DBuf<float> d_minv32;
+  // workspace (FP64 solvers and kernels)
   DBuf<double> X[3], MX[3], G[2], xbest, wbody, carry, rio, hist, y, terms, local, reg, ndx;
+  // workspace of the FP32 (mixed-precision) projected-gradient solver
+  DBuf<float> X32[3], MX32[3], G32[2], xbest32, wbody32, carry32;
   DBuf<unsigned char> skip;
   DBuf<int> colors;
+  DBuf<double> out;  // [2][16] solver results (FP32 stage, FP64 stage)
   DevProblem<double> view(bool withScaling) const {
     DevProblem<double> v;
     v.nc = nc;
@@ -163,6 +183,47 @@ struct sb_problem {
     v.scal = withScaling ? d_scal.p : d_ones.p;
     v.hasW = hasW;
     return v;
+  }
+  DevProblem<float> view32(bool withScaling) const {
+    DevProblem<float> v;
+    v.nc = nc;
+    v.nb = nb;
+    v.H = H.view32();
+    v.HT = HT.view32();
+    v.W = W.view32();
+    v.minv = d_minv32.p;
+    v.b = d_b.p;
+    v.mu = d_mu.p;
+    v.diag = d_diag.p;
+    v.scal = withScaling ? d_scal.p : d_ones.p;
+    v.hasW = hasW;
+    return v;
+  }
+  // Solver arguments of one problem for storage type T; x_io, hist and out are
+  // set by the caller.
+  void fill(sb::PGProblem<double>& pb, bool withScaling) const {
+    pb.P = view(withScaling);
+    for (int k = 0; k < 3; ++k) {
+      pb.X[k] = X[k].p;
+      pb.MX[k] = MX[k].p;
+    }
+    pb.G[0] = G[0].p;
+    pb.G[1] = G[1].p;
+    pb.xbest = xbest.p;
+    pb.wbody = wbody.p;
+    pb.carry = carry.p;
+  }
+  void fill(sb::PGProblem<float>& pb, bool withScaling) const {
+    pb.P = view32(withScaling);
+    for (int k = 0; k < 3; ++k) {
+      pb.X[k] = X32[k].p;
+      pb.MX[k] = MX32[k].p;
+    }
+    pb.G[0] = G32[0].p;
+    pb.G[1] = G32[1].p;
+    pb.xbest = xbest32.p;
+    pb.wbody = wbody32.p;
+    pb.carry = carry32.p;
   }
 };
 
@@ -210,7 +271,13 @@ int upload_problem(sb_problem* P) {
     sbhost::expandSymmetric(nc, 3, P->wrp.data(), P->wci.data(), P->wv.data(), frp, fci, fv);
     SB_HIP(P->W.upload(sbhost::buildTiles(nc, nc, 3, 3, frp.data(), fci.data(), fv.data()), s));
   }
-  SB_HIP(P->d_minv.upload(minv_element_major(nb, P->minv), s));
+  {
+    const std::vector<double> em = minv_element_major(nb, P->minv);
+    SB_HIP(P->d_minv.upload(em, s));
+    const std::vector<float> em32 = to_float(em);
+    SB_HIP(P->d_minv32.upload(em32, s));
+    SB_HIP(hipStreamSynchronize(s));
+  }
   SB_HIP(P->d_b.upload(P->b, s));
   SB_HIP(P->d_mu.upload(P->mu, s));
   // diagonal blocks: from the assembled W when present, else from H Minv H^T
@@ -245,6 +312,16 @@ int upload_problem(sb_problem* P) {
   SB_HIP(P->wbody.alloc(6 * (size_t)nb));
   size_t carry = std::max({P->H.carryNeed(3), P->HT.carryNeed(6), P->W.carryNeed(3), (size_t)8});
   SB_HIP(P->carry.alloc(carry));
+  for (int k = 0; k < 3; ++k) {
+    SB_HIP(P->X32[k].alloc(n3));
+    SB_HIP(P->MX32[k].alloc(n3));
+  }
+  SB_HIP(P->G32[0].alloc(n3));
+  SB_HIP(P->G32[1].alloc(n3));
+  SB_HIP(P->xbest32.alloc(n3));
+  SB_HIP(P->wbody32.alloc(6 * (size_t)nb));
+  SB_HIP(P->carry32.alloc(carry));
+  SB_HIP(P->out.alloc(32));
   SB_HIP(hipStreamSynchronize(s));
   return SB_OK;
 }
@@ -284,8 +361,19 @@ int read_status(sb_ctx* ctx, double* out, int nout) {
   return SB_OK;
 }
 
+// Persistent kernels need every block resident at once (their grid barrier is
+// a software barrier; no hardware wave-sync resource is used). A cooperative
+// launch guarantees residency. The runtime refuses it with hipErrorIllegalState
+// while another process holds the device's cooperative queue; the same grid,
+// sized to the occupancy limit, is then launched plainly - on an otherwise idle
+// device it is resident, and if it is not, the bounded barrier wait ends the
+// solve with SB_ERR_BARRIER_TIMEOUT instead of hanging.
 hipError_t launch_persistent(sb_ctx* ctx, const void* fn, int grid, int threads, void** args, hipStream_t s) {
-  if (ctx->coopLaunch) return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+  if (ctx->coopLaunch) {
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+    if (e != hipErrorIllegalState) return e;
+    (void)hipGetLastError();
+  }
   return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s);
 }
 
@@ -299,6 +387,45 @@ sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
   g.nblocks = (unsigned)blocks;
   g.timeout_ticks = 100ull * 1000 * 1000 * 20;  // 20 s at 100 MHz
   return g;
+}
+
+int check_pg_options(const sb_pg_options* o, bool hasW) {
+  if (o->variant != SB_PG_APGD && o->variant != SB_PG_DESCENT && o->variant != SB_PG_SPG) return SB_ERR_UNSUPPORTED;
+  if (o->op != SB_OP_ASSEMBLED && o->op != SB_OP_MATRIX_FREE) return SB_ERR_INVALID;
+  if (o->op == SB_OP_ASSEMBLED && !hasW) return SB_ERR_INVALID;
+  if (o->max_iters < 1 || o->ls_iters < 1 || !(o->tol >= 0)) return SB_ERR_INVALID;
+  if (o->precision != SB_PRECISION_FP64 && o->precision != SB_PRECISION_MIXED) return SB_ERR_INVALID;
+  if (o->precision == SB_PRECISION_MIXED && !(o->mixed_tol_factor >= 0 && o->mixed_tol_factor <= 1))
+    return SB_ERR_INVALID;
+  return SB_OK;
+}
+
+sb::PGParams make_params(const sb_pg_options* o, bool mf) {
+  sb::PGParams p;
+  p.variant = o->variant;
+  p.coulomb = o->law == SB_LAW_COULOMB;
+  p.inf_norm = o->inf_norm;
+  const bool useScal = (o->scaling == SB_SCALING_DIAGONAL) || (o->scaling == SB_SCALING_REFERENCE && !mf);
+  p.use_scaling = useScal ? 1 : 0;
+  p.tol = o->tol;
+  p.max_iters = o->max_iters;
+  p.ls_iters = o->ls_iters;
+  p.ls_opt = o->ls_optimistic;
+  p.ls_pess = o->ls_pessimistic;
+  return p;
+}
+
+// Launches one persistent single-problem solver kernel (grid barrier state reset first).
+template <typename Args>
+int launch_pg_grid(sb_ctx* ctx, const void* fn, Args* a, int threads, int requested, int workWaves) {
+  const int grid = grid_for(ctx, fn, threads, requested, workWaves);
+  int rc = ensure_grid_scratch(ctx, grid);
+  if (rc) return rc;
+  a->gs = make_sync(ctx, grid);
+  SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, ctx->stream));
+  void* args[] = {a};
+  SB_HIP(launch_persistent(ctx, fn, grid, threads, args, ctx->stream));
+  return SB_OK;
 }
 
 // smallest eigenvalue of a symmetric 3x3 block (Jacobi rotations)
@@ -351,6 +478,11 @@ __global__ void k_split_row(const int* lrp, const int* lci, const double* lv, in
 
 }  // namespace
 
+namespace {
+int create_problem(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, const int32_t* rp, const int32_t* ci,
+                   const double* hv, const double* minv, const double* b, const double* mu, int flags);
+}  // namespace
+
 extern "C" {
 
 const char* sb_status_string(int status) {
@@ -383,6 +515,8 @@ void sb_pg_options_default(sb_pg_options* o) {
   o->ls_armijo = 0.5;
   o->grid_blocks = 0;
   o->block_threads = 0;
+  o->precision = SB_PRECISION_FP64;
+  o->mixed_tol_factor = 0.8;
 }
 
 void sb_gs_options_default(sb_gs_options* o) {
@@ -546,6 +680,7 @@ int sb_bsr_split_row_multiply(sb_ctx* ctx, const sb_bsr* A, int row, const doubl
   DBuf<double> dx, dr;
   SB_HIP(dx.upload(x, (size_t)3 * A->nbr, s));
   SB_HIP(dr.upload(res3, 3, s));
+  (void)hipGetLastError();
   k_split_row<<<1, 64, 0, s>>>(A->d_lrp.p, A->d_lci.p, A->d_lvals.p, A->nbr, row, dx.p, dr.p);
   SB_HIP(hipGetLastError());
   SB_HIP(hipMemcpyAsync(res3, dr.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -558,14 +693,29 @@ int sb_problem_create(sb_ctx* ctx, sb_problem** out, int nc, int nb, const int32
                       const double* hv, const double* minv, const double* b, const double* mu, int flags) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
-  if (!out || nc <= 0 || nb <= 0 || !rp || !ci || !hv || !minv || !b || !mu) return SB_ERR_INVALID;
+  if (!out) return SB_ERR_INVALID;
+  std::unique_ptr<sb_problem> P;
+  rc = create_problem(ctx, P, nc, nb, rp, ci, hv, minv, b, mu, flags);
+  if (rc) return rc;
+  *out = P.release();
+  return SB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int create_problem(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, const int32_t* rp, const int32_t* ci,
+                   const double* hv, const double* minv, const double* b, const double* mu, int flags) {
+  int rc;
+  if (nc <= 0 || nb <= 0 || !rp || !ci || !hv || !minv || !b || !mu) return SB_ERR_INVALID;
   if (rp[0] != 0) return SB_ERR_INVALID;
   for (int i = 0; i < nc; ++i)
     if (rp[i + 1] < rp[i]) return SB_ERR_INVALID;
   const int nnz = rp[nc];
   for (int p = 0; p < nnz; ++p)
     if (ci[p] < 0 || ci[p] >= nb) return SB_ERR_INVALID;
-  std::unique_ptr<sb_problem> P(new (std::nothrow) sb_problem());
+  P.reset(new (std::nothrow) sb_problem());
   if (!P) return SB_ERR_OOM;
   P->ctx = ctx;
   P->nc = nc;
@@ -582,9 +732,12 @@ int sb_problem_create(sb_ctx* ctx, sb_problem** out, int nc, int nb, const int32
   }
   rc = upload_problem(P.get());
   if (rc) return rc;
-  *out = P.release();
   return SB_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* wrp, const int32_t* wci, const double* wv) {
   int rc = check_ctx(ctx);
@@ -776,84 +929,79 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   if (rc) return rc;
   if (!Pc || !o || !r) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
-  if (o->variant != SB_PG_APGD && o->variant != SB_PG_DESCENT && o->variant != SB_PG_SPG) return SB_ERR_UNSUPPORTED;
-  if (o->op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
-  if (o->max_iters < 1 || o->ls_iters < 1 || !(o->tol >= 0)) return SB_ERR_INVALID;
+  if ((rc = check_pg_options(o, P->hasW))) return rc;
   if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   hipStream_t s = ctx->stream;
   const size_t n3 = 3 * (size_t)P->nc;
   if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
   const int histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
   if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
-
-  sb::PGArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.nc = P->nc;
-  a.nb = P->nb;
-  a.variant = o->variant;
-  a.coulomb = o->law == SB_LAW_COULOMB;
-  a.inf_norm = o->inf_norm;
   const bool mf = (o->op == SB_OP_MATRIX_FREE);
-  const bool useScal = (o->scaling == SB_SCALING_DIAGONAL) || (o->scaling == SB_SCALING_REFERENCE && !mf);
-  a.use_scaling = useScal ? 1 : 0;
-  a.tol = o->tol;
-  a.max_iters = o->max_iters;
-  a.ls_iters = o->ls_iters;
-  a.ls_opt = o->ls_optimistic;
-  a.ls_pess = o->ls_pessimistic;
-  a.P = P->view(true);
-  for (int k = 0; k < 3; ++k) {
-    a.X[k] = P->X[k].p;
-    a.MX[k] = P->MX[k].p;
-  }
-  a.G[0] = P->G[0].p;
-  a.G[1] = P->G[1].p;
-  a.xbest = P->xbest.p;
-  a.wbody = P->wbody.p;
-  a.carry = P->carry.p;
-  a.x_io = P->rio.p;
-  a.hist = P->hist.p;
-  a.hist_cap = histCap;
-  a.out = ctx->out.p;
-  const void* fn = mf ? (const void*)&sb::pg_kernel<true> : (const void*)&sb::pg_kernel<false>;
+  const bool mixed = (o->precision == SB_PRECISION_MIXED);
+  const sb::PGParams prm = make_params(o, mf);
   const int workWaves = mf ? std::max(P->H.host.ntiles, P->HT.host.ntiles) : P->W.host.ntiles;
   const int threads = block_threads_for(o->block_threads, kMaxThreads);
   if (threads < 0) return SB_ERR_INVALID;
-  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, workWaves);
-  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
-  a.gs = make_sync(ctx, grid);
-  SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
-  void* args[] = {&a};
+
   SB_HIP(hipEventRecord(ctx->ev0, s));
-  SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
+  if (mixed) {
+    // FP32 stage to tol * factor (no history: the callback stream is the FP64 one)
+    sb::PGGridArgs<float> a;
+    std::memset(&a, 0, sizeof(a));
+    a.prm = prm;
+    a.prm.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
+    P->fill(a.pb, true);
+    a.pb.x_io = P->rio.p;
+    a.pb.out = P->out.p;
+    const void* fn = mf ? (const void*)&sb::pg_kernel<true, float> : (const void*)&sb::pg_kernel<false, float>;
+    if ((rc = launch_pg_grid(ctx, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
+  }
+  {
+    // FP64 solve; in mixed precision it starts from the FP32 result, so it only
+    // verifies the tolerance unless refinement is needed
+    sb::PGGridArgs<double> a;
+    std::memset(&a, 0, sizeof(a));
+    a.prm = prm;
+    P->fill(a.pb, true);
+    a.pb.x_io = P->rio.p;
+    a.pb.hist = P->hist.p;
+    a.pb.hist_cap = histCap;
+    a.pb.out = P->out.p + 16;
+    const void* fn = mf ? (const void*)&sb::pg_kernel<true, double> : (const void*)&sb::pg_kernel<false, double>;
+    if ((rc = launch_pg_grid(ctx, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
+  }
   SB_HIP(hipEventRecord(ctx->ev1, s));
-  double out[12];
-  if ((rc = read_status(ctx, out, 12))) return rc;
+  double out[32];
+  SB_HIP(hipMemcpyAsync(out, P->out.p, sizeof(out), hipMemcpyDeviceToHost, s));
   if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
-  for (int k = 0; k < 4; ++k) ctx->phase_us[k] = out[7 + k];
-  int hl = std::min((int)out[1] + 1, histCap);
+  SB_HIP(hipStreamSynchronize(s));
+  const double* o32 = out;
+  const double* o64 = out + 16;
+  for (int k = 0; k < 4; ++k) ctx->phase_us[k] = o64[7 + k] + (mixed ? o32[7 + k] : 0.0);
   if (st) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    st->iterations = (int)out[1];
-    st->spmvs = (int)out[2];
-    st->ls_trials = (int)out[3];
-    st->nesterov_steps = (int)out[4];
-    st->converged = (int)out[5];
+    st->iterations = (int)o64[1] + (mixed ? (int)o32[1] : 0);
+    st->spmvs = (int)o64[2] + (mixed ? (int)o32[2] : 0);
+    st->ls_trials = (int)o64[3] + (mixed ? (int)o32[3] : 0);
+    st->nesterov_steps = (int)o64[4] + (mixed ? (int)o32[4] : 0);
+    st->converged = (int)o64[5];
     st->local_failures = 0;
-    st->residual = out[0];
+    st->residual = o64[0];
     st->device_ms = ms;
-    st->history_length = (int)out[1] + ((int)out[5] ? 1 : 0);
+    st->history_length = (int)o64[1] + ((int)o64[5] ? 1 : 0);
+    st->refine_iterations = mixed ? (int)o64[1] : 0;
+    st->refine_spmvs = mixed ? (int)o64[2] : 0;
+    st->mixed_residual = mixed ? o32[0] : 0.0;
     if (histCap > 0) {
-      hl = std::min(st->history_length, histCap);
-      if (hl > 0) SB_HIP(hipMemcpyAsync(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost, s));
+      const int hl = std::min(st->history_length, histCap);
+      if (hl > 0) SB_HIP(hipMemcpy(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost));
     }
   }
-  SB_HIP(hipStreamSynchronize(s));
-  if (residual) *residual = out[0];
-  if (out[6] != 0.0) return SB_ERR_BARRIER_TIMEOUT;
-  if (!std::isfinite(out[0])) return SB_ERR_NAN;
-  return out[5] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
+  if (residual) *residual = o64[0];
+  if (o64[6] != 0.0 || (mixed && o32[6] != 0.0)) return SB_ERR_BARRIER_TIMEOUT;
+  if (!std::isfinite(o64[0])) return SB_ERR_NAN;
+  return o64[5] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
 }
 
 int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const int64_t* colors, int ncolors,
@@ -1058,6 +1206,328 @@ int sb_bench_spmv(sb_ctx* ctx, const sb_problem* Pc, int kind, int reps, double*
   float t = 0.f;
   SB_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
   *ms = (double)t / reps;
+  return SB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+
+struct sb_batch {
+  sb_ctx* ctx = nullptr;
+  int n = 0;
+  int64_t totalNc = 0;
+  bool hasW = true;
+  std::vector<std::unique_ptr<sb_problem>> probs;
+  std::vector<int64_t> off;  // offset of problem k in the concatenated r (doubles)
+  std::vector<int> order;    // problems by decreasing work estimate
+  DBuf<double> xio;          // [3 sum n_c]
+  DBuf<double> outs;         // [2][n][16]: FP32 stage, FP64 stage
+  DBuf<sb::PGProblem<float>> d32;
+  DBuf<sb::PGProblem<double>> d64;
+  DBuf<int> d_order;
+  DBuf<unsigned> queue;      // [2] queue positions of the two stages
+  DBuf<int> dummyErr;
+  int descScaling = -1;      // scaling mode the descriptor arrays were built for
+  bool tuned = false;
+  sb_batch_config tunedCfg{0, 0};
+  // captured launch sequence and the key it was captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> graphKey;
+  ~sb_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+template <typename T, int NT>
+const void* batch_fn(bool mf) {
+  return mf ? (const void*)&sb::pg_batch_kernel<true, T, NT> : (const void*)&sb::pg_batch_kernel<false, T, NT>;
+}
+template <typename T>
+const void* batch_fn(bool mf, int threads) {
+  switch (threads) {
+    case 256: return batch_fn<T, 256>(mf);
+    case 512: return batch_fn<T, 512>(mf);
+    case 1024: return batch_fn<T, 1024>(mf);
+    default: return nullptr;
+  }
+}
+
+int batch_occupancy(const void* fn, int threads) {
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, threads, 0) != hipSuccess || perCU < 1) perCU = 1;
+  return perCU;
+}
+
+// Per-problem solver descriptors (device arrays); they depend only on the
+// residual scaling mode.
+int batch_descriptors(sb_batch* B, bool withScaling) {
+  const int mode = withScaling ? 1 : 0;
+  if (B->descScaling == mode) return SB_OK;
+  std::vector<sb::PGProblem<float>> h32(B->n);
+  std::vector<sb::PGProblem<double>> h64(B->n);
+  for (int k = 0; k < B->n; ++k) {
+    std::memset(&h32[k], 0, sizeof(h32[k]));
+    std::memset(&h64[k], 0, sizeof(h64[k]));
+    B->probs[k]->fill(h32[k], withScaling);
+    B->probs[k]->fill(h64[k], withScaling);
+    h32[k].x_io = h64[k].x_io = B->xio.p + B->off[k];
+    h32[k].out = B->outs.p + (size_t)16 * k;
+    h64[k].out = B->outs.p + (size_t)16 * (B->n + k);
+  }
+  // the arrays were allocated at creation: their addresses are captured in the graph
+  hipStream_t s = B->ctx->stream;
+  SB_HIP(hipMemcpyAsync(B->d32.p, h32.data(), sizeof(h32[0]) * B->n, hipMemcpyHostToDevice, s));
+  SB_HIP(hipMemcpyAsync(B->d64.p, h64.data(), sizeof(h64[0]) * B->n, hipMemcpyHostToDevice, s));
+  SB_HIP(hipStreamSynchronize(s));
+  B->descScaling = mode;
+  return SB_OK;
+}
+
+struct BatchPlan {
+  bool mf, mixed;
+  int threads, blocksPerCU, grid;      // main stage
+  int threads64, grid64;               // FP64 stage when it follows an FP32 stage
+  sb::PGParams prm32, prm64;
+};
+
+int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, BatchPlan& pl) {
+  sb_ctx* ctx = B->ctx;
+  pl.mf = (o->op == SB_OP_MATRIX_FREE);
+  pl.mixed = (o->precision == SB_PRECISION_MIXED);
+  sb_batch_config c{0, 0};
+  if (cfg) c = *cfg;
+  if (c.block_threads == 0 && B->tuned) c.block_threads = B->tunedCfg.block_threads;
+  if (c.blocks_per_cu == 0 && B->tuned) c.blocks_per_cu = B->tunedCfg.blocks_per_cu;
+  if (c.block_threads == 0) c.block_threads = 256;
+  const void* fn = pl.mixed ? batch_fn<float>(pl.mf, c.block_threads) : batch_fn<double>(pl.mf, c.block_threads);
+  if (!fn) return SB_ERR_INVALID;
+  if (c.blocks_per_cu == 0) c.blocks_per_cu = batch_occupancy(fn, c.block_threads);
+  if (c.blocks_per_cu < 1 || c.blocks_per_cu > 8) return SB_ERR_INVALID;
+  pl.threads = c.block_threads;
+  pl.blocksPerCU = c.blocks_per_cu;
+  pl.grid = std::max(1, std::min(B->n, c.blocks_per_cu * ctx->ncu));
+  pl.threads64 = 256;
+  pl.grid64 = std::max(1, std::min(B->n, batch_occupancy(batch_fn<double>(pl.mf, 256), 256) * ctx->ncu));
+  pl.prm64 = make_params(o, pl.mf);
+  pl.prm32 = pl.prm64;
+  pl.prm32.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
+  return SB_OK;
+}
+
+// Captures [reset queues, FP32 stage, FP64 stage] (or [reset, FP64 stage]) into
+// the batch's graph; reused while the options and geometry are unchanged.
+int batch_graph(sb_batch* B, const BatchPlan& pl) {
+  std::vector<unsigned char> key(sizeof(BatchPlan));
+  std::memcpy(key.data(), &pl, sizeof(BatchPlan));
+  if (B->exec && key == B->graphKey) return SB_OK;
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+  hipStream_t s = B->ctx->stream;
+  sb::PGBatchArgs<float> a32;
+  sb::PGBatchArgs<double> a64;
+  std::memset(&a32, 0, sizeof(a32));
+  std::memset(&a64, 0, sizeof(a64));
+  a32.prm = pl.prm32;
+  a32.nprob = B->n;
+  a32.order = B->d_order.p;
+  a32.queue = B->queue.p;
+  a32.probs = B->d32.p;
+  a64.prm = pl.prm64;
+  a64.nprob = B->n;
+  a64.order = B->d_order.p;
+  a64.queue = B->queue.p + 1;
+  a64.probs = B->d64.p;
+  SB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = sb::launch_reset_sync(B->queue.p, 2, B->dummyErr.p, s);
+  if (e == hipSuccess && pl.mixed) {
+    void* args[] = {&a32};
+    e = hipLaunchKernel(batch_fn<float>(pl.mf, pl.threads), dim3(pl.grid), dim3(pl.threads), args, 0, s);
+  }
+  if (e == hipSuccess) {
+    void* args[] = {&a64};
+    const int th = pl.mixed ? pl.threads64 : pl.threads;
+    const int gr = pl.mixed ? pl.grid64 : pl.grid;
+    e = hipLaunchKernel(batch_fn<double>(pl.mf, th), dim3(gr), dim3(th), args, 0, s);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(s, &g);
+  SB_HIP(e);
+  SB_HIP(e2);
+  B->graph = g;
+  SB_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  B->graphKey = key;
+  return SB_OK;
+}
+
+int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, double* r, int on_device,
+              sb_batch_stats* st) {
+  int rc = check_pg_options(o, B->hasW);
+  if (rc) return rc;
+  BatchPlan pl;
+  std::memset(&pl, 0, sizeof(pl));
+  if ((rc = batch_plan(B, o, cfg, pl))) return rc;
+  const bool useScal = (o->scaling == SB_SCALING_DIAGONAL) || (o->scaling == SB_SCALING_REFERENCE && !pl.mf);
+  if ((rc = batch_descriptors(B, useScal))) return rc;
+  if ((rc = batch_graph(B, pl))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t n3 = 3 * (size_t)B->totalNc;
+  if (r) {
+    if ((rc = copy_in(ctx, B->xio.p, r, n3, on_device))) return rc;
+  } else {
+    SB_HIP(hipMemsetAsync(B->xio.p, 0, n3 * sizeof(double), s));
+  }
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  SB_HIP(hipGraphLaunch(B->exec, s));
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  std::vector<double> outs((size_t)32 * B->n);
+  SB_HIP(hipMemcpyAsync(outs.data(), B->outs.p, sizeof(double) * outs.size(), hipMemcpyDeviceToHost, s));
+  if (r && (rc = copy_out(ctx, r, B->xio.p, n3, on_device))) return rc;
+  SB_HIP(hipStreamSynchronize(s));
+  int nconv = 0, totalIt = 0, maxIt = 0, refined = 0, failed = 0;
+  int64_t spmvs = 0;
+  for (int k = 0; k < B->n; ++k) {
+    const double* o32 = &outs[(size_t)16 * k];
+    const double* o64 = &outs[(size_t)16 * (B->n + k)];
+    const int it = (int)o64[1] + (pl.mixed ? (int)o32[1] : 0);
+    totalIt += it;
+    maxIt = std::max(maxIt, it);
+    spmvs += (int64_t)o64[2] + (pl.mixed ? (int64_t)o32[2] : 0);
+    const bool conv = o64[5] != 0.0;
+    nconv += conv ? 1 : 0;
+    if (pl.mixed && o64[1] > 0) ++refined;
+    if (o64[6] != 0.0 || (pl.mixed && o32[6] != 0.0) || !std::isfinite(o64[0])) ++failed;
+    if (st) {
+      if (st->iterations) st->iterations[k] = it;
+      if (st->residuals) st->residuals[k] = o64[0];
+      if (st->converged) st->converged[k] = conv ? 1 : 0;
+    }
+  }
+  if (st) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    st->n_converged = nconv;
+    st->total_iterations = totalIt;
+    st->max_iterations = maxIt;
+    st->refined_problems = refined;
+    st->total_spmvs = spmvs;
+    st->device_ms = ms;
+    st->config.block_threads = pl.threads;
+    st->config.blocks_per_cu = pl.blocksPerCU;
+  }
+  if (failed) return SB_ERR_NAN;
+  return nconv == B->n ? SB_OK : SB_NOT_CONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const int* nbs, const int32_t* const* rps,
+                    const int32_t* const* cis, const double* const* hvs, const double* const* minvs,
+                    const double* const* bs, const double* const* mus, int flags) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!out || n < 1 || !ncs || !nbs || !rps || !cis || !hvs || !minvs || !bs || !mus) return SB_ERR_INVALID;
+  std::unique_ptr<sb_batch> B(new (std::nothrow) sb_batch());
+  if (!B) return SB_ERR_OOM;
+  B->ctx = ctx;
+  B->n = n;
+  B->hasW = (flags & SB_PROBLEM_ASSEMBLE_W) != 0;
+  B->probs.resize(n);
+  B->off.resize(n);
+  for (int k = 0; k < n; ++k) {
+    if ((rc = create_problem(ctx, B->probs[k], ncs[k], nbs[k], rps[k], cis[k], hvs[k], minvs[k], bs[k], mus[k], flags)))
+      return rc;
+    B->off[k] = 3 * B->totalNc;
+    B->totalNc += ncs[k];
+  }
+  // work estimate: stored blocks of the operator (largest first, stable)
+  std::vector<int64_t> cost(n);
+  for (int k = 0; k < n; ++k) {
+    const sb_problem* P = B->probs[k].get();
+    cost[k] = (int64_t)P->H.host.nblocks * 2 + P->HT.host.nblocks + (B->hasW ? P->W.host.nblocks : 0);
+  }
+  B->order.resize(n);
+  for (int k = 0; k < n; ++k) B->order[k] = k;
+  std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  hipStream_t s = ctx->stream;
+  SB_HIP(B->d_order.upload(B->order, s));
+  SB_HIP(B->xio.alloc(3 * (size_t)B->totalNc));
+  SB_HIP(B->outs.alloc((size_t)32 * n));
+  SB_HIP(B->queue.alloc(2));
+  SB_HIP(B->dummyErr.alloc(1));
+  SB_HIP(B->d32.alloc(n));
+  SB_HIP(B->d64.alloc(n));
+  SB_HIP(hipStreamSynchronize(s));
+  *out = B.release();
+  return SB_OK;
+}
+
+int sb_batch_destroy(sb_batch* B) {
+  if (!B) return SB_ERR_INVALID;
+  (void)hipSetDevice(B->ctx->device);
+  delete B;
+  return SB_OK;
+}
+
+int sb_batch_sizes(const sb_batch* B, int* n, int64_t* total) {
+  if (!B) return SB_ERR_INVALID;
+  if (n) *n = B->n;
+  if (total) *total = B->totalNc;
+  return SB_OK;
+}
+
+int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, double* r,
+                   int on_device, sb_batch_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!B || !o || !r || B->ctx != ctx) return SB_ERR_INVALID;
+  return batch_run(ctx, B, o, cfg, r, on_device, st);
+}
+
+int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int probe_iters, sb_batch_config* best,
+                      double* best_ms) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!B || !o || B->ctx != ctx || probe_iters < 1) return SB_ERR_INVALID;
+  sb_pg_options po = *o;
+  po.max_iters = probe_iters;
+  const bool mf = (o->op == SB_OP_MATRIX_FREE);
+  const bool mixed = (o->precision == SB_PRECISION_MIXED);
+  double bestMs = 1e300;
+  sb_batch_config bestCfg{0, 0};
+  const int threadsList[3] = {256, 512, 1024};
+  for (int th : threadsList) {
+    const void* fn = mixed ? batch_fn<float>(mf, th) : batch_fn<double>(mf, th);
+    const int occ = batch_occupancy(fn, th);
+    for (int bpc = 1; bpc <= std::min(occ, 8); ++bpc) {
+      if (th * bpc > 2048) continue;  // at most 32 waves per CU
+      sb_batch_config c{th, bpc};
+      double ms = 1e300;
+      for (int rep = 0; rep < 2; ++rep) {
+        sb_batch_stats st;
+        std::memset(&st, 0, sizeof(st));
+        rc = batch_run(ctx, B, &po, &c, nullptr, 0, &st);
+        if (rc < 0) return rc;
+        ms = std::min(ms, st.device_ms);
+      }
+      if (ms < bestMs) {
+        bestMs = ms;
+        bestCfg = c;
+      }
+    }
+  }
+  B->tuned = true;
+  B->tunedCfg = bestCfg;
+  if (best) *best = bestCfg;
+  if (best_ms) *best_ms = bestMs;
   return SB_OK;
 }
 

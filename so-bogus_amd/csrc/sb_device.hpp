@@ -65,7 +65,8 @@ __device__ __forceinline__ float cld(const float* p) {
   return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Load of a value another block wrote in an earlier phase of the same kernel.
-__device__ __forceinline__ double mld(const double* p) {
+template <typename T>
+__device__ __forceinline__ T mld(const T* p) {
 #if SB_COHERENT_GATHER
   return cld(p);
 #else

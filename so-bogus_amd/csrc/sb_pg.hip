@@ -1,62 +1,76 @@
-// Persistent projected-gradient solver (APGD / Descent / SPG) for one dual
-// friction problem, fused with the per-contact cone projection, De Saxce
-// change of variable and Fischer-Burmeister residual.
+// Projected-gradient solver (APGD / Descent / SPG) for dual friction problems,
+// fused with the per-contact cone projection, De Saxce change of variable and
+// Fischer-Burmeister residual.
 //
 // Algorithm: PgMethod<APGD|Descent> and PgMethod<SPG>
-// (src/Core/BlockSolvers/ProjectedGradient.impl.hpp:74-155, 288-351), with
-// test_residual / guess_alpha / nesterov_inertia (:21-52) and the residual of
-// ConstrainedSolverBase::eval (ConstrainedSolverBase.impl.hpp:21-74).
+// (src/Core/BlockSolvers/ProjectedGradient.impl.hpp:79-155, 291-351), with
+// test_residual / guess_alpha / nesterov_inertia (:22-54) and the residual of
+// ConstrainedSolverBase::eval (ConstrainedSolverBase.impl.hpp:23-74).
 //
-// Every block of a co-resident grid runs the same control flow. Each phase is
-// one pass over the data (a block-SpMV gather with the per-contact work fused
-// into the row leaders, or a per-contact update) followed by a grid barrier
-// that also reduces the phase's dot products deterministically; the decisions
-// (line-search acceptance, restart, stopping) are then taken redundantly by
-// every block from the same reduced values. The line-search candidate
-// xs = P(x - alpha g) is formed on the fly inside the gather.
+// The solver body is written once for a team (the threads solving one problem:
+// a whole persistent grid, or one workgroup of the batched kernel) and a storage
+// type T (double, or float for the mixed-precision bulk). Each phase is one pass
+// over the data (a block-SpMV gather with the per-contact work fused into the row
+// leaders, or a per-contact update) ended by a team reduction of the phase's dot
+// products; the decisions (line-search acceptance, restart, stopping) are then
+// taken redundantly by every member from the same reduced values. The
+// line-search candidate xs = P(x - alpha g) is formed on the fly inside the
+// gather.
+//
+// Precision: with T = float the matrices, the iterates, the products and the
+// projection are FP32; b, mu, the row scalings, the Fischer-Burmeister residual,
+// the energy J and every dot product and decision are FP64 (the FP32 values are
+// widened exactly before use).
 //
 // Differences from the reference's floating-point sequence (same mathematics):
 //  * after a Nesterov step the product M x is formed as
 //    M xs + beta (M xs - M prev) from the two products already computed
 //    (M is linear), instead of a new SpMV;
-//  * dot products are summed in a fixed parallel order.
+//  * dot products are summed in a fixed parallel order;
+//  * the first residual is evaluated before guess_alpha (the reference evaluates
+//    it at iteration 0 just after); a warm start that already meets the
+//    tolerance therefore costs one operator application instead of two.
 #include <hip/hip_runtime.h>
 
 #include "sb_law.hpp"
 #include "sb_problem.hpp"
+#include "sb_team.hpp"
 
 namespace sb {
 
 namespace {
 
-struct Ctl {  // grid geometry of this thread
-  int gw, nwaves, tid, nthreads;
-};
-
-// Read-only inputs (b, x_io): plain loads.
-__device__ __forceinline__ void ld3(const double* p, int i, double v[3]) {
+template <typename T>
+__device__ __forceinline__ void dld3(const double* p, int i, double v[3]) {
   v[0] = p[3 * i];
   v[1] = p[3 * i + 1];
   v[2] = p[3 * i + 2];
 }
-// Vectors written by other blocks in earlier phases.
-__device__ __forceinline__ void mld3(const double* p, int i, double v[3]) {
-  v[0] = mld(p + 3 * i);
-  v[1] = mld(p + 3 * i + 1);
-  v[2] = mld(p + 3 * i + 2);
+template <typename Team, typename T>
+__device__ __forceinline__ void tld3(const T* p, int i, T v[3]) {
+  v[0] = Team::ld(p + 3 * i);
+  v[1] = Team::ld(p + 3 * i + 1);
+  v[2] = Team::ld(p + 3 * i + 2);
 }
-// Coherent stores of vectors that other blocks read in later phases.
-__device__ __forceinline__ void cst3(double* p, int i, const double v[3]) {
-  cst(p + 3 * i, v[0]);
-  cst(p + 3 * i + 1, v[1]);
-  cst(p + 3 * i + 2, v[2]);
+template <typename Team, typename T>
+__device__ __forceinline__ void tst3(T* p, int i, const T v[3]) {
+  Team::st(p + 3 * i, v[0]);
+  Team::st(p + 3 * i + 1, v[1]);
+  Team::st(p + 3 * i + 2, v[2]);
+}
+template <typename T>
+__device__ __forceinline__ void widen3(const T v[3], double d[3]) {
+  d[0] = (double)v[0];
+  d[1] = (double)v[1];
+  d[2] = (double)v[2];
 }
 
 // Minv_k v for the element-major [36][nb] storage.
-__device__ __forceinline__ void minv_apply(const double* minv, int nb, int k, const double v[6], double w[6]) {
+template <typename T>
+__device__ __forceinline__ void minv_apply(const T* minv, int nb, int k, const T v[6], T w[6]) {
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
-    double acc = 0.0;
+    T acc = T(0);
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc += minv[(size_t)(r * 6 + c) * nb + k] * v[c];
     w[r] = acc;
@@ -66,20 +80,19 @@ __device__ __forceinline__ void minv_apply(const double* minv, int nb, int k, co
 // One operator application M x with x given by the loader, finished by the
 // leader on complete contact rows (with the row data fetched by pre). Contains
 // 0 (assembled, no long rows), 1 (matrix-free) or 2 (matrix-free with hub
-// bodies) internal barriers.
-template <bool MF, typename Loader, typename Pre, typename Leader>
-__device__ __forceinline__ bool apply_op(const PGArgs& a, const Ctl& c, Barrier& bar, Loader& ld, Pre& pre,
-                                         Leader& lead) {
+// bodies) internal team barriers.
+template <bool MF, typename T, typename Team, typename Loader, typename Pre, typename Leader>
+__device__ __forceinline__ bool apply_op(const PGProblem<T>& pb, Team& tm, Loader& ld, Pre& pre, Leader& lead) {
+  const DevProblem<T>& P = pb.P;
   if (!MF) {
-    for (int t = c.gw; t < a.P.W.ntiles; t += c.nwaves)
-      tile_gather<3, 3, double, double>(a.P.W, t, ld, pre, lead, a.carry);
-    if (a.P.W.nlong > 0) {
-      if (!bar.sync()) return false;
-      for (int k = c.gw; k < a.P.W.nlong; k += c.nwaves) {
-        double v[3];
-        long_row_sum<3, double, double>(a.P.W, k, a.carry, v);
+    for (int t = tm.gw; t < P.W.ntiles; t += tm.nwaves) tile_gather<3, 3, T, T>(P.W, t, ld, pre, lead, pb.carry);
+    if (P.W.nlong > 0) {
+      if (!tm.sync()) return false;
+      for (int k = tm.gw; k < P.W.nlong; k += tm.nwaves) {
+        T v[3];
+        long_row_sum<3, T, T>(P.W, k, pb.carry, v);
         if (lane_id() == 0) {
-          const int row = a.P.W.long_row[k];
+          const int row = P.W.long_row[k];
           lead(row, v, pre(row));
         }
       }
@@ -87,29 +100,27 @@ __device__ __forceinline__ bool apply_op(const PGArgs& a, const Ctl& c, Barrier&
     return true;
   }
   NoPre nopre;
-  auto leadA = [&](int k, double v[6], NoPre::type) {
-    double w[6];
-    minv_apply(a.P.minv, a.nb, k, v, w);
+  auto leadA = [&](int k, T v[6], NoPre::type) {
+    T w[6];
+    minv_apply<T>(P.minv, P.nb, k, v, w);
 #pragma unroll
-    for (int r = 0; r < 6; ++r) cst(&a.wbody[6 * k + r], w[r]);
+    for (int r = 0; r < 6; ++r) Team::st(&pb.wbody[6 * k + r], w[r]);
   };
-  for (int t = c.gw; t < a.P.HT.ntiles; t += c.nwaves)
-    tile_gather<6, 3, double, double>(a.P.HT, t, ld, nopre, leadA, a.carry);
-  if (!bar.sync()) return false;
-  if (a.P.HT.nlong > 0) {
-    for (int k = c.gw; k < a.P.HT.nlong; k += c.nwaves) {
-      double v[6];
-      long_row_sum<6, double, double>(a.P.HT, k, a.carry, v);
-      if (lane_id() == 0) leadA(a.P.HT.long_row[k], v, NoPre::type{});
+  for (int t = tm.gw; t < P.HT.ntiles; t += tm.nwaves) tile_gather<6, 3, T, T>(P.HT, t, ld, nopre, leadA, pb.carry);
+  if (!tm.sync()) return false;
+  if (P.HT.nlong > 0) {
+    for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {
+      T v[6];
+      long_row_sum<6, T, T>(P.HT, k, pb.carry, v);
+      if (lane_id() == 0) leadA(P.HT.long_row[k], v, NoPre::type{});
     }
-    if (!bar.sync()) return false;
+    if (!tm.sync()) return false;
   }
-  auto ldB = [&](int k, double xc[6]) {
+  auto ldB = [&](int k, T xc[6]) {
 #pragma unroll
-    for (int r = 0; r < 6; ++r) xc[r] = mld(&a.wbody[6 * k + r]);
+    for (int r = 0; r < 6; ++r) xc[r] = Team::ld(&pb.wbody[6 * k + r]);
   };
-  for (int t = c.gw; t < a.P.H.ntiles; t += c.nwaves)
-    tile_gather<3, 6, double, double>(a.P.H, t, ldB, pre, lead, a.carry);
+  for (int t = tm.gw; t < P.H.ntiles; t += tm.nwaves) tile_gather<3, 6, T, T>(P.H, t, ldB, pre, lead, pb.carry);
   return true;
 }
 
@@ -122,11 +133,7 @@ __device__ __forceinline__ double nesterov(double& theta, double q) {
   return (tp - tp2) / (tp2 + theta);
 }
 
-__device__ __forceinline__ void grid_copy(const Ctl& c, double* dst, const double* src, int n) {
-  for (int k = c.tid; k < n; k += c.nthreads) cst(dst + k, cld(src + k));
-}
-
-// Residual contribution, COV, gradient and energy terms at (x_i, Mx_i).
+// Residual contribution, COV, gradient and energy terms at (x_i, Mx_i), FP64.
 __device__ __forceinline__ void eval_contact(bool coulomb, double mu, double s, const double b[3], const double x[3],
                                              const double Mx[3], double& res, double g[3], double& J) {
   const double y[3] = {Mx[0] + b[0], Mx[1] + b[1], Mx[2] + b[2]};
@@ -139,182 +146,200 @@ __device__ __forceinline__ void eval_contact(bool coulomb, double mu, double s, 
   J = x[0] * (0.5 * Mx[0] + b[0] + s0) + x[1] * (0.5 * Mx[1] + b[1]) + x[2] * (0.5 * Mx[2] + b[2]);
 }
 
-__device__ __forceinline__ void proj_step(double mu, double al, const double x[3], const double g[3], double xs[3]) {
+template <typename T>
+__device__ __forceinline__ void proj_step(T mu, T al, const T x[3], const T g[3], T xs[3]) {
   xs[0] = x[0] - al * g[0];
   xs[1] = x[1] - al * g[1];
   xs[2] = x[2] - al * g[2];
-  project<double>(mu, xs[0], xs[1], xs[2]);
+  project<T>(mu, xs[0], xs[1], xs[2]);
 }
 
 struct InitRow {
   double x[3], b[3], mu, s;
 };
+template <typename T>
 struct TrialRow {  // own-contact data of a line-search trial row
-  double x[3], g[3], p[3], m[3], b[3], mu, s;
+  T x[3], g[3], p[3], m[3];
+  double b[3], mu, s;
 };
+template <typename T>
 struct SpgRow {
-  double x[3], g[3], p[3], m[3], mp[3], b[3], mu, s;
+  T x[3], g[3], p[3], m[3], mp[3];
+  double b[3], mu, s;
 };
+
+// Projected warm start of contact i (FP64), rounded to the storage type.
+template <typename T>
+__device__ __forceinline__ void start_point(const PGProblem<T>& pb, int i, T xc[3]) {
+  double v[3];
+  dld3<T>(pb.x_io, i, v);
+  project<double>(pb.P.mu[i], v[0], v[1], v[2]);
+  xc[0] = (T)v[0];
+  xc[1] = (T)v[1];
+  xc[2] = (T)v[2];
+}
 
 }  // namespace
 
-template <bool MF>
-__global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
-  Ctl c;
-  const int wpb = blockDim.x / SB_WAVE;
-  c.gw = blockIdx.x * wpb + wave_in_block();
-  c.nwaves = gridDim.x * wpb;
-  c.tid = blockIdx.x * blockDim.x + threadIdx.x;
-  c.nthreads = gridDim.x * blockDim.x;
-  const int n3 = 3 * a.nc;
-  const bool coulomb = a.coulomb != 0;
-  const unsigned resMax = a.inf_norm ? 1u : 0u;  // bit of the residual partial when it is a max
-  const bool spg = (a.variant == 4);
-  const bool useNesterov = (a.variant == 3 || spg);
+// The solve of one problem by one team. tsum accumulates phase times
+// (s_memrealtime ticks): [0] operator-phase work, [1] operator-phase reduction
+// and wait, [2] update-phase work, [3] update-phase wait.
+template <bool MF, typename T, typename Team>
+__device__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, uint64_t tsum[4]) {
+  const DevProblem<T>& P = pb.P;
+  const int nc = P.nc;
+  const int n3 = 3 * nc;
+  const bool coulomb = prm.coulomb != 0;
+  const unsigned resMax = prm.inf_norm ? 1u : 0u;  // bit of the residual partial when it is a max
+  const bool spg = (prm.variant == 4);
+  const bool useNesterov = (prm.variant == 3 || spg);
   int it = 0, spmvs = 0, lsTrials = 0, nest = 0;
   bool ok = true;
   double minRes = -1.0, res = 0.0, J = 0.0, alpha = 1.0, theta = 1.0;
   int rx = 0, rprev = 0, rg = 0;
-  Barrier bar;
-  bar.gs = a.gs;
-  // phase-time breakdown seen by block 0 (s_memrealtime ticks): [0] operator-phase work,
-  // [1] operator-phase reduction + barrier wait, [2] update-phase work, [3] update-phase wait
-  uint64_t tsum[4] = {0, 0, 0, 0};
   uint64_t tp0 = 0, tp1 = 0;
-  auto rowScale = [&](int i) { return a.use_scaling ? a.P.scal[i] : 1.0; };
+  auto rowScale = [&](int i) { return prm.use_scaling ? P.scal[i] : 1.0; };
 
   auto initPre = [&](int i) {
     InitRow r;
-    ld3(a.x_io, i, r.x);
-    r.mu = a.P.mu[i];
-    project<double>(r.mu, r.x[0], r.x[1], r.x[2]);
-    ld3(a.P.b, i, r.b);
+    T xr[3];
+    start_point<T>(pb, i, xr);
+    widen3<T>(xr, r.x);
+    r.mu = P.mu[i];
+    dld3<T>(P.b, i, r.b);
     r.s = rowScale(i);
     return r;
   };
-  // ---- initial phases: x <- P(x0); alpha from guess_alpha; first residual.
+  // ---- x <- P(x0): M x, gradient and the first residual
   {
-    // M (P(x0) - 1)
     double part[2] = {0.0, 0.0};
-    auto ld = [&](int j, double xc[3]) {
-      double v[3];
-      ld3(a.x_io, j, v);
-      project<double>(a.P.mu[j], v[0], v[1], v[2]);
-      xc[0] = v[0] - 1.0;
-      xc[1] = v[1] - 1.0;
-      xc[2] = v[2] - 1.0;
-    };
-    auto lead = [&](int i, double v[3], const InitRow& r) {
-      part[0] += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-      part[1] += (r.x[0] - 1.0) * (r.x[0] - 1.0) + (r.x[1] - 1.0) * (r.x[1] - 1.0) + (r.x[2] - 1.0) * (r.x[2] - 1.0);
-    };
-    ok = apply_op<MF>(a, c, bar, ld, initPre, lead) && bar.reduce<2>(part, 0u);
-    ++spmvs;
-    if (ok) {
-      const double nMx = bar.red[0], nd = bar.red[1];
-      if (nMx > 1.e-16) alpha = fmin(1.0, nd / nMx);
-    }
-  }
-  if (ok) {
-    // M P(x0), written as x with its gradient and first residual
-    double part[2] = {0.0, 0.0};
-    auto ld = [&](int j, double xc[3]) {
-      ld3(a.x_io, j, xc);
-      project<double>(a.P.mu[j], xc[0], xc[1], xc[2]);
-    };
-    auto lead = [&](int i, double v[3], const InitRow& r) {
-      cst3(a.X[0], i, r.x);
-      cst3(a.MX[0], i, v);
-      double rr, g[3], Ji;
-      eval_contact(coulomb, r.mu, r.s, r.b, r.x, v, rr, g, Ji);
-      cst3(a.G[0], i, g);
+    auto ld = [&](int j, T xc[3]) { start_point<T>(pb, j, xc); };
+    auto lead = [&](int i, T v[3], const InitRow& r) {
+      T xr[3] = {(T)r.x[0], (T)r.x[1], (T)r.x[2]};
+      tst3<Team, T>(pb.X[0], i, xr);
+      tst3<Team, T>(pb.MX[0], i, v);
+      double vd[3], rr, g[3], Ji;
+      widen3<T>(v, vd);
+      eval_contact(coulomb, r.mu, r.s, r.b, r.x, vd, rr, g, Ji);
+      const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
+      tst3<Team, T>(pb.G[0], i, gt);
       part[0] = resMax ? fmax(part[0], rr) : part[0] + rr;
       part[1] += Ji;
     };
-    ok = apply_op<MF>(a, c, bar, ld, initPre, lead) && bar.reduce<2>(part, resMax);
+    ok = apply_op<MF, T>(pb, tm, ld, initPre, lead) && tm.template reduce<2>(part, resMax);
     ++spmvs;
     if (ok) {
-      res = bar.red[0];
-      if (!a.inf_norm) res /= (1.0 + a.nc);
-      J = bar.red[1];
+      res = tm.red[0];
+      if (!prm.inf_norm) res /= (1.0 + nc);
+      J = tm.red[1];
+    }
+  }
+  // ---- guess_alpha: alpha = min(1, |P(x0) - 1|^2 / |M (P(x0) - 1)|^2), only when iterating
+  if (ok && !(res < prm.tol) && prm.max_iters > 0) {
+    double part[2] = {0.0, 0.0};
+    auto ld = [&](int j, T xc[3]) {
+      start_point<T>(pb, j, xc);
+      xc[0] -= T(1);
+      xc[1] -= T(1);
+      xc[2] -= T(1);
+    };
+    auto lead = [&](int i, T v[3], const InitRow& r) {
+      double vd[3];
+      widen3<T>(v, vd);
+      part[0] += vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
+      part[1] += (r.x[0] - 1.0) * (r.x[0] - 1.0) + (r.x[1] - 1.0) * (r.x[1] - 1.0) + (r.x[2] - 1.0) * (r.x[2] - 1.0);
+    };
+    ok = apply_op<MF, T>(pb, tm, ld, initPre, lead) && tm.template reduce<2>(part, 0u);
+    ++spmvs;
+    if (ok) {
+      const double nMx = tm.red[0], nd = tm.red[1];
+      if (nMx > 1.e-16) alpha = fmin(1.0, nd / nMx);
     }
   }
 
   bool converged = false;
   while (ok) {
     // ---- test_residual at the current x (iteration `it`)
-    if (blockIdx.x == 0 && threadIdx.x == 0 && it < a.hist_cap) a.hist[it] = res;
+    if (tm.leader() && it < pb.hist_cap) pb.hist[it] = res;
     if (it == 0 || res < minRes) {
       minRes = res;
-      grid_copy(c, a.xbest, a.X[rx], n3);
+      for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(pb.xbest + k, Team::cload(pb.X[rx] + k));
     }
-    if (res < a.tol) {
+    if (res < prm.tol) {
       converged = true;
       break;
     }
     int rxs = 0;
     while (rxs == rx || rxs == rprev) ++rxs;
-    const double* Xx = a.X[rx];
-    const double* Gx = a.G[rg];
+    const T* Xx = pb.X[rx];
+    const T* Gx = pb.G[rg];
     if (!spg) {
       // ---- APGD / Descent: backtracking line search (ProjectedGradient.impl.hpp:117-133)
-      alpha *= a.ls_opt;
+      alpha *= prm.ls_opt;
       double Js = 0, decr = 0, nd = 0, decr2 = 0, resN = 0, JN = 0;
       auto pre = [&](int i) {
-        TrialRow r;
-        mld3(Xx, i, r.x);
-        mld3(Gx, i, r.g);
-        mld3(a.X[rprev], i, r.p);
-        mld3(a.MX[rx], i, r.m);
-        ld3(a.P.b, i, r.b);
-        r.mu = a.P.mu[i];
+        TrialRow<T> r;
+        tld3<Team, T>(Xx, i, r.x);
+        tld3<Team, T>(Gx, i, r.g);
+        tld3<Team, T>(pb.X[rprev], i, r.p);
+        tld3<Team, T>(pb.MX[rx], i, r.m);
+        dld3<T>(P.b, i, r.b);
+        r.mu = P.mu[i];
         r.s = rowScale(i);
         return r;
       };
-      for (int ls = 0; ls < a.ls_iters && ok; ++ls) {
+      for (int ls = 0; ls < prm.ls_iters && ok; ++ls) {
         double part[6] = {0, 0, 0, 0, 0, 0};
-        const double al = alpha;
-        auto ld = [&](int j, double xc[3]) {
-          double xv[3], gv[3];
-          mld3(Xx, j, xv);
-          mld3(Gx, j, gv);
-          proj_step(a.P.mu[j], al, xv, gv, xc);
+        const T al = (T)alpha;
+        auto ld = [&](int j, T xc[3]) {
+          T xv[3], gv[3];
+          tld3<Team, T>(Xx, j, xv);
+          tld3<Team, T>(Gx, j, gv);
+          proj_step<T>((T)P.mu[j], al, xv, gv, xc);
         };
-        auto lead = [&](int i, double v[3], const TrialRow& r) {
-          double xs[3];
-          proj_step(r.mu, al, r.x, r.g, xs);
-          cst3(a.X[rxs], i, xs);
-          cst3(a.MX[rxs], i, v);
-          const double s0 = cov_normal<double>(coulomb, r.mu, r.m[1] + r.b[1], r.m[2] + r.b[2]);
-          part[0] += xs[0] * (0.5 * v[0] + r.b[0] + s0) + xs[1] * (0.5 * v[1] + r.b[1]) +
-                     xs[2] * (0.5 * v[2] + r.b[2]);
-          const double d[3] = {xs[0] - r.x[0], xs[1] - r.x[1], xs[2] - r.x[2]};
-          part[1] += r.g[0] * d[0] + r.g[1] * d[1] + r.g[2] * d[2];
+        auto lead = [&](int i, T v[3], const TrialRow<T>& r) {
+          T xst[3];
+          proj_step<T>((T)r.mu, al, r.x, r.g, xst);
+          tst3<Team, T>(pb.X[rxs], i, xst);
+          tst3<Team, T>(pb.MX[rxs], i, v);
+          double xs[3], vd[3], x[3], g[3], p[3], m[3];
+          widen3<T>(xst, xs);
+          widen3<T>(v, vd);
+          widen3<T>(r.x, x);
+          widen3<T>(r.g, g);
+          widen3<T>(r.p, p);
+          widen3<T>(r.m, m);
+          const double s0 = cov_normal<double>(coulomb, r.mu, m[1] + r.b[1], m[2] + r.b[2]);
+          part[0] += xs[0] * (0.5 * vd[0] + r.b[0] + s0) + xs[1] * (0.5 * vd[1] + r.b[1]) +
+                     xs[2] * (0.5 * vd[2] + r.b[2]);
+          const double d[3] = {xs[0] - x[0], xs[1] - x[1], xs[2] - x[2]};
+          part[1] += g[0] * d[0] + g[1] * d[1] + g[2] * d[2];
           part[2] += d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-          part[3] += (xs[0] - r.p[0]) * r.g[0] + (xs[1] - r.p[1]) * r.g[1] + (xs[2] - r.p[2]) * r.g[2];
+          part[3] += (xs[0] - p[0]) * g[0] + (xs[1] - p[1]) * g[1] + (xs[2] - p[2]) * g[2];
           double rr, gn[3], Jn;
-          eval_contact(coulomb, r.mu, r.s, r.b, xs, v, rr, gn, Jn);
-          cst3(a.G[1 - rg], i, gn);
+          eval_contact(coulomb, r.mu, r.s, r.b, xs, vd, rr, gn, Jn);
+          const T gt[3] = {(T)gn[0], (T)gn[1], (T)gn[2]};
+          tst3<Team, T>(pb.G[1 - rg], i, gt);
           part[4] = resMax ? fmax(part[4], rr) : part[4] + rr;
           part[5] += Jn;
         };
         tp0 = realtime();
-        ok = apply_op<MF>(a, c, bar, ld, pre, lead);
+        ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
         tp1 = realtime();
-        ok = ok && bar.reduce<6>(part, resMax << 4);
+        ok = ok && tm.template reduce<6>(part, resMax << 4);
         tsum[0] += tp1 - tp0;
         tsum[1] += realtime() - tp1;
         ++spmvs;
         ++lsTrials;
         if (!ok) break;
-        Js = bar.red[0];
-        decr = bar.red[1];
-        nd = bar.red[2];
-        decr2 = bar.red[3];
-        resN = bar.red[4];
-        JN = bar.red[5];
+        Js = tm.red[0];
+        decr = tm.red[1];
+        nd = tm.red[2];
+        decr2 = tm.red[3];
+        resN = tm.red[4];
+        JN = tm.red[5];
         if (Js < J + decr + 0.5 / alpha * nd) break;
-        alpha *= a.ls_pess;
+        alpha *= prm.ls_pess;
       }
       if (!ok) break;
       // ---- Nesterov extrapolation or restart (:135-149)
@@ -324,32 +349,37 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
         while (rout == rxs || rout == rprev) ++rout;
         tp0 = realtime();
         double part[2] = {0.0, 0.0};
-        for (int i = c.tid; i < a.nc; i += c.nthreads) {
-          double xs[3], pv[3], Ms[3], Mp[3], xn[3], Mn[3], b[3];
-          mld3(a.X[rxs], i, xs);
-          mld3(a.X[rprev], i, pv);
-          mld3(a.MX[rxs], i, Ms);
-          mld3(a.MX[rprev], i, Mp);
-          ld3(a.P.b, i, b);
+        for (int i = tm.tid; i < nc; i += tm.nthreads) {
+          T xst[3], pvt[3], Mst[3], Mpt[3];
+          tld3<Team, T>(pb.X[rxs], i, xst);
+          tld3<Team, T>(pb.X[rprev], i, pvt);
+          tld3<Team, T>(pb.MX[rxs], i, Mst);
+          tld3<Team, T>(pb.MX[rprev], i, Mpt);
+          double b[3], xn[3], Mn[3];
+          dld3<T>(P.b, i, b);
+          T xnt[3], Mnt[3];
           for (int k = 0; k < 3; ++k) {
-            xn[k] = xs[k] + beta * (xs[k] - pv[k]);
-            Mn[k] = Ms[k] + beta * (Ms[k] - Mp[k]);
+            xnt[k] = (T)((double)xst[k] + beta * ((double)xst[k] - (double)pvt[k]));
+            Mnt[k] = (T)((double)Mst[k] + beta * ((double)Mst[k] - (double)Mpt[k]));
           }
-          cst3(a.X[rout], i, xn);
-          cst3(a.MX[rout], i, Mn);
+          tst3<Team, T>(pb.X[rout], i, xnt);
+          tst3<Team, T>(pb.MX[rout], i, Mnt);
+          widen3<T>(xnt, xn);
+          widen3<T>(Mnt, Mn);
           double rr, g[3], Ji;
-          eval_contact(coulomb, a.P.mu[i], rowScale(i), b, xn, Mn, rr, g, Ji);
-          cst3(a.G[rg], i, g);
+          eval_contact(coulomb, P.mu[i], rowScale(i), b, xn, Mn, rr, g, Ji);
+          const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
+          tst3<Team, T>(pb.G[rg], i, gt);
           part[0] = resMax ? fmax(part[0], rr) : part[0] + rr;
           part[1] += Ji;
         }
         tp1 = realtime();
-        ok = bar.reduce<2>(part, resMax);
+        ok = tm.template reduce<2>(part, resMax);
         tsum[2] += tp1 - tp0;
         tsum[3] += realtime() - tp1;
         if (!ok) break;
-        res = bar.red[0];
-        J = bar.red[1];
+        res = tm.red[0];
+        J = tm.red[1];
         rx = rout;
         rprev = rxs;
         ++nest;
@@ -361,39 +391,47 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
         J = JN;
         theta = 1.0;
       }
-      if (!a.inf_norm) res /= (1.0 + a.nc);
+      if (!prm.inf_norm) res /= (1.0 + nc);
     } else {
       // ---- SPG (ProjectedGradient.impl.hpp:316-346): one product per iteration.
       double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      const double al = alpha;
+      const T al = (T)alpha;
       auto pre = [&](int i) {
-        SpgRow r;
-        mld3(Xx, i, r.x);
-        mld3(Gx, i, r.g);
-        mld3(a.X[rprev], i, r.p);
-        mld3(a.MX[rx], i, r.m);
-        mld3(a.MX[rprev], i, r.mp);
-        ld3(a.P.b, i, r.b);
-        r.mu = a.P.mu[i];
+        SpgRow<T> r;
+        tld3<Team, T>(Xx, i, r.x);
+        tld3<Team, T>(Gx, i, r.g);
+        tld3<Team, T>(pb.X[rprev], i, r.p);
+        tld3<Team, T>(pb.MX[rx], i, r.m);
+        tld3<Team, T>(pb.MX[rprev], i, r.mp);
+        dld3<T>(P.b, i, r.b);
+        r.mu = P.mu[i];
         r.s = rowScale(i);
         return r;
       };
-      auto ld = [&](int j, double xc[3]) {
-        double xv[3], gv[3];
-        mld3(Xx, j, xv);
-        mld3(Gx, j, gv);
-        proj_step(a.P.mu[j], al, xv, gv, xc);
+      auto ld = [&](int j, T xc[3]) {
+        T xv[3], gv[3];
+        tld3<Team, T>(Xx, j, xv);
+        tld3<Team, T>(Gx, j, gv);
+        proj_step<T>((T)P.mu[j], al, xv, gv, xc);
       };
-      auto lead = [&](int i, double v[3], const SpgRow& r) {
-        double xs[3];
-        proj_step(r.mu, al, r.x, r.g, xs);
-        cst3(a.X[rxs], i, xs);
-        cst3(a.MX[rxs], i, v);
+      auto lead = [&](int i, T v[3], const SpgRow<T>& r) {
+        T xst[3];
+        proj_step<T>((T)r.mu, al, r.x, r.g, xst);
+        tst3<Team, T>(pb.X[rxs], i, xst);
+        tst3<Team, T>(pb.MX[rxs], i, v);
+        double xs[3], vd[3], x[3], g[3], p[3], m[3], mp[3];
+        widen3<T>(xst, xs);
+        widen3<T>(v, vd);
+        widen3<T>(r.x, x);
+        widen3<T>(r.g, g);
+        widen3<T>(r.p, p);
+        widen3<T>(r.m, m);
+        widen3<T>(r.mp, mp);
         double ad = 0, ae = 0, cd = 0, ce = 0, dd = 0, de = 0, ee = 0, d2 = 0;
         for (int k = 0; k < 3; ++k) {
-          const double av = xs[k] - r.x[k], cv = xs[k] - r.p[k];
-          const double dv = v[k] - r.m[k], ev = v[k] - r.mp[k];
-          d2 += cv * r.g[k];
+          const double av = xs[k] - x[k], cv = xs[k] - p[k];
+          const double dv = vd[k] - m[k], ev = vd[k] - mp[k];
+          d2 += cv * g[k];
           ad += av * dv;
           ae += av * ev;
           cd += cv * dv;
@@ -410,86 +448,140 @@ __global__ void __launch_bounds__(512) pg_kernel(PGArgs a) {
         part[5] += de;
         part[6] += ee;
         double rr, gn[3], Jn;
-        eval_contact(coulomb, r.mu, r.s, r.b, xs, v, rr, gn, Jn);
-        cst3(a.G[1 - rg], i, gn);
+        eval_contact(coulomb, r.mu, r.s, r.b, xs, vd, rr, gn, Jn);
+        const T gt[3] = {(T)gn[0], (T)gn[1], (T)gn[2]};
+        tst3<Team, T>(pb.G[1 - rg], i, gt);
         part[7] = resMax ? fmax(part[7], rr) : part[7] + rr;
       };
       tp0 = realtime();
-      ok = apply_op<MF>(a, c, bar, ld, pre, lead);
+      ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
       tp1 = realtime();
-      ok = ok && bar.reduce<8>(part, resMax << 7);
+      ok = ok && tm.template reduce<8>(part, resMax << 7);
       tsum[0] += tp1 - tp0;
       tsum[1] += realtime() - tp1;
       ++spmvs;
       if (!ok) break;
-      const double decr2 = bar.red[0];
+      const double decr2 = tm.red[0];
       double wz, zz;
       if (decr2 < 0.0) {
         const double beta = nesterov(theta, 0.0);
-        wz = bar.red[1] + beta * bar.red[2] + beta * beta * bar.red[3];
-        zz = bar.red[4] + 2.0 * beta * bar.red[5] + beta * beta * bar.red[6];
+        wz = tm.red[1] + beta * tm.red[2] + beta * beta * tm.red[3];
+        zz = tm.red[4] + 2.0 * beta * tm.red[5] + beta * beta * tm.red[6];
         int rout = 0;
         while (rout == rxs || rout == rprev) ++rout;
         tp0 = realtime();
         double p2[1] = {0.0};
-        for (int i = c.tid; i < a.nc; i += c.nthreads) {
-          double xs[3], pv[3], Ms[3], Mp[3], xn[3], Mn[3], b[3];
-          mld3(a.X[rxs], i, xs);
-          mld3(a.X[rprev], i, pv);
-          mld3(a.MX[rxs], i, Ms);
-          mld3(a.MX[rprev], i, Mp);
-          ld3(a.P.b, i, b);
+        for (int i = tm.tid; i < nc; i += tm.nthreads) {
+          T xst[3], pvt[3], Mst[3], Mpt[3];
+          tld3<Team, T>(pb.X[rxs], i, xst);
+          tld3<Team, T>(pb.X[rprev], i, pvt);
+          tld3<Team, T>(pb.MX[rxs], i, Mst);
+          tld3<Team, T>(pb.MX[rprev], i, Mpt);
+          double b[3], xn[3], Mn[3];
+          dld3<T>(P.b, i, b);
+          T xnt[3], Mnt[3];
           for (int k = 0; k < 3; ++k) {
-            xn[k] = xs[k] + beta * (xs[k] - pv[k]);
-            Mn[k] = Ms[k] + beta * (Ms[k] - Mp[k]);
+            xnt[k] = (T)((double)xst[k] + beta * ((double)xst[k] - (double)pvt[k]));
+            Mnt[k] = (T)((double)Mst[k] + beta * ((double)Mst[k] - (double)Mpt[k]));
           }
-          cst3(a.X[rout], i, xn);
-          cst3(a.MX[rout], i, Mn);
+          tst3<Team, T>(pb.X[rout], i, xnt);
+          tst3<Team, T>(pb.MX[rout], i, Mnt);
+          widen3<T>(xnt, xn);
+          widen3<T>(Mnt, Mn);
           double rr, g[3], Ji;
-          eval_contact(coulomb, a.P.mu[i], rowScale(i), b, xn, Mn, rr, g, Ji);
-          cst3(a.G[rg], i, g);
+          eval_contact(coulomb, P.mu[i], rowScale(i), b, xn, Mn, rr, g, Ji);
+          const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
+          tst3<Team, T>(pb.G[rg], i, gt);
           p2[0] = resMax ? fmax(p2[0], rr) : p2[0] + rr;
         }
         tp1 = realtime();
-        ok = bar.reduce<1>(p2, resMax);
+        ok = tm.template reduce<1>(p2, resMax);
         tsum[2] += tp1 - tp0;
         tsum[3] += realtime() - tp1;
         if (!ok) break;
-        res = bar.red[0];
+        res = tm.red[0];
         rx = rout;
         rprev = rxs;
         ++nest;
       } else {
-        wz = bar.red[1];
-        zz = bar.red[4];
+        wz = tm.red[1];
+        zz = tm.red[4];
         rx = rxs;
         rprev = rxs;
         rg = 1 - rg;
-        res = bar.red[7];
+        res = tm.red[7];
         theta = 1.0;
       }
-      if (!a.inf_norm) res /= (1.0 + a.nc);
+      if (!prm.inf_norm) res /= (1.0 + nc);
       alpha = wz / zz;
       alpha = fmin(1.e6, fmax(1.e-6, alpha));
     }
     ++it;
-    if (it >= a.max_iters) break;
+    if (it >= prm.max_iters) break;
   }
-  // x = x_best (ProjectedGradient.impl.hpp:152)
-  if (ok) grid_copy(c, a.x_io, a.xbest, n3);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.out[0] = minRes;
-    a.out[1] = (double)it;
-    a.out[2] = (double)spmvs;
-    a.out[3] = (double)lsTrials;
-    a.out[4] = (double)nest;
-    a.out[5] = converged ? 1.0 : 0.0;
-    a.out[6] = ok ? 0.0 : 1.0;
-    for (int k = 0; k < 4; ++k) a.out[7 + k] = 0.01 * (double)tsum[k];  // microseconds (100 MHz clock)
+  // x = x_best (ProjectedGradient.impl.hpp:152); the same thread wrote xbest[k] or
+  // a barrier separates the writer
+  if (ok)
+    for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(pb.x_io + k, (double)Team::cload(pb.xbest + k));
+  if (tm.leader()) {
+    pb.out[0] = minRes;
+    pb.out[1] = (double)it;
+    pb.out[2] = (double)spmvs;
+    pb.out[3] = (double)lsTrials;
+    pb.out[4] = (double)nest;
+    pb.out[5] = converged ? 1.0 : 0.0;
+    pb.out[6] = ok ? 0.0 : 1.0;
+    for (int k = 0; k < 4; ++k) pb.out[7 + k] = 0.01 * (double)tsum[k];  // microseconds (100 MHz clock)
   }
 }
 
-template __global__ void pg_kernel<false>(PGArgs);
-template __global__ void pg_kernel<true>(PGArgs);
+// ------------------------------------------------------------------ kernels
+
+// One problem on the whole co-resident grid.
+template <bool MF, typename T>
+__global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
+  GridTeam tm(a.gs);
+  uint64_t tsum[4] = {0, 0, 0, 0};
+  pg_body<MF, T, GridTeam>(a.prm, a.pb, tm, tsum);
+}
+
+// Many problems: each workgroup takes the next problem from the queue and solves
+// it alone. Workgroups never wait for each other, so any grid size is valid.
+template <bool MF, typename T, int NT>
+__global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a) {
+  __shared__ int s_next;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int k = __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();
+    if (k >= a.nprob) break;
+    const int p = __builtin_amdgcn_readfirstlane(a.order[k]);
+    BlockTeam tm;
+    uint64_t tsum[4] = {0, 0, 0, 0};
+    pg_body<MF, T, BlockTeam>(a.prm, a.probs[p], tm, tsum);
+    __syncthreads();
+  }
+}
+
+template __global__ void pg_kernel<false, double>(PGGridArgs<double>);
+template __global__ void pg_kernel<true, double>(PGGridArgs<double>);
+template __global__ void pg_kernel<false, float>(PGGridArgs<float>);
+template __global__ void pg_kernel<true, float>(PGGridArgs<float>);
+
+#define SB_BATCH_INST(MF, T, NT) template __global__ void pg_batch_kernel<MF, T, NT>(PGBatchArgs<T>);
+SB_BATCH_INST(false, double, 256)
+SB_BATCH_INST(true, double, 256)
+SB_BATCH_INST(false, float, 256)
+SB_BATCH_INST(true, float, 256)
+SB_BATCH_INST(false, double, 512)
+SB_BATCH_INST(true, double, 512)
+SB_BATCH_INST(false, float, 512)
+SB_BATCH_INST(true, float, 512)
+SB_BATCH_INST(false, double, 1024)
+SB_BATCH_INST(true, double, 1024)
+SB_BATCH_INST(false, float, 1024)
+SB_BATCH_INST(true, float, 1024)
+#undef SB_BATCH_INST
 
 }  // namespace sb

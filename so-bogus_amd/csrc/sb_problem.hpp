@@ -26,9 +26,9 @@ struct DevProblem {
   bool hasW = false;
 };
 
-struct PGArgs {
-  int nc, nb;
-  int variant;   // 1 Descent, 3 APGD, 4 SPG
+// Projected-gradient parameters shared by every problem of a launch.
+struct PGParams {
+  int variant;      // 1 Descent, 3 APGD, 4 SPG
   int coulomb;
   int inf_norm;
   int use_scaling;  // 1: residual row scaling from scal[], 0: all ones
@@ -36,18 +36,42 @@ struct PGArgs {
   int max_iters;
   int ls_iters;
   double ls_opt, ls_pess;
-  DevProblem<double> P;
-  double* X[3];
-  double* MX[3];
-  double* G[2];
-  double* xbest;
-  double* wbody;  // [6 nb]
-  double* carry;
-  double* x_io;   // in: warm start, out: result
+};
+
+// One problem's device data and workspace; T is the storage (and bulk
+// arithmetic) type of the matrices and iterate vectors.
+template <typename T>
+struct PGProblem {
+  DevProblem<T> P;
+  T* X[3];
+  T* MX[3];
+  T* G[2];
+  T* xbest;
+  T* wbody;  // [6 nb]
+  T* carry;
+  double* x_io;  // in: warm start, out: result
   double* hist;
   int hist_cap;
+  double* out;   // [16]: [0] residual [1] iterations [2] spmvs [3] ls trials [4] nesterov steps
+                 // [5] converged [6] failed [7..10] phase times (us)
+};
+
+// Single problem, whole persistent grid.
+template <typename T>
+struct PGGridArgs {
+  PGParams prm;
+  PGProblem<T> pb;
   GridSync gs;
-  double* out;    // [0] residual [1] iterations [2] spmvs [3] ls trials [4] nesterov steps [5] converged
+};
+
+// Batch: workgroups take problems (in the given order) from an atomic queue.
+template <typename T>
+struct PGBatchArgs {
+  PGParams prm;
+  int nprob;
+  const int* order;           // problem indices, most expensive first
+  unsigned* queue;            // [1] next position in order
+  const PGProblem<T>* probs;  // [nprob]
 };
 
 struct GSArgs {

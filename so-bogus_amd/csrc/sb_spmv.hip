@@ -78,6 +78,7 @@ __global__ void k_bsr_generic_long(TileMat<double> A, int br, double alpha, doub
 
 hipError_t launch_bsr_generic(const TileMat<double>& A, int br, int bc, double alpha, const double* x, double beta,
                               double* y, double* carry, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   const int threads = 256, wpb = threads / SB_WAVE;
   if (A.ntiles > 0)
     k_bsr_generic<8><<<(A.ntiles + wpb - 1) / wpb, threads, 0, s>>>(A, br, bc, alpha, x, beta, y, carry);
@@ -153,19 +154,23 @@ static void run_tiles(const TileMat<double>& A, LD ld, LE le, double* carry, hip
 }
 
 hipError_t launch_w(const DevProblem<double>& P, const double* x, double* y, double* carry, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   run_tiles<3, 3>(P.W, VecLoader3{x}, Store3{y}, carry, s);
   return hipGetLastError();
 }
 hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, double* carry, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   run_tiles<3, 6>(P.H, VecLoader6{w}, Store3{y}, carry, s);
   return hipGetLastError();
 }
 hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* w, double* carry, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   run_tiles<6, 3>(P.HT, VecLoader3{r}, StoreMinv{w, P.minv, P.nb}, carry, s);
   return hipGetLastError();
 }
 hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* y, double* wbody, double* carry,
                           hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   const hipError_t e = launch_ht_minv(P, x, wbody, carry, s);
   if (e != hipSuccess) return e;
   return launch_h(P, wbody, y, carry, s);
@@ -187,6 +192,7 @@ __global__ void __launch_bounds__(256) k_fb_terms(int nc, int coulomb, const dou
 
 hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* scal, const double* u,
                            const double* b, const double* r, double* terms, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   if (nc > 0) k_fb_terms<<<(nc + 255) / 256, 256, 0, s>>>(nc, coulomb, mu, scal, u, b, r, terms);
   return hipGetLastError();
 }
@@ -198,6 +204,7 @@ __global__ void __launch_bounds__(256) k_copy(const double4* __restrict__ src, d
 }
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   const size_t n = bytes / sizeof(double4);
   k_copy<<<4096, 256, 0, s>>>((const double4*)src, (double4*)dst, n);
   return hipGetLastError();
@@ -211,6 +218,7 @@ __global__ void k_reset_sync(unsigned* words, int n, int* error) {
 }
 
 hipError_t launch_reset_sync(unsigned* words, int n, int* error, hipStream_t s) {
+  (void)hipGetLastError();  // clear a stale error: report this launch only
   k_reset_sync<<<1, 256, 0, s>>>(words, n, error);
   return hipGetLastError();
 }

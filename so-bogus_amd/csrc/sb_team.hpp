@@ -1,0 +1,107 @@
+// Teams: the set of threads that solves one problem together.
+//
+//  * GridTeam  - every block of a co-resident (persistent) grid; phases are
+//                separated by the fence-free grid barrier and reductions go
+//                through the deterministic grid reduction (Barrier).
+//  * BlockTeam - one workgroup; phases are separated by __syncthreads and
+//                reductions go through LDS. Used by the batched solver, where
+//                every workgroup takes whole problems from a queue.
+//
+// Both expose the same interface to the solver bodies: the wave index and wave
+// count used to stride over tiles, the thread index and thread count used to
+// stride over contacts, sync(), reduce<NV>() leaving the results in red[], and
+// st/ld/cld for vectors that other members of the team read in later phases.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sb_device.hpp"
+
+namespace sb {
+
+struct GridTeam {
+  Barrier bar;
+  int gw, nwaves, tid, nthreads;
+  double red[kMaxPartials];
+  __device__ explicit GridTeam(const GridSync& g) {
+    bar.gs = g;
+    const int wpb = blockDim.x / SB_WAVE;
+    gw = blockIdx.x * wpb + wave_in_block();
+    nwaves = gridDim.x * wpb;
+    tid = blockIdx.x * blockDim.x + threadIdx.x;
+    nthreads = gridDim.x * blockDim.x;
+  }
+  __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
+  __device__ bool sync() { return bar.sync(); }
+  template <int NV>
+  __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {
+    if (!bar.reduce<NV>(v, max_mask)) return false;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k] = bar.red[k];
+    return true;
+  }
+  template <typename T>
+  __device__ static void st(T* p, T v) {
+    cst(p, v);
+  }
+  template <typename T>
+  __device__ static T ld(const T* p) {
+    return mld(p);
+  }
+  template <typename T>
+  __device__ static T cload(const T* p) {
+    return cld(p);
+  }
+};
+
+struct BlockTeam {
+  int gw, nwaves, tid, nthreads;
+  double red[kMaxPartials];
+  __device__ BlockTeam() {
+    gw = wave_in_block();
+    nwaves = blockDim.x / SB_WAVE;
+    tid = threadIdx.x;
+    nthreads = blockDim.x;
+  }
+  __device__ bool leader() const { return threadIdx.x == 0; }
+  // Workgroup barrier; __syncthreads orders global-memory accesses of the
+  // workgroup (all its waves share one CU and its L1).
+  __device__ bool sync() {
+    __syncthreads();
+    return true;
+  }
+  // Fixed-order reduction: xor tree inside each wave, then the per-wave values
+  // in wave order. Every thread ends with the same bits in red[].
+  template <int NV>
+  __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {
+    __shared__ double s_p[NV][16];  // up to 1024 threads
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double w = ((max_mask >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+      if (lane_id() == 0) s_p[k][wave_in_block()] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const bool isMax = (max_mask >> k) & 1u;
+      double acc = 0.0;
+      for (int q = 0; q < nwaves; ++q) acc = isMax ? (s_p[k][q] > acc ? s_p[k][q] : acc) : acc + s_p[k][q];
+      red[k] = acc;
+    }
+    __syncthreads();
+    return true;
+  }
+  template <typename T>
+  __device__ static void st(T* p, T v) {
+    *p = v;
+  }
+  template <typename T>
+  __device__ static T ld(const T* p) {
+    return *p;
+  }
+  template <typename T>
+  __device__ static T cload(const T* p) {
+    return *p;
+  }
+};
+
+}  // namespace sb

@@ -254,3 +254,8 @@ def small_friction_pb():
     P = FrictionProblem(nc=2, nb=2, rowptr=rowptr, colidx=colidx, hvals=hvals, minv=minv.reshape(2, 36),
                         b=b, mu=mu, seed=-1, meta={"dofs": dofs})
     return P, sol
+
+
+def batch(n_problems: int = 512, seed0: int = 4000, lo: int = 64, hi: int = 20000) -> list[FrictionProblem]:
+    """The ragged batch: problem k from generate(nc_k, nb_k, seed0 + k)."""
+    return [generate(nc, nb, seed) for nc, nb, seed in batch_sizes(n_problems, seed0, lo, hi)]

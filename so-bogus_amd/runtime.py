@@ -27,6 +27,7 @@ PURE_NEWTON, PURE_ENUMERATIVE, HYBRID, REV_HYBRID = range(4)
 LAW_SOC, LAW_COULOMB = 0, 1
 OP_ASSEMBLED, OP_MATRIX_FREE = 0, 1
 SCALING_REFERENCE, SCALING_DIAGONAL, SCALING_NONE = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
 COLORING_REFERENCE, COLORING_GREEDY = 0, 1
 BSR_SYMMETRIC = 1
 PROBLEM_ASSEMBLE_W = 1
@@ -38,7 +39,8 @@ EXPORTED = [
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_gs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
-    "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times",
+    "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
+    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune",
 ]
 
 
@@ -46,7 +48,8 @@ class PGOptions(C.Structure):
     _fields_ = [("variant", C.c_int), ("law", C.c_int), ("op", C.c_int), ("scaling", C.c_int),
                 ("inf_norm", C.c_int), ("tol", C.c_double), ("max_iters", C.c_int), ("ls_iters", C.c_int),
                 ("ls_optimistic", C.c_double), ("ls_pessimistic", C.c_double), ("ls_armijo", C.c_double),
-                ("grid_blocks", C.c_int), ("block_threads", C.c_int)]
+                ("grid_blocks", C.c_int), ("block_threads", C.c_int), ("precision", C.c_int),
+                ("mixed_tol_factor", C.c_double)]
 
 
 class GSOptions(C.Structure):
@@ -60,7 +63,19 @@ class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("spmvs", C.c_int), ("ls_trials", C.c_int), ("nesterov_steps", C.c_int),
                 ("converged", C.c_int), ("local_failures", C.c_int), ("residual", C.c_double),
                 ("device_ms", C.c_double), ("history", C.POINTER(C.c_double)), ("history_capacity", C.c_int),
-                ("history_length", C.c_int)]
+                ("history_length", C.c_int), ("refine_iterations", C.c_int), ("refine_spmvs", C.c_int),
+                ("mixed_residual", C.c_double)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("block_threads", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("n_converged", C.c_int), ("total_iterations", C.c_int), ("max_iterations", C.c_int),
+                ("refined_problems", C.c_int), ("total_spmvs", C.c_int64), ("device_ms", C.c_double),
+                ("config", BatchConfig), ("iterations", C.POINTER(C.c_int)), ("residuals", C.POINTER(C.c_double)),
+                ("converged", C.POINTER(C.c_int))]
 
 
 _dp = C.POINTER(C.c_double)
@@ -124,6 +139,14 @@ def lib():
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
         "sb_last_phase_times": (C.c_int, [_vp, _dp]),
         "sb_bench_copy": (C.c_int, [_vp, C.c_size_t, C.c_int, _dp]),
+        "sb_batch_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(_ip), C.POINTER(_ip), C.POINTER(_dp), C.POINTER(_dp),
+                                      C.POINTER(_dp), C.POINTER(_dp), C.c_int]),
+        "sb_batch_destroy": (C.c_int, [_vp]),
+        "sb_batch_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), _lp]),
+        "sb_batch_solve": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), C.POINTER(BatchConfig), _vp, C.c_int,
+                                     C.POINTER(BatchStats)]),
+        "sb_batch_autotune": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), C.c_int, C.POINTER(BatchConfig), _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -299,6 +322,9 @@ class SolveResult:
     local_failures: int
     device_ms: float
     history: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    refine_iterations: int = 0
+    refine_spmvs: int = 0
+    mixed_residual: float = 0.0
 
 
 class DualFrictionProblem:
@@ -375,7 +401,9 @@ class DualFrictionProblem:
         return SolveResult(status=code, residual=res.value, r=rout, iterations=st.iterations, spmvs=st.spmvs,
                            ls_trials=st.ls_trials, nesterov_steps=st.nesterov_steps, converged=bool(st.converged),
                            local_failures=st.local_failures, device_ms=st.device_ms,
-                           history=hist[:min(st.history_length, hist_cap)].copy())
+                           history=hist[:min(st.history_length, hist_cap)].copy(),
+                           refine_iterations=st.refine_iterations, refine_spmvs=st.refine_spmvs,
+                           mixed_residual=st.mixed_residual)
 
     def solve_pg(self, r0, opts: PGOptions | None = None, on_device=False, hist_cap=0, **kw) -> SolveResult:
         """ProjectedGradient::solve through DualFrictionProblem::solveWith(pg, r, staticProblem)."""
@@ -399,6 +427,95 @@ class DualFrictionProblem:
         ms = C.c_double()
         _check(lib().sb_bench_spmv(self.ctx.h, self.h, kind, reps, C.byref(ms)), "sb_bench_spmv")
         return ms.value
+
+
+@dataclass
+class BatchResult:
+    status: int
+    r: np.ndarray | None            # concatenated results (None when solved on device)
+    residuals: np.ndarray           # FP64 residual of each problem's result
+    iterations: np.ndarray
+    converged: np.ndarray
+    n_converged: int
+    total_iterations: int
+    max_iterations: int
+    refined_problems: int
+    total_spmvs: int
+    device_ms: float
+    config: tuple[int, int]
+
+
+class Batch:
+    """Many independent dual friction problems solved in one launch (sb_batch)."""
+
+    def __init__(self, ctx: Context, probs, assemble: bool = False):
+        self.ctx = ctx
+        self.n = len(probs)
+        self.ncs = np.array([p.nc for p in probs], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(3 * self.ncs)])
+        keep = []
+
+        def arr(ctype, items):
+            a = (ctype * self.n)(*items)
+            keep.append(a)
+            return a
+        rps = [_i32(p.rowptr) for p in probs]
+        cis = [_i32(p.colidx) for p in probs]
+        hvs = [_f64(p.hvals).reshape(-1) for p in probs]
+        mis = [_f64(p.minv).reshape(-1) for p in probs]
+        bs = [_f64(p.b) for p in probs]
+        mus = [_f64(p.mu) for p in probs]
+        h = _vp()
+        _check(lib().sb_batch_create(
+            ctx.h, C.byref(h), self.n, arr(C.c_int, [p.nc for p in probs]), arr(C.c_int, [p.nb for p in probs]),
+            arr(_ip, [_i(a) for a in rps]), arr(_ip, [_i(a) for a in cis]), arr(_dp, [_d(a) for a in hvs]),
+            arr(_dp, [_d(a) for a in mis]), arr(_dp, [_d(a) for a in bs]), arr(_dp, [_d(a) for a in mus]),
+            PROBLEM_ASSEMBLE_W if assemble else 0), "sb_batch_create")
+        self.h = h
+        self.total_contacts = int(self.ncs.sum())
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sb_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def split(self, r) -> list[np.ndarray]:
+        return [r[self.offsets[k]:self.offsets[k + 1]] for k in range(self.n)]
+
+    def solve(self, r0=None, opts: PGOptions | None = None, config=None, on_device=False, **kw) -> BatchResult:
+        o = opts if opts is not None else pg_options(**kw)
+        cfg = BatchConfig(*(config or (0, 0)))
+        its = np.zeros(self.n, dtype=np.int32)
+        res = np.zeros(self.n)
+        conv = np.zeros(self.n, dtype=np.int32)
+        st = BatchStats()
+        st.iterations = its.ctypes.data_as(C.POINTER(C.c_int))
+        st.residuals = _d(res)
+        st.converged = conv.ctypes.data_as(C.POINTER(C.c_int))
+        if on_device:
+            rp, rout = r0.ptr, None
+        else:
+            rout = np.zeros(3 * self.total_contacts) if r0 is None else _f64(r0).copy()
+            rp = rout.ctypes.data_as(_vp)
+        code = lib().sb_batch_solve(self.ctx.h, self.h, C.byref(o), C.byref(cfg), rp, 1 if on_device else 0,
+                                    C.byref(st))
+        _check(code, "sb_batch_solve", allow_not_converged=True)
+        return BatchResult(status=code, r=rout, residuals=res, iterations=its, converged=conv.astype(bool),
+                           n_converged=st.n_converged, total_iterations=st.total_iterations,
+                           max_iterations=st.max_iterations, refined_problems=st.refined_problems,
+                           total_spmvs=st.total_spmvs, device_ms=st.device_ms,
+                           config=(st.config.block_threads, st.config.blocks_per_cu))
+
+    def autotune(self, opts: PGOptions | None = None, probe_iters: int = 50, **kw):
+        o = opts if opts is not None else pg_options(**kw)
+        best = BatchConfig()
+        ms = C.c_double()
+        _check(lib().sb_batch_autotune(self.ctx.h, self.h, C.byref(o), probe_iters, C.byref(best), C.byref(ms)),
+               "sb_batch_autotune")
+        return (best.block_threads, best.blocks_per_cu), ms.value
 
 
 def gpu_available() -> bool:
