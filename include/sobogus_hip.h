@@ -103,6 +103,8 @@ typedef struct sb_pg_options {
   int block_threads;      /* 0 = automatic (512); else 256 or 512 threads per block */
   int precision;          /* sb_precision, FP64 by default */
   double mixed_tol_factor;/* mixed: the FP32 stage stops at tol * factor (0.8) */
+  int mixed_stall_iters;  /* mixed: the FP32 stage also stops after this many iterations without a 1%
+                             decrease of its best residual (100; its rounding floor was reached) */
 } sb_pg_options;
 
 /* GaussSeidel parameters (GaussSeidelBase.hpp:39-91) */
@@ -208,6 +210,9 @@ typedef struct sb_batch sb_batch;
 typedef struct sb_batch_config {
   int block_threads;      /* 256, 512 or 1024 threads per workgroup; 0: tuned or default */
   int blocks_per_cu;      /* workgroups launched per CU (1..8); 0: tuned or default */
+  int contacts_per_block; /* team sizing: a problem of n_c contacts is solved by
+                             pow2ceil(n_c / contacts_per_block) workgroups (at most 16);
+                             0: tuned or default (4000); < 0: one workgroup per problem */
 } sb_batch_config;
 typedef struct sb_batch_stats {
   int n_converged;        /* problems whose FP64 residual is below tol */

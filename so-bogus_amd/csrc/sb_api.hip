@@ -397,6 +397,7 @@ int check_pg_options(const sb_pg_options* o, bool hasW) {
   if (o->precision != SB_PRECISION_FP64 && o->precision != SB_PRECISION_MIXED) return SB_ERR_INVALID;
   if (o->precision == SB_PRECISION_MIXED && !(o->mixed_tol_factor >= 0 && o->mixed_tol_factor <= 1))
     return SB_ERR_INVALID;
+  if (o->precision == SB_PRECISION_MIXED && o->mixed_stall_iters < 0) return SB_ERR_INVALID;
   return SB_OK;
 }
 
@@ -412,6 +413,7 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
   p.ls_iters = o->ls_iters;
   p.ls_opt = o->ls_optimistic;
   p.ls_pess = o->ls_pessimistic;
+  p.stall_iters = 0;
   return p;
 }
 
@@ -517,6 +519,7 @@ void sb_pg_options_default(sb_pg_options* o) {
   o->block_threads = 0;
   o->precision = SB_PRECISION_FP64;
   o->mixed_tol_factor = 0.8;
+  o->mixed_stall_iters = 100;
 }
 
 void sb_gs_options_default(sb_gs_options* o) {
@@ -950,6 +953,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
     std::memset(&a, 0, sizeof(a));
     a.prm = prm;
     a.prm.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
+    a.prm.stall_iters = o->mixed_stall_iters;
     P->fill(a.pb, true);
     a.pb.x_io = P->rio.p;
     a.pb.out = P->out.p;
@@ -1219,19 +1223,22 @@ struct sb_batch {
   int64_t totalNc = 0;
   bool hasW = true;
   std::vector<std::unique_ptr<sb_problem>> probs;
-  std::vector<int64_t> off;  // offset of problem k in the concatenated r (doubles)
-  std::vector<int> order;    // problems by decreasing work estimate
-  DBuf<double> xio;          // [3 sum n_c]
-  DBuf<double> outs;         // [2][n][16]: FP32 stage, FP64 stage
+  std::vector<int64_t> off;   // offset of problem k in the concatenated r (doubles)
+  std::vector<int64_t> cost;  // work estimate: stored operator blocks
+  DBuf<double> xio;           // [3 sum n_c]
+  DBuf<double> outs;          // [2][n][16]: FP32 stage, FP64 stage
   DBuf<sb::PGProblem<float>> d32;
   DBuf<sb::PGProblem<double>> d64;
-  DBuf<int> d_order;
-  DBuf<unsigned> queue;      // [2] queue positions of the two stages
-  DBuf<int> dummyErr;
-  int descScaling = -1;      // scaling mode the descriptor arrays were built for
+  DBuf<int> d_order;          // [n] problems grouped by class
+  DBuf<unsigned> syncw;       // [kQueueWords + teams * kTeamSyncWords]: class queues, team barriers
+  DBuf<double> teamPartials;  // [teams][2][kMaxTeamBlocks][kMaxPartials]
+  DBuf<int> teamSlot;         // [teams]
+  DBuf<int> error;
+  int maxTeams = 0;
+  int descScaling = -1;       // scaling mode the descriptor arrays were built for
   bool tuned = false;
-  sb_batch_config tunedCfg{0, 0};
-  // captured launch sequence and the key it was captured for
+  sb_batch_config tunedCfg{0, 0, 0};
+  // captured launch sequence and the plan it was captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<unsigned char> graphKey;
@@ -1242,6 +1249,9 @@ struct sb_batch {
 };
 
 namespace {
+
+constexpr unsigned kQueueWords = 32;  // class queue counters (one 128-byte line)
+constexpr int kDefaultContactsPerBlock = 4000;
 
 template <typename T, int NT>
 const void* batch_fn(bool mf) {
@@ -1290,37 +1300,179 @@ int batch_descriptors(sb_batch* B, bool withScaling) {
 
 struct BatchPlan {
   bool mf, mixed;
-  int threads, blocksPerCU, grid;      // main stage
-  int threads64, grid64;               // FP64 stage when it follows an FP32 stage
+  int threads, blocksPerCU, contactsPerBlock, grid;
+  int nclass, nteams;
+  sb::BatchClass cls[sb::kMaxBatchClasses];
   sb::PGParams prm32, prm64;
 };
 
-int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, BatchPlan& pl) {
+int pow2ceil(int64_t v) {
+  int k = 1;
+  while (k < v && k < (1 << 30)) k <<= 1;
+  return k;
+}
+
+// Splits the problems into classes by team size and the grid into the classes'
+// block ranges, in proportion to each class's work; fills order (problems by
+// class, most expensive first).
+void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
+  const int G = pl.grid;
+  int kcap = 1;  // largest team: a power of two, at most half the grid
+  while (kcap * 2 <= std::min(sb::kMaxTeamBlocks, G / 2)) kcap *= 2;
+  std::vector<int> kOf(B->n);
+  for (int p = 0; p < B->n; ++p) {
+    int k = 1;
+    if (pl.contactsPerBlock > 0) {
+      const int64_t need = (B->probs[p]->nc + pl.contactsPerBlock - 1) / pl.contactsPerBlock;
+      k = std::min(pow2ceil(need), kcap);
+    }
+    kOf[p] = k;
+  }
+  // classes in decreasing team size
+  std::vector<int> ks;
+  for (int k = sb::kMaxTeamBlocks; k >= 1; k >>= 1)
+    for (int p = 0; p < B->n; ++p)
+      if (kOf[p] == k) {
+        ks.push_back(k);
+        break;
+      }
+  const int C = (int)ks.size();
+  std::vector<double> work(C, 0.0);
+  std::vector<int> count(C, 0);
+  for (int p = 0; p < B->n; ++p) {
+    const int c = (int)(std::find(ks.begin(), ks.end(), kOf[p]) - ks.begin());
+    work[c] += (double)B->cost[p];
+    ++count[c];
+  }
+  double total = 0;
+  for (double w : work) total += w;
+  // teams per class: proportional share of the blocks, at least one, at most one per problem
+  std::vector<int> teams(C);
+  int used = 0;
+  for (int c = 0; c < C; ++c) {
+    const int share = (int)std::floor(G * work[c] / total / ks[c]);
+    teams[c] = std::max(1, std::min(share, count[c]));
+    used += teams[c] * ks[c];
+  }
+  while (used > G) {  // shrink the class with the most teams per unit of work
+    int cmax = -1;
+    double best = -1;
+    for (int c = 0; c < C; ++c)
+      if (teams[c] > 1 && teams[c] * ks[c] / work[c] > best) {
+        best = teams[c] * ks[c] / work[c];
+        cmax = c;
+      }
+    if (cmax < 0) break;
+    --teams[cmax];
+    used -= ks[cmax];
+  }
+  for (bool grew = true; grew;) {  // hand leftover blocks to the class with the most work per block
+    grew = false;
+    int cbest = -1;
+    double best = -1;
+    for (int c = 0; c < C; ++c)
+      if (teams[c] < count[c] && used + ks[c] <= G && work[c] / (teams[c] * ks[c]) > best) {
+        best = work[c] / (teams[c] * ks[c]);
+        cbest = c;
+      }
+    if (cbest >= 0) {
+      ++teams[cbest];
+      used += ks[cbest];
+      grew = true;
+    }
+  }
+  order.clear();
+  int b0 = 0, t0 = 0;
+  pl.nclass = C;
+  for (int c = 0; c < C; ++c) {
+    sb::BatchClass& cl = pl.cls[c];
+    cl.b0 = b0;
+    cl.k = ks[c];
+    cl.nteams = teams[c];
+    cl.q0 = (int)order.size();
+    std::vector<int> mine;
+    for (int p = 0; p < B->n; ++p)
+      if (kOf[p] == ks[c]) mine.push_back(p);
+    std::stable_sort(mine.begin(), mine.end(), [&](int a, int b) { return B->cost[a] > B->cost[b]; });
+    order.insert(order.end(), mine.begin(), mine.end());
+    cl.q1 = (int)order.size();
+    cl.team0 = t0;
+    b0 += teams[c] * ks[c];
+    t0 += teams[c];
+  }
+  pl.grid = b0;
+  pl.nteams = t0;
+}
+
+int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, BatchPlan& pl,
+               std::vector<int>& order) {
   sb_ctx* ctx = B->ctx;
   pl.mf = (o->op == SB_OP_MATRIX_FREE);
   pl.mixed = (o->precision == SB_PRECISION_MIXED);
-  sb_batch_config c{0, 0};
+  sb_batch_config c{0, 0, 0};
   if (cfg) c = *cfg;
   if (c.block_threads == 0 && B->tuned) c.block_threads = B->tunedCfg.block_threads;
   if (c.blocks_per_cu == 0 && B->tuned) c.blocks_per_cu = B->tunedCfg.blocks_per_cu;
-  if (c.block_threads == 0) c.block_threads = 256;
-  const void* fn = pl.mixed ? batch_fn<float>(pl.mf, c.block_threads) : batch_fn<double>(pl.mf, c.block_threads);
-  if (!fn) return SB_ERR_INVALID;
-  if (c.blocks_per_cu == 0) c.blocks_per_cu = batch_occupancy(fn, c.block_threads);
+  if (c.contacts_per_block == 0 && B->tuned) c.contacts_per_block = B->tunedCfg.contacts_per_block;
+  if (c.block_threads == 0) c.block_threads = 512;
+  if (c.contacts_per_block == 0) c.contacts_per_block = kDefaultContactsPerBlock;
+  const void* f64 = batch_fn<double>(pl.mf, c.block_threads);
+  const void* f32 = batch_fn<float>(pl.mf, c.block_threads);
+  if (!f64 || !f32) return SB_ERR_INVALID;
+  // teams of several blocks need every block resident: stay within the occupancy of
+  // every kernel the plan launches
+  int occ = batch_occupancy(f64, c.block_threads);
+  if (pl.mixed) occ = std::min(occ, batch_occupancy(f32, c.block_threads));
+  if (c.blocks_per_cu == 0) c.blocks_per_cu = occ;
   if (c.blocks_per_cu < 1 || c.blocks_per_cu > 8) return SB_ERR_INVALID;
+  c.blocks_per_cu = std::min(c.blocks_per_cu, occ);
   pl.threads = c.block_threads;
   pl.blocksPerCU = c.blocks_per_cu;
-  pl.grid = std::max(1, std::min(B->n, c.blocks_per_cu * ctx->ncu));
-  pl.threads64 = 256;
-  pl.grid64 = std::max(1, std::min(B->n, batch_occupancy(batch_fn<double>(pl.mf, 256), 256) * ctx->ncu));
+  pl.contactsPerBlock = c.contacts_per_block;
+  pl.grid = c.blocks_per_cu * ctx->ncu;
+  batch_classes(B, pl, order);
   pl.prm64 = make_params(o, pl.mf);
   pl.prm32 = pl.prm64;
   pl.prm32.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
+  pl.prm32.stall_iters = o->mixed_stall_iters;
   return SB_OK;
 }
 
-// Captures [reset queues, FP32 stage, FP64 stage] (or [reset, FP64 stage]) into
-// the batch's graph; reused while the options and geometry are unchanged.
+int batch_buffers(sb_batch* B, int nteams) {
+  if (nteams <= B->maxTeams) return SB_OK;
+  SB_HIP(B->syncw.alloc(kQueueWords + (size_t)nteams * sb::kTeamSyncWords));
+  SB_HIP(B->teamPartials.alloc((size_t)nteams * 2 * sb::kMaxTeamBlocks * sb::kMaxPartials));
+  SB_HIP(B->teamSlot.alloc(nteams));
+  B->maxTeams = nteams;
+  // the graph holds the old addresses
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+  B->graphKey.clear();
+  return SB_OK;
+}
+
+template <typename T>
+void batch_args(sb::PGBatchArgs<T>& a, const sb_batch* B, const BatchPlan& pl, const sb::PGParams& prm,
+                const sb::PGProblem<T>* probs) {
+  std::memset(&a, 0, sizeof(a));
+  a.prm = prm;
+  a.nprob = B->n;
+  a.order = B->d_order.p;
+  a.probs = probs;
+  a.nclass = pl.nclass;
+  for (int c = 0; c < pl.nclass; ++c) a.cls[c] = pl.cls[c];
+  a.queue = B->syncw.p;
+  a.team_sync = B->syncw.p + kQueueWords;
+  a.team_partials = B->teamPartials.p;
+  a.team_slot = B->teamSlot.p;
+  a.error = B->error.p;
+  a.timeout_ticks = 100ull * 1000 * 1000 * 20;  // 20 s at 100 MHz
+}
+
+// Captures [reset, FP32 stage, reset, FP64 stage] (or [reset, FP64 stage]) into the
+// batch's graph; reused while the plan is unchanged.
 int batch_graph(sb_batch* B, const BatchPlan& pl) {
   std::vector<unsigned char> key(sizeof(BatchPlan));
   std::memcpy(key.data(), &pl, sizeof(BatchPlan));
@@ -1332,29 +1484,21 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
   hipStream_t s = B->ctx->stream;
   sb::PGBatchArgs<float> a32;
   sb::PGBatchArgs<double> a64;
-  std::memset(&a32, 0, sizeof(a32));
-  std::memset(&a64, 0, sizeof(a64));
-  a32.prm = pl.prm32;
-  a32.nprob = B->n;
-  a32.order = B->d_order.p;
-  a32.queue = B->queue.p;
-  a32.probs = B->d32.p;
-  a64.prm = pl.prm64;
-  a64.nprob = B->n;
-  a64.order = B->d_order.p;
-  a64.queue = B->queue.p + 1;
-  a64.probs = B->d64.p;
+  batch_args(a32, B, pl, pl.prm32, B->d32.p);
+  batch_args(a64, B, pl, pl.prm64, B->d64.p);
+  const int nwords = (int)(kQueueWords + (size_t)pl.nteams * sb::kTeamSyncWords);
   SB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = sb::launch_reset_sync(B->queue.p, 2, B->dummyErr.p, s);
-  if (e == hipSuccess && pl.mixed) {
+  hipError_t e = hipSuccess;
+  if (pl.mixed) {
+    e = sb::launch_reset_sync(B->syncw.p, nwords, B->error.p, s);
     void* args[] = {&a32};
-    e = hipLaunchKernel(batch_fn<float>(pl.mf, pl.threads), dim3(pl.grid), dim3(pl.threads), args, 0, s);
+    if (e == hipSuccess)
+      e = hipLaunchKernel(batch_fn<float>(pl.mf, pl.threads), dim3(pl.grid), dim3(pl.threads), args, 0, s);
   }
+  if (e == hipSuccess) e = sb::launch_reset_sync(B->syncw.p, nwords, B->error.p, s);
   if (e == hipSuccess) {
     void* args[] = {&a64};
-    const int th = pl.mixed ? pl.threads64 : pl.threads;
-    const int gr = pl.mixed ? pl.grid64 : pl.grid;
-    e = hipLaunchKernel(batch_fn<double>(pl.mf, th), dim3(gr), dim3(th), args, 0, s);
+    e = hipLaunchKernel(batch_fn<double>(pl.mf, pl.threads), dim3(pl.grid), dim3(pl.threads), args, 0, s);
   }
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(s, &g);
@@ -1372,11 +1516,14 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
   if (rc) return rc;
   BatchPlan pl;
   std::memset(&pl, 0, sizeof(pl));
-  if ((rc = batch_plan(B, o, cfg, pl))) return rc;
+  std::vector<int> order;
+  if ((rc = batch_plan(B, o, cfg, pl, order))) return rc;
   const bool useScal = (o->scaling == SB_SCALING_DIAGONAL) || (o->scaling == SB_SCALING_REFERENCE && !pl.mf);
   if ((rc = batch_descriptors(B, useScal))) return rc;
+  if ((rc = batch_buffers(B, pl.nteams))) return rc;
   if ((rc = batch_graph(B, pl))) return rc;
   hipStream_t s = ctx->stream;
+  SB_HIP(hipMemcpyAsync(B->d_order.p, order.data(), sizeof(int) * B->n, hipMemcpyHostToDevice, s));
   const size_t n3 = 3 * (size_t)B->totalNc;
   if (r) {
     if ((rc = copy_in(ctx, B->xio.p, r, n3, on_device))) return rc;
@@ -1387,7 +1534,9 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
   SB_HIP(hipGraphLaunch(B->exec, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
   std::vector<double> outs((size_t)32 * B->n);
+  int err = 0;
   SB_HIP(hipMemcpyAsync(outs.data(), B->outs.p, sizeof(double) * outs.size(), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(&err, B->error.p, sizeof(int), hipMemcpyDeviceToHost, s));
   if (r && (rc = copy_out(ctx, r, B->xio.p, n3, on_device))) return rc;
   SB_HIP(hipStreamSynchronize(s));
   int nconv = 0, totalIt = 0, maxIt = 0, refined = 0, failed = 0;
@@ -1420,7 +1569,9 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
     st->device_ms = ms;
     st->config.block_threads = pl.threads;
     st->config.blocks_per_cu = pl.blocksPerCU;
+    st->config.contacts_per_block = pl.contactsPerBlock;
   }
+  if (err) return SB_ERR_BARRIER_TIMEOUT;
   if (failed) return SB_ERR_NAN;
   return nconv == B->n ? SB_OK : SB_NOT_CONVERGED;
 }
@@ -1442,27 +1593,20 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
   B->hasW = (flags & SB_PROBLEM_ASSEMBLE_W) != 0;
   B->probs.resize(n);
   B->off.resize(n);
+  B->cost.resize(n);
   for (int k = 0; k < n; ++k) {
     if ((rc = create_problem(ctx, B->probs[k], ncs[k], nbs[k], rps[k], cis[k], hvs[k], minvs[k], bs[k], mus[k], flags)))
       return rc;
     B->off[k] = 3 * B->totalNc;
     B->totalNc += ncs[k];
-  }
-  // work estimate: stored blocks of the operator (largest first, stable)
-  std::vector<int64_t> cost(n);
-  for (int k = 0; k < n; ++k) {
     const sb_problem* P = B->probs[k].get();
-    cost[k] = (int64_t)P->H.host.nblocks * 2 + P->HT.host.nblocks + (B->hasW ? P->W.host.nblocks : 0);
+    B->cost[k] = (int64_t)P->H.host.nblocks * 2 + P->HT.host.nblocks + (B->hasW ? P->W.host.nblocks : 0);
   }
-  B->order.resize(n);
-  for (int k = 0; k < n; ++k) B->order[k] = k;
-  std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   hipStream_t s = ctx->stream;
-  SB_HIP(B->d_order.upload(B->order, s));
+  SB_HIP(B->d_order.alloc(n));
   SB_HIP(B->xio.alloc(3 * (size_t)B->totalNc));
   SB_HIP(B->outs.alloc((size_t)32 * n));
-  SB_HIP(B->queue.alloc(2));
-  SB_HIP(B->dummyErr.alloc(1));
+  SB_HIP(B->error.alloc(1));
   SB_HIP(B->d32.alloc(n));
   SB_HIP(B->d64.alloc(n));
   SB_HIP(hipStreamSynchronize(s));
@@ -1502,25 +1646,28 @@ int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int prob
   const bool mf = (o->op == SB_OP_MATRIX_FREE);
   const bool mixed = (o->precision == SB_PRECISION_MIXED);
   double bestMs = 1e300;
-  sb_batch_config bestCfg{0, 0};
+  sb_batch_config bestCfg{0, 0, 0};
   const int threadsList[3] = {256, 512, 1024};
+  const int cpbList[5] = {-1, 8000, 4000, 2000, 1000};
   for (int th : threadsList) {
-    const void* fn = mixed ? batch_fn<float>(mf, th) : batch_fn<double>(mf, th);
-    const int occ = batch_occupancy(fn, th);
+    int occ = batch_occupancy(batch_fn<double>(mf, th), th);
+    if (mixed) occ = std::min(occ, batch_occupancy(batch_fn<float>(mf, th), th));
     for (int bpc = 1; bpc <= std::min(occ, 8); ++bpc) {
       if (th * bpc > 2048) continue;  // at most 32 waves per CU
-      sb_batch_config c{th, bpc};
-      double ms = 1e300;
-      for (int rep = 0; rep < 2; ++rep) {
-        sb_batch_stats st;
-        std::memset(&st, 0, sizeof(st));
-        rc = batch_run(ctx, B, &po, &c, nullptr, 0, &st);
-        if (rc < 0) return rc;
-        ms = std::min(ms, st.device_ms);
-      }
-      if (ms < bestMs) {
-        bestMs = ms;
-        bestCfg = c;
+      for (int cpb : cpbList) {
+        sb_batch_config c{th, bpc, cpb};
+        double ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+          sb_batch_stats st;
+          std::memset(&st, 0, sizeof(st));
+          rc = batch_run(ctx, B, &po, &c, nullptr, 0, &st);
+          if (rc < 0) return rc;
+          ms = std::min(ms, st.device_ms);
+        }
+        if (ms < bestMs) {
+          bestMs = ms;
+          bestCfg = c;
+        }
       }
     }
   }

@@ -64,6 +64,9 @@ __device__ __forceinline__ void cst(float* p, float v) {
 __device__ __forceinline__ float cld(const float* p) {
   return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void cst(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // Load of a value another block wrote in an earlier phase of the same kernel.
 template <typename T>
 __device__ __forceinline__ T mld(const T* p) {
@@ -100,6 +103,7 @@ __device__ __forceinline__ uint64_t realtime() { return __builtin_amdgcn_s_memre
 
 struct Barrier {
   GridSync gs;
+  unsigned rank = 0;         // this block's index among the gs.nblocks blocks (must be set)
   unsigned epoch = 0;
   unsigned nred = 0;         // reductions done (selects the partials buffer)
   double red[kMaxPartials];  // results of the last reduction (identical in every block)
@@ -114,7 +118,7 @@ struct Barrier {
     __syncthreads();
     if (threadIdx.x == 0) {
       const unsigned G = gs.nblocks < kSyncGroups ? gs.nblocks : kSyncGroups;
-      const unsigned g = blockIdx.x % G;
+      const unsigned g = rank % G;
       const unsigned gsize = gs.nblocks / G + ((g < gs.nblocks % G) ? 1u : 0u);
       const unsigned old = __hip_atomic_fetch_add(&gs.group[g * kSyncStride], 1u, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
@@ -178,7 +182,7 @@ struct Barrier {
       const int nw = (blockDim.x + SB_WAVE - 1) / SB_WAVE;
       double acc = 0.0;
       for (int q = 0; q < nw; ++q) acc = isMax ? (s_p[k][q] > acc ? s_p[k][q] : acc) : acc + s_p[k][q];
-      cst(&part[(size_t)blockIdx.x * kMaxPartials + k], acc);
+      cst(&part[(size_t)rank * kMaxPartials + k], acc);
     }
     if (!sync()) return false;
     // every wave reduces one partial index over all blocks: lane l takes blocks l, l+64,

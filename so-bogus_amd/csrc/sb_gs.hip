@@ -48,6 +48,7 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
   const size_t nnz = (size_t)W.nblocks;
   Barrier bar;
   bar.gs = a.gs;
+  bar.rank = blockIdx.x;
 
   // y = W x + b ; err = eval(y, x) [and err0 = eval(b, 0) when asked]
   auto eval_phase = [&](bool withZero, double& err, double& err0) -> bool {

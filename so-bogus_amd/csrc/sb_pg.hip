@@ -258,6 +258,8 @@ __device__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, u
   }
 
   bool converged = false;
+  double stallRef = 1.7976931348623157e308;  // best residual at the last 1% improvement
+  int lastImprove = 0;
   while (ok) {
     // ---- test_residual at the current x (iteration `it`)
     if (tm.leader() && it < pb.hist_cap) pb.hist[it] = res;
@@ -269,6 +271,11 @@ __device__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, u
       converged = true;
       break;
     }
+    if (minRes < 0.99 * stallRef) {
+      stallRef = minRes;
+      lastImprove = it;
+    }
+    if (prm.stall_iters > 0 && it - lastImprove >= prm.stall_iters) break;
     int rxs = 0;
     while (rxs == rx || rxs == rprev) ++rxs;
     const T* Xx = pb.X[rx];
@@ -540,27 +547,66 @@ __device__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, u
 // One problem on the whole co-resident grid.
 template <bool MF, typename T>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
-  GridTeam tm(a.gs);
+  GridTeam tm(a.gs, blockIdx.x);
   uint64_t tsum[4] = {0, 0, 0, 0};
   pg_body<MF, T, GridTeam>(a.prm, a.pb, tm, tsum);
 }
 
-// Many problems: each workgroup takes the next problem from the queue and solves
-// it alone. Workgroups never wait for each other, so any grid size is valid.
+// Many problems. Block b belongs to the class whose block range holds it; its
+// team takes the next problem of the class's queue and solves it alone. Teams
+// never wait for other teams. A team of k > 1 blocks needs its blocks resident
+// together (the grid is sized to the occupancy limit); its barrier waits are
+// bounded and set the sticky error flag on timeout.
 template <bool MF, typename T, int NT>
 __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a) {
   __shared__ int s_next;
+  int c = 0;
+  while (c + 1 < a.nclass && (int)blockIdx.x >= a.cls[c + 1].b0) ++c;
+  const BatchClass cl = a.cls[c];
+  const int local = (int)blockIdx.x - cl.b0;
+  const int team = local / cl.k;
+  const unsigned rank = (unsigned)(local % cl.k);
+  if (team >= cl.nteams) return;  // blocks beyond the class's whole teams
+  const int nq = cl.q1 - cl.q0;
+  if (cl.k == 1) {
+    for (;;) {
+      if (threadIdx.x == 0) s_next = (int)atomicAdd(&a.queue[c], 1u);
+      __syncthreads();
+      const int k = __builtin_amdgcn_readfirstlane(s_next);
+      __syncthreads();
+      if (k >= nq) break;
+      const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
+      BlockTeam tm;
+      uint64_t tsum[4] = {0, 0, 0, 0};
+      pg_body<MF, T, BlockTeam>(a.prm, a.probs[p], tm, tsum);
+      __syncthreads();
+    }
+    return;
+  }
+  const int gt = cl.team0 + team;
+  GridSync g;
+  g.group = a.team_sync + (size_t)gt * kTeamSyncWords;
+  g.release = g.group + kSyncGroups * kSyncStride;
+  g.top = g.group + 2 * kSyncGroups * kSyncStride;
+  g.error = a.error;
+  g.partials = a.team_partials + (size_t)gt * 2 * kMaxTeamBlocks * kMaxPartials;
+  g.nblocks = (unsigned)cl.k;
+  g.timeout_ticks = a.timeout_ticks;
+  GridTeam tm(g, rank);
   for (;;) {
-    if (threadIdx.x == 0) s_next = (int)atomicAdd(a.queue, 1u);
+    // the team leader draws the next problem and hands it over through the barrier
+    if (rank == 0 && threadIdx.x == 0) cst(&a.team_slot[gt], (int)atomicAdd(&a.queue[c], 1u));
+    if (!tm.sync()) break;
+    if (threadIdx.x == 0) s_next = __hip_atomic_load(&a.team_slot[gt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int k = __builtin_amdgcn_readfirstlane(s_next);
-    __syncthreads();
-    if (k >= a.nprob) break;
-    const int p = __builtin_amdgcn_readfirstlane(a.order[k]);
-    BlockTeam tm;
+    // the slot is rewritten only after the next problem's first barrier, which every
+    // member passes after this read
+    if (k >= nq) break;
+    const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
     uint64_t tsum[4] = {0, 0, 0, 0};
-    pg_body<MF, T, BlockTeam>(a.prm, a.probs[p], tm, tsum);
-    __syncthreads();
+    pg_body<MF, T, GridTeam>(a.prm, a.probs[p], tm, tsum);
+    if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
   }
 }
 

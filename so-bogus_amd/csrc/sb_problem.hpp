@@ -36,6 +36,8 @@ struct PGParams {
   int max_iters;
   int ls_iters;
   double ls_opt, ls_pess;
+  int stall_iters;  // > 0: stop (not converged) after this many iterations without a 1% decrease of
+                    // the best residual (the FP32 stage of a mixed solve hands over to FP64)
 };
 
 // One problem's device data and workspace; T is the storage (and bulk
@@ -64,14 +66,35 @@ struct PGGridArgs {
   GridSync gs;
 };
 
-// Batch: workgroups take problems (in the given order) from an atomic queue.
+// Batch: the grid is split into classes of teams; a team is k consecutive
+// workgroups (k = 1: one workgroup, synchronised by __syncthreads; k > 1: a
+// co-resident group with its own barrier and reduction state). Every team of a
+// class takes problems from the class's queue (a range of order[], most
+// expensive first) and solves them one at a time.
+constexpr int kMaxBatchClasses = 8;
+constexpr int kMaxTeamBlocks = 16;
+constexpr unsigned kTeamSyncWords = 2 * kSyncGroups * kSyncStride + kSyncStride;  // per team
+struct BatchClass {
+  int b0;       // first block of the class
+  int k;        // blocks per team
+  int nteams;
+  int q0, q1;   // the class's problems: order[q0 .. q1)
+  int team0;    // global index of the class's first team
+};
 template <typename T>
 struct PGBatchArgs {
   PGParams prm;
   int nprob;
-  const int* order;           // problem indices, most expensive first
-  unsigned* queue;            // [1] next position in order
+  const int* order;           // problem indices grouped by class, most expensive first
   const PGProblem<T>* probs;  // [nprob]
+  int nclass;
+  BatchClass cls[kMaxBatchClasses];
+  unsigned* queue;            // [nclass] next position of each class's queue
+  unsigned* team_sync;        // [nteams][kTeamSyncWords] barrier words of the k > 1 teams
+  double* team_partials;      // [nteams][2][kMaxTeamBlocks][kMaxPartials]
+  int* team_slot;             // [nteams] problem handed to the team
+  int* error;                 // sticky barrier-timeout flag
+  uint64_t timeout_ticks;
 };
 
 struct GSArgs {

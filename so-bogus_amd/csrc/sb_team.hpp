@@ -1,8 +1,9 @@
 // Teams: the set of threads that solves one problem together.
 //
-//  * GridTeam  - every block of a co-resident (persistent) grid; phases are
-//                separated by the fence-free grid barrier and reductions go
-//                through the deterministic grid reduction (Barrier).
+//  * GridTeam  - co-resident blocks (a whole persistent grid, or k consecutive
+//                blocks of the batched kernel); phases are separated by the
+//                fence-free grid barrier and reductions go through the
+//                deterministic grid reduction (Barrier) over the team's blocks.
 //  * BlockTeam - one workgroup; phases are separated by __syncthreads and
 //                reductions go through LDS. Used by the batched solver, where
 //                every workgroup takes whole problems from a queue.
@@ -21,16 +22,21 @@ namespace sb {
 struct GridTeam {
   Barrier bar;
   int gw, nwaves, tid, nthreads;
+  bool lead;
   double red[kMaxPartials];
-  __device__ explicit GridTeam(const GridSync& g) {
+  // The blocks rank = 0..g.nblocks-1 of a team (the whole grid, or a group of
+  // consecutive blocks of the batched kernel) that share the barrier state g.
+  __device__ GridTeam(const GridSync& g, unsigned rank) {
     bar.gs = g;
+    bar.rank = rank;
     const int wpb = blockDim.x / SB_WAVE;
-    gw = blockIdx.x * wpb + wave_in_block();
-    nwaves = gridDim.x * wpb;
-    tid = blockIdx.x * blockDim.x + threadIdx.x;
-    nthreads = gridDim.x * blockDim.x;
+    gw = (int)rank * wpb + wave_in_block();
+    nwaves = (int)g.nblocks * wpb;
+    tid = (int)rank * blockDim.x + threadIdx.x;
+    nthreads = (int)g.nblocks * blockDim.x;
+    lead = rank == 0 && threadIdx.x == 0;
   }
-  __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
+  __device__ bool leader() const { return lead; }
   __device__ bool sync() { return bar.sync(); }
   template <int NV>
   __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {

@@ -49,7 +49,7 @@ class PGOptions(C.Structure):
                 ("inf_norm", C.c_int), ("tol", C.c_double), ("max_iters", C.c_int), ("ls_iters", C.c_int),
                 ("ls_optimistic", C.c_double), ("ls_pessimistic", C.c_double), ("ls_armijo", C.c_double),
                 ("grid_blocks", C.c_int), ("block_threads", C.c_int), ("precision", C.c_int),
-                ("mixed_tol_factor", C.c_double)]
+                ("mixed_tol_factor", C.c_double), ("mixed_stall_iters", C.c_int)]
 
 
 class GSOptions(C.Structure):
@@ -68,7 +68,7 @@ class Stats(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("block_threads", C.c_int), ("blocks_per_cu", C.c_int)]
+    _fields_ = [("block_threads", C.c_int), ("blocks_per_cu", C.c_int), ("contacts_per_block", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -442,7 +442,7 @@ class BatchResult:
     refined_problems: int
     total_spmvs: int
     device_ms: float
-    config: tuple[int, int]
+    config: tuple[int, int, int]
 
 
 class Batch:
@@ -487,7 +487,7 @@ class Batch:
 
     def solve(self, r0=None, opts: PGOptions | None = None, config=None, on_device=False, **kw) -> BatchResult:
         o = opts if opts is not None else pg_options(**kw)
-        cfg = BatchConfig(*(config or (0, 0)))
+        cfg = BatchConfig(*(config or (0, 0, 0)))
         its = np.zeros(self.n, dtype=np.int32)
         res = np.zeros(self.n)
         conv = np.zeros(self.n, dtype=np.int32)
@@ -507,7 +507,8 @@ class Batch:
                            n_converged=st.n_converged, total_iterations=st.total_iterations,
                            max_iterations=st.max_iterations, refined_problems=st.refined_problems,
                            total_spmvs=st.total_spmvs, device_ms=st.device_ms,
-                           config=(st.config.block_threads, st.config.blocks_per_cu))
+                           config=(st.config.block_threads, st.config.blocks_per_cu,
+                                   st.config.contacts_per_block))
 
     def autotune(self, opts: PGOptions | None = None, probe_iters: int = 50, **kw):
         o = opts if opts is not None else pg_options(**kw)
@@ -515,7 +516,7 @@ class Batch:
         ms = C.c_double()
         _check(lib().sb_batch_autotune(self.ctx.h, self.h, C.byref(o), probe_iters, C.byref(best), C.byref(ms)),
                "sb_batch_autotune")
-        return (best.block_threads, best.blocks_per_cu), ms.value
+        return (best.block_threads, best.blocks_per_cu, best.contacts_per_block), ms.value
 
 
 def gpu_available() -> bool:

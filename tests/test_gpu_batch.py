@@ -74,9 +74,11 @@ def test_batch_graph_replay_is_deterministic(R, ctx, small_batch):
         again = B.solve(**kw)
         assert np.array_equal(again.r, first.r)
         assert np.array_equal(again.iterations, first.iterations)
-    # another geometry (grid size does not change a problem's arithmetic)
-    other = B.solve(config=(512, 1), **kw)
-    assert np.array_equal(other.r, first.r)
+    # another grid size with the same workgroup size and one workgroup per problem:
+    # a problem's arithmetic does not depend on which workgroup solves it
+    one = B.solve(config=(512, 1, -1), **kw)
+    other = B.solve(config=(512, 2, -1), **kw)
+    assert np.array_equal(other.r, one.r)
 
 
 def test_batch_warm_start_and_autotune(R, O, ctx, small_batch):
@@ -84,7 +86,7 @@ def test_batch_warm_start_and_autotune(R, O, ctx, small_batch):
     kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-6,
               max_iters=5000, precision=R.PRECISION_MIXED)
     cfg, ms = B.autotune(probe_iters=20, **kw)
-    assert cfg[0] in (256, 512, 1024) and 1 <= cfg[1] <= 8 and ms > 0
+    assert cfg[0] in (256, 512, 1024) and 1 <= cfg[1] <= 8 and cfg[2] != 0 and ms > 0
     res = B.solve(**kw)
     assert res.config == cfg
     warm = B.solve(r0=res.r, **kw)
@@ -103,11 +105,25 @@ def test_batch_hub_rows_match_grid_solver(G, R, ctx):
     B = R.Batch(ctx, probs)
     kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-30,
               max_iters=15, precision=R.PRECISION_FP64)
-    res = B.solve(**kw)
-    assert res.status == R.SB_NOT_CONVERGED
-    out = B.split(res.r)
-    for p, rk in zip(probs, out):
+    single = B.solve(config=(512, 1, -1), **kw)   # one workgroup per problem
+    teams = B.solve(config=(512, 1, 1000), **kw)  # teams of 4 workgroups
+    assert single.status == R.SB_NOT_CONVERGED and teams.status == R.SB_NOT_CONVERGED
+    for p, r1, r4 in zip(probs, B.split(single.r), B.split(teams.r)):
         assert np.bincount(p.colidx, minlength=p.nb).max() > 64
         dp = R.DualFrictionProblem(ctx, p, assemble=False)
         g = dp.solve_pg(np.zeros(3 * p.nc), **kw)
-        assert np.linalg.norm(rk - g.r) <= 1e-6 * np.linalg.norm(g.r)
+        assert np.linalg.norm(r1 - g.r) <= 1e-6 * np.linalg.norm(g.r)
+        assert np.linalg.norm(r4 - g.r) <= 1e-6 * np.linalg.norm(g.r)
+
+
+def test_batch_multi_workgroup_teams(R, O, ctx, small_batch):
+    """Teams of 2-4 workgroups per problem (contacts_per_block 500) give
+    oracle-verified results, and replays of the same plan are bit-identical."""
+    B = R.Batch(ctx, small_batch)
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-6,
+              max_iters=5000, precision=R.PRECISION_FP64, config=(256, 2, 500))
+    res = B.solve(**kw)
+    assert res.n_converged >= len(small_batch) - 2
+    _check_each(O, small_batch, res, B.split(res.r), 1e-6, 5000)
+    again = B.solve(**kw)
+    assert np.array_equal(again.r, res.r)
