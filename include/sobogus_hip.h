@@ -237,9 +237,12 @@ int sb_batch_sizes(const sb_batch* B, int* n_problems, int64_t* total_contacts);
    result out); host or device (on_device) memory. cfg may be NULL. */
 int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, const sb_batch_config* cfg, double* r,
                    int on_device, sb_batch_stats* stats);
-/* Times every launch geometry the kernels allow on this batch (each capped at
-   probe_iters iterations per problem, r0 = 0), keeps the fastest for later solves
-   of this batch and returns it. */
+/* Autotuner: times every launch geometry the kernels allow on this batch (each
+   capped at probe_iters iterations per problem, r0 = 0), records each problem's
+   iteration count in one full solve (long-running problems then get larger
+   teams), times the best geometry of each team sizing on full solves, and keeps
+   the fastest for later solves of this batch. Solves are reproducible between
+   calls to this function: only it changes the schedule. */
 int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, int probe_iters, sb_batch_config* best,
                       double* best_ms);
 

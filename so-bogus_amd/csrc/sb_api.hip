@@ -1224,7 +1224,8 @@ struct sb_batch {
   bool hasW = true;
   std::vector<std::unique_ptr<sb_problem>> probs;
   std::vector<int64_t> off;   // offset of problem k in the concatenated r (doubles)
-  std::vector<int64_t> cost;  // work estimate: stored operator blocks
+  std::vector<int64_t> cost;  // work estimate per iteration: stored operator blocks
+  std::vector<int> iterHint;  // iterations each problem took in the last full solve (0: unknown)
   DBuf<double> xio;           // [3 sum n_c]
   DBuf<double> outs;          // [2][n][16]: FP32 stage, FP64 stage
   DBuf<sb::PGProblem<float>> d32;
@@ -1319,15 +1320,29 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
   const int G = pl.grid;
   int kcap = 1;  // largest team: a power of two, at most half the grid
   while (kcap * 2 <= std::min(sb::kMaxTeamBlocks, G / 2)) kcap *= 2;
+  // iteration weight of each problem relative to the median, from the last full
+  // solve of this batch (1 when unknown)
+  std::vector<double> wIt(B->n, 1.0);
+  {
+    std::vector<int> h;
+    for (int v : B->iterHint)
+      if (v > 0) h.push_back(v);
+    if ((int)h.size() == B->n) {
+      std::nth_element(h.begin(), h.begin() + h.size() / 2, h.end());
+      const double med = std::max(1, h[h.size() / 2]);
+      for (int p = 0; p < B->n; ++p) wIt[p] = std::max(0.25, B->iterHint[p] / med);
+    }
+  }
   std::vector<int> kOf(B->n);
   for (int p = 0; p < B->n; ++p) {
     int k = 1;
     if (pl.contactsPerBlock > 0) {
-      const int64_t need = (B->probs[p]->nc + pl.contactsPerBlock - 1) / pl.contactsPerBlock;
+      const int64_t need = (int64_t)std::ceil(B->probs[p]->nc * wIt[p] / pl.contactsPerBlock);
       k = std::min(pow2ceil(need), kcap);
     }
     kOf[p] = k;
   }
+  auto costOf = [&](int p) { return (double)B->cost[p] * wIt[p]; };
   // classes in decreasing team size
   std::vector<int> ks;
   for (int k = sb::kMaxTeamBlocks; k >= 1; k >>= 1)
@@ -1341,7 +1356,7 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
   std::vector<int> count(C, 0);
   for (int p = 0; p < B->n; ++p) {
     const int c = (int)(std::find(ks.begin(), ks.end(), kOf[p]) - ks.begin());
-    work[c] += (double)B->cost[p];
+    work[c] += costOf(p);
     ++count[c];
   }
   double total = 0;
@@ -1393,7 +1408,7 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
     std::vector<int> mine;
     for (int p = 0; p < B->n; ++p)
       if (kOf[p] == ks[c]) mine.push_back(p);
-    std::stable_sort(mine.begin(), mine.end(), [&](int a, int b) { return B->cost[a] > B->cost[b]; });
+    std::stable_sort(mine.begin(), mine.end(), [&](int a, int b) { return costOf(a) > costOf(b); });
     order.insert(order.end(), mine.begin(), mine.end());
     cl.q1 = (int)order.size();
     cl.team0 = t0;
@@ -1454,13 +1469,11 @@ int batch_buffers(sb_batch* B, int nteams) {
 }
 
 template <typename T>
-void batch_args(sb::PGBatchArgs<T>& a, const sb_batch* B, const BatchPlan& pl, const sb::PGParams& prm,
-                const sb::PGProblem<T>* probs) {
+void batch_args(sb::PGBatchArgs<T>& a, const sb_batch* B, const BatchPlan& pl, const sb::PGParams& prm) {
   std::memset(&a, 0, sizeof(a));
   a.prm = prm;
   a.nprob = B->n;
   a.order = B->d_order.p;
-  a.probs = probs;
   a.nclass = pl.nclass;
   for (int c = 0; c < pl.nclass; ++c) a.cls[c] = pl.cls[c];
   a.queue = B->syncw.p;
@@ -1484,20 +1497,22 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
   hipStream_t s = B->ctx->stream;
   sb::PGBatchArgs<float> a32;
   sb::PGBatchArgs<double> a64;
-  batch_args(a32, B, pl, pl.prm32, B->d32.p);
-  batch_args(a64, B, pl, pl.prm64, B->d64.p);
+  batch_args(a32, B, pl, pl.prm32);
+  batch_args(a64, B, pl, pl.prm64);
+  const sb::PGProblem<float>* p32 = B->d32.p;
+  const sb::PGProblem<double>* p64 = B->d64.p;
   const int nwords = (int)(kQueueWords + (size_t)pl.nteams * sb::kTeamSyncWords);
   SB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
   if (pl.mixed) {
     e = sb::launch_reset_sync(B->syncw.p, nwords, B->error.p, s);
-    void* args[] = {&a32};
+    void* args[] = {&a32, &p32};
     if (e == hipSuccess)
       e = hipLaunchKernel(batch_fn<float>(pl.mf, pl.threads), dim3(pl.grid), dim3(pl.threads), args, 0, s);
   }
   if (e == hipSuccess) e = sb::launch_reset_sync(B->syncw.p, nwords, B->error.p, s);
   if (e == hipSuccess) {
-    void* args[] = {&a64};
+    void* args[] = {&a64, &p64};
     e = hipLaunchKernel(batch_fn<double>(pl.mf, pl.threads), dim3(pl.grid), dim3(pl.threads), args, 0, s);
   }
   hipGraph_t g = nullptr;
@@ -1511,7 +1526,7 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
 }
 
 int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, double* r, int on_device,
-              sb_batch_stats* st) {
+              sb_batch_stats* st, bool updateHints) {
   int rc = check_pg_options(o, B->hasW);
   if (rc) return rc;
   BatchPlan pl;
@@ -1552,6 +1567,7 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
     nconv += conv ? 1 : 0;
     if (pl.mixed && o64[1] > 0) ++refined;
     if (o64[6] != 0.0 || (pl.mixed && o32[6] != 0.0) || !std::isfinite(o64[0])) ++failed;
+    if (updateHints) B->iterHint[k] = std::max(1, it);
     if (st) {
       if (st->iterations) st->iterations[k] = it;
       if (st->residuals) st->residuals[k] = o64[0];
@@ -1594,6 +1610,7 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
   B->probs.resize(n);
   B->off.resize(n);
   B->cost.resize(n);
+  B->iterHint.assign(n, 0);
   for (int k = 0; k < n; ++k) {
     if ((rc = create_problem(ctx, B->probs[k], ncs[k], nbs[k], rps[k], cis[k], hvs[k], minvs[k], bs[k], mus[k], flags)))
       return rc;
@@ -1633,7 +1650,8 @@ int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_ba
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!B || !o || !r || B->ctx != ctx) return SB_ERR_INVALID;
-  return batch_run(ctx, B, o, cfg, r, on_device, st);
+  // the plan (and so every problem's arithmetic) changes only through sb_batch_autotune
+  return batch_run(ctx, B, o, cfg, r, on_device, st, false);
 }
 
 int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int probe_iters, sb_batch_config* best,
@@ -1641,12 +1659,24 @@ int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int prob
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!B || !o || B->ctx != ctx || probe_iters < 1) return SB_ERR_INVALID;
-  sb_pg_options po = *o;
-  po.max_iters = probe_iters;
   const bool mf = (o->op == SB_OP_MATRIX_FREE);
   const bool mixed = (o->precision == SB_PRECISION_MIXED);
-  double bestMs = 1e300;
-  sb_batch_config bestCfg{0, 0, 0};
+  auto timeRun = [&](const sb_pg_options* opts, const sb_batch_config& c, bool hints, double& ms) -> int {
+    ms = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      sb_batch_stats st;
+      std::memset(&st, 0, sizeof(st));
+      const int r = batch_run(ctx, B, opts, &c, nullptr, 0, &st, hints && rep == 0);
+      if (r < 0) return r;
+      ms = std::min(ms, st.device_ms);
+    }
+    return SB_OK;
+  };
+  // 1. screen every geometry on probe_iters iterations per problem (the per-iteration
+  //    cost of each launch shape)
+  sb_pg_options po = *o;
+  po.max_iters = std::min(probe_iters, o->max_iters);
+  std::vector<std::pair<double, sb_batch_config>> screened;
   const int threadsList[3] = {256, 512, 1024};
   const int cpbList[5] = {-1, 8000, 4000, 2000, 1000};
   for (int th : threadsList) {
@@ -1655,20 +1685,34 @@ int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int prob
     for (int bpc = 1; bpc <= std::min(occ, 8); ++bpc) {
       if (th * bpc > 2048) continue;  // at most 32 waves per CU
       for (int cpb : cpbList) {
-        sb_batch_config c{th, bpc, cpb};
-        double ms = 1e300;
-        for (int rep = 0; rep < 2; ++rep) {
-          sb_batch_stats st;
-          std::memset(&st, 0, sizeof(st));
-          rc = batch_run(ctx, B, &po, &c, nullptr, 0, &st);
-          if (rc < 0) return rc;
-          ms = std::min(ms, st.device_ms);
-        }
-        if (ms < bestMs) {
-          bestMs = ms;
-          bestCfg = c;
-        }
+        const sb_batch_config c{th, bpc, cpb};
+        double ms;
+        if ((rc = timeRun(&po, c, false, ms))) return rc;
+        screened.push_back({ms, c});
       }
+    }
+  }
+  std::sort(screened.begin(), screened.end(),
+            [](const std::pair<double, sb_batch_config>& a, const std::pair<double, sb_batch_config>& b) {
+              return a.first < b.first;
+            });
+  // 2. one full solve with the best screened shape records every problem's
+  //    iteration count (long-running problems then get larger teams); the best
+  //    shape of each team sizing is then timed on full solves
+  double ms0;
+  if ((rc = timeRun(o, screened[0].second, true, ms0))) return rc;
+  double bestMs = 1e300;
+  sb_batch_config bestCfg = screened[0].second;
+  for (int cpb : cpbList) {  // the best screened shape of each team sizing
+    for (const auto& sc : screened) {
+      if (sc.second.contacts_per_block != cpb) continue;
+      double ms;
+      if ((rc = timeRun(o, sc.second, false, ms))) return rc;
+      if (ms < bestMs) {
+        bestMs = ms;
+        bestCfg = sc.second;
+      }
+      break;
     }
   }
   B->tuned = true;

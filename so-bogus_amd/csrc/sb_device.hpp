@@ -104,6 +104,8 @@ __device__ __forceinline__ uint64_t realtime() { return __builtin_amdgcn_s_memre
 struct Barrier {
   GridSync gs;
   unsigned rank = 0;         // this block's index among the gs.nblocks blocks (must be set)
+  bool acquire = true;       // invalidate L1/L2 after the barrier (readers use plain loads); false:
+                             // readers use coherent loads and the caches are left alone
   unsigned epoch = 0;
   unsigned nred = 0;         // reductions done (selects the partials buffer)
   double red[kMaxPartials];  // results of the last reduction (identical in every block)
@@ -150,7 +152,7 @@ struct Barrier {
       // Acquire: drop this CU's L1 and this XCD's L2 copies of lines written by
       // other blocks before the barrier (their coherent stores already reached
       // memory), so that plain cached loads in the next phase read fresh values.
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (acquire) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #endif
     }
     __syncthreads();

@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
         part[1] = resMax ? fmax(part[1], r0) : part[1] + r0;
       }
     };
-    for (int t = gw; t < W.ntiles; t += nwaves) tile_gather<3, 3, double, double>(W, t, ld, nopre, lead, a.carry);
+    tile_loop<3, 3, double, double>(W, gw, nwaves, ld, nopre, lead, a.carry);
     if (W.nlong > 0) {
       if (!bar.sync()) return false;
       for (int k = gw; k < W.nlong; k += nwaves) {

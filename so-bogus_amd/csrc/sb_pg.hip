@@ -85,7 +85,7 @@ template <bool MF, typename T, typename Team, typename Loader, typename Pre, typ
 __device__ __forceinline__ bool apply_op(const PGProblem<T>& pb, Team& tm, Loader& ld, Pre& pre, Leader& lead) {
   const DevProblem<T>& P = pb.P;
   if (!MF) {
-    for (int t = tm.gw; t < P.W.ntiles; t += tm.nwaves) tile_gather<3, 3, T, T>(P.W, t, ld, pre, lead, pb.carry);
+    tile_loop<3, 3, T, T>(P.W, tm.gw, tm.nwaves, ld, pre, lead, pb.carry);
     if (P.W.nlong > 0) {
       if (!tm.sync()) return false;
       for (int k = tm.gw; k < P.W.nlong; k += tm.nwaves) {
@@ -106,7 +106,7 @@ __device__ __forceinline__ bool apply_op(const PGProblem<T>& pb, Team& tm, Loade
 #pragma unroll
     for (int r = 0; r < 6; ++r) Team::st(&pb.wbody[6 * k + r], w[r]);
   };
-  for (int t = tm.gw; t < P.HT.ntiles; t += tm.nwaves) tile_gather<6, 3, T, T>(P.HT, t, ld, nopre, leadA, pb.carry);
+  tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, pb.carry);
   if (!tm.sync()) return false;
   if (P.HT.nlong > 0) {
     for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {
@@ -120,7 +120,7 @@ __device__ __forceinline__ bool apply_op(const PGProblem<T>& pb, Team& tm, Loade
 #pragma unroll
     for (int r = 0; r < 6; ++r) xc[r] = Team::ld(&pb.wbody[6 * k + r]);
   };
-  for (int t = tm.gw; t < P.H.ntiles; t += tm.nwaves) tile_gather<3, 6, T, T>(P.H, t, ldB, pre, lead, pb.carry);
+  tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, pb.carry);
   return true;
 }
 
@@ -185,7 +185,7 @@ __device__ __forceinline__ void start_point(const PGProblem<T>& pb, int i, T xc[
 // (s_memrealtime ticks): [0] operator-phase work, [1] operator-phase reduction
 // and wait, [2] update-phase work, [3] update-phase wait.
 template <bool MF, typename T, typename Team>
-__device__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, uint64_t tsum[4]) {
+__device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, uint64_t tsum[4]) {
   const DevProblem<T>& P = pb.P;
   const int nc = P.nc;
   const int n3 = 3 * nc;
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
 // together (the grid is sized to the occupancy limit); its barrier waits are
 // bounded and set the sticky error flag on timeout.
 template <bool MF, typename T, int NT>
-__global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a) {
+__global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
   __shared__ int s_next;
   int c = 0;
   while (c + 1 < a.nclass && (int)blockIdx.x >= a.cls[c + 1].b0) ++c;
@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a) {
       const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
       BlockTeam tm;
       uint64_t tsum[4] = {0, 0, 0, 0};
-      pg_body<MF, T, BlockTeam>(a.prm, a.probs[p], tm, tsum);
+      pg_body<MF, T, BlockTeam>(a.prm, probs[p], tm, tsum);
       __syncthreads();
     }
     return;
@@ -592,7 +592,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a) {
   g.partials = a.team_partials + (size_t)gt * 2 * kMaxTeamBlocks * kMaxPartials;
   g.nblocks = (unsigned)cl.k;
   g.timeout_ticks = a.timeout_ticks;
-  GridTeam tm(g, rank);
+  BatchTeam tm(g, rank);
   for (;;) {
     // the team leader draws the next problem and hands it over through the barrier
     if (rank == 0 && threadIdx.x == 0) cst(&a.team_slot[gt], (int)atomicAdd(&a.queue[c], 1u));
@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a) {
     if (k >= nq) break;
     const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
     uint64_t tsum[4] = {0, 0, 0, 0};
-    pg_body<MF, T, GridTeam>(a.prm, a.probs[p], tm, tsum);
+    pg_body<MF, T, BatchTeam>(a.prm, probs[p], tm, tsum);
     if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
   }
 }
@@ -615,7 +615,8 @@ template __global__ void pg_kernel<true, double>(PGGridArgs<double>);
 template __global__ void pg_kernel<false, float>(PGGridArgs<float>);
 template __global__ void pg_kernel<true, float>(PGGridArgs<float>);
 
-#define SB_BATCH_INST(MF, T, NT) template __global__ void pg_batch_kernel<MF, T, NT>(PGBatchArgs<T>);
+#define SB_BATCH_INST(MF, T, NT) \
+  template __global__ void pg_batch_kernel<MF, T, NT>(PGBatchArgs<T>, const PGProblem<T>* __restrict__);
 SB_BATCH_INST(false, double, 256)
 SB_BATCH_INST(true, double, 256)
 SB_BATCH_INST(false, float, 256)

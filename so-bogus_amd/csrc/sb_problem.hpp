@@ -86,7 +86,6 @@ struct PGBatchArgs {
   PGParams prm;
   int nprob;
   const int* order;           // problem indices grouped by class, most expensive first
-  const PGProblem<T>* probs;  // [nprob]
   int nclass;
   BatchClass cls[kMaxBatchClasses];
   unsigned* queue;            // [nclass] next position of each class's queue

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) k_tiles(TileMat<double> A, LD ld, LE le, 
   const int t = blockIdx.x * wpb + wave_in_block();
   if (t >= A.ntiles) return;
   NoPre pre;
-  tile_gather<BR, BC, double, double>(A, t, ld, pre, le, carry);
+  tile_gather<BR, BC, double, double>(A, t, tile_indices(A, t), ld, pre, le, carry);
 }
 
 template <int BR, typename LE>

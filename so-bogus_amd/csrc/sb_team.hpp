@@ -19,16 +19,23 @@
 
 namespace sb {
 
-struct GridTeam {
+// Acquire = true: the barrier invalidates this XCD's caches and vectors written by
+// other blocks are read with plain loads (best when the whole chip runs one
+// problem). Acquire = false: no invalidation; such vectors are read with
+// coherent loads (many small teams share each XCD's L2: an invalidation by one
+// team would evict the others' matrices).
+template <bool Acquire>
+struct GridTeamT {
   Barrier bar;
   int gw, nwaves, tid, nthreads;
   bool lead;
   double red[kMaxPartials];
   // The blocks rank = 0..g.nblocks-1 of a team (the whole grid, or a group of
   // consecutive blocks of the batched kernel) that share the barrier state g.
-  __device__ GridTeam(const GridSync& g, unsigned rank) {
+  __device__ GridTeamT(const GridSync& g, unsigned rank) {
     bar.gs = g;
     bar.rank = rank;
+    bar.acquire = Acquire;
     const int wpb = blockDim.x / SB_WAVE;
     gw = (int)rank * wpb + wave_in_block();
     nwaves = (int)g.nblocks * wpb;
@@ -51,13 +58,16 @@ struct GridTeam {
   }
   template <typename T>
   __device__ static T ld(const T* p) {
-    return mld(p);
+    if (Acquire) return mld(p);
+    return cld(p);
   }
   template <typename T>
   __device__ static T cload(const T* p) {
     return cld(p);
   }
 };
+using GridTeam = GridTeamT<true>;
+using BatchTeam = GridTeamT<false>;
 
 struct BlockTeam {
   int gw, nwaves, tid, nthreads;

@@ -40,29 +40,47 @@ struct NoPre {  // row prefetch that loads nothing
   __device__ type operator()(int) const { return type{}; }
 };
 
-// Processes tile t of A. load(col, xc[BC]) returns the column vector (in the
-// accumulation type Acc). pre(row) is evaluated by every active lane as soon as
-// its row is known (so the row's own data is fetched while the gathers are in
-// flight); lead(row, v[BR], pre) consumes complete row sums in the row's last
-// lane. Partial sums of long tiles go to carry[t * BR + r].
+// Per-lane indices of one tile: the block range and, for lane l, the row and
+// column of block q0 + l (row -1 past the tile's end).
+struct TileIdx {
+  int q0, q1, row, col;
+};
+template <typename T>
+__device__ __forceinline__ TileIdx tile_indices(const TileMat<T>& A, int t) {
+  TileIdx ix;
+  ix.q0 = A.tstart[t];
+  ix.q1 = A.tstart[t + 1];
+  const int q = ix.q0 + lane_id();
+  if (q < ix.q1) {
+    ix.row = A.brow[q];
+    ix.col = A.bcol[q];
+  } else {
+    ix.row = -1;
+    ix.col = 0;
+  }
+  return ix;
+}
+
+// Processes tile t of A whose indices are ix. load(col, xc[BC]) returns the
+// column vector (in the accumulation type Acc). pre(row) is evaluated by every
+// active lane (so the row's own data is fetched while the gathers are in flight);
+// lead(row, v[BR], pre) consumes complete row sums in the row's last lane.
+// Partial sums of long tiles go to carry[t * BR + r].
 template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
-__device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& load, Pre& pre, Leader& lead,
-                                            Acc* carry) {
+__device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const TileIdx& ix, Loader& load, Pre& pre,
+                                            Leader& lead, Acc* carry) {
   const int l = lane_id();
-  const int q0 = A.tstart[t], q1 = A.tstart[t + 1];
-  const int q = q0 + l;
-  const bool act = q < q1;
-  int row = -1;
+  const int q = ix.q0 + l;
+  const bool act = ix.row >= 0;
+  const int row = ix.row;
   Acc v[BR];
 #pragma unroll
   for (int r = 0; r < BR; ++r) v[r] = Acc(0);
   decltype(pre(0)) rp{};
   if (act) {
-    row = A.brow[q];
-    const int col = A.bcol[q];
-    rp = pre(row);
     Acc xc[BC];
-    load(col, xc);
+    load(ix.col, xc);
+    rp = pre(row);
     const T* vq = A.vals + q;
     const size_t nb = (size_t)A.nblocks;
 #pragma unroll
@@ -93,6 +111,23 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, Loader& 
     } else {
       lead(row, v, rp);
     }
+  }
+}
+
+// Tiles first, first + stride, ... of A, software-pipelined: the indices of the
+// next tile are loaded before the current tile's gathers are issued, so each tile
+// costs one dependent memory round trip instead of three.
+template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
+__device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int stride, Loader& load, Pre& pre,
+                                          Leader& lead, Acc* carry) {
+  if (first >= A.ntiles) return;
+  TileIdx cur = tile_indices(A, first);
+  for (int t = first; t < A.ntiles; t += stride) {
+    const int tn = t + stride;
+    TileIdx nxt = cur;
+    if (tn < A.ntiles) nxt = tile_indices(A, tn);
+    tile_gather<BR, BC, T, Acc>(A, t, cur, load, pre, lead, carry);
+    cur = nxt;
   }
 }
 
