@@ -225,6 +225,8 @@ typedef struct sb_batch_stats {
   int* iterations;        /* optional caller array [n_problems] */
   double* residuals;      /* optional caller array [n_problems]: FP64 residual of each result */
   int* converged;         /* optional caller array [n_problems] */
+  int* stage_spmvs;       /* optional caller array [n_problems][2]: operator applications of the
+                             FP32 stage (0 in FP64 precision) and of the FP64 stage */
 } sb_batch_stats;
 
 /* Problems k = 0..n_problems-1 as in sb_problem_create (flags: SB_PROBLEM_ASSEMBLE_W). */
@@ -253,6 +255,8 @@ int sb_device_free(sb_ctx* ctx, void* ptr);
 int sb_memcpy_h2d(sb_ctx* ctx, void* dst, const void* src, size_t bytes);
 int sb_memcpy_d2h(sb_ctx* ctx, void* dst, const void* src, size_t bytes);
 int sb_synchronize(sb_ctx* ctx);
+/* Sets bytes of device memory to value (stream-ordered, completes before return). */
+int sb_memset_device(sb_ctx* ctx, void* dst, int value, size_t bytes);
 
 /* ---------------------------------------------------------------- benchmarking hooks
    Times `reps` back-to-back launches of one block-SpMV kernel of the problem on the

@@ -1144,6 +1144,15 @@ int sb_memcpy_d2h(sb_ctx* ctx, void* dst, const void* src, size_t bytes) {
   SB_HIP(hipStreamSynchronize(ctx->stream));
   return SB_OK;
 }
+int sb_memset_device(sb_ctx* ctx, void* dst, int value, size_t bytes) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!dst && bytes) return SB_ERR_INVALID;
+  SB_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
 int sb_synchronize(sb_ctx* ctx) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
@@ -1569,6 +1578,10 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
     if (o64[6] != 0.0 || (pl.mixed && o32[6] != 0.0) || !std::isfinite(o64[0])) ++failed;
     if (updateHints) B->iterHint[k] = std::max(1, it);
     if (st) {
+      if (st->stage_spmvs) {
+        st->stage_spmvs[2 * k] = pl.mixed ? (int)o32[2] : 0;
+        st->stage_spmvs[2 * k + 1] = (int)o64[2];
+      }
       if (st->iterations) st->iterations[k] = it;
       if (st->residuals) st->residuals[k] = o64[0];
       if (st->converged) st->converged[k] = conv ? 1 : 0;

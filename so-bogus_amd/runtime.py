@@ -40,7 +40,7 @@ EXPORTED = [
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_gs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
-    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune",
+    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_memset_device",
 ]
 
 
@@ -75,7 +75,7 @@ class BatchStats(C.Structure):
     _fields_ = [("n_converged", C.c_int), ("total_iterations", C.c_int), ("max_iterations", C.c_int),
                 ("refined_problems", C.c_int), ("total_spmvs", C.c_int64), ("device_ms", C.c_double),
                 ("config", BatchConfig), ("iterations", C.POINTER(C.c_int)), ("residuals", C.POINTER(C.c_double)),
-                ("converged", C.POINTER(C.c_int))]
+                ("converged", C.POINTER(C.c_int)), ("stage_spmvs", C.POINTER(C.c_int))]
 
 
 _dp = C.POINTER(C.c_double)
@@ -136,6 +136,7 @@ def lib():
         "sb_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
         "sb_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
         "sb_synchronize": (C.c_int, [_vp]),
+        "sb_memset_device": (C.c_int, [_vp, _vp, C.c_int, C.c_size_t]),
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
         "sb_last_phase_times": (C.c_int, [_vp, _dp]),
         "sb_bench_copy": (C.c_int, [_vp, C.c_size_t, C.c_int, _dp]),
@@ -239,6 +240,10 @@ class DeviceArray:
         a = _f64(a)
         assert a.size == self.n
         _check(lib().sb_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), 8 * self.n), "h2d")
+        return self
+
+    def zero(self):
+        _check(lib().sb_memset_device(self.ctx.h, self.ptr, 0, 8 * self.n), "sb_memset_device")
         return self
 
     def download(self) -> np.ndarray:
@@ -442,6 +447,7 @@ class BatchResult:
     refined_problems: int
     total_spmvs: int
     device_ms: float
+    stage_spmvs: np.ndarray         # [n][2] operator applications: FP32 stage, FP64 stage
     config: tuple[int, int, int]
 
 
@@ -491,7 +497,9 @@ class Batch:
         its = np.zeros(self.n, dtype=np.int32)
         res = np.zeros(self.n)
         conv = np.zeros(self.n, dtype=np.int32)
+        stage = np.zeros((self.n, 2), dtype=np.int32)
         st = BatchStats()
+        st.stage_spmvs = stage.ctypes.data_as(C.POINTER(C.c_int))
         st.iterations = its.ctypes.data_as(C.POINTER(C.c_int))
         st.residuals = _d(res)
         st.converged = conv.ctypes.data_as(C.POINTER(C.c_int))
@@ -506,7 +514,7 @@ class Batch:
         return BatchResult(status=code, r=rout, residuals=res, iterations=its, converged=conv.astype(bool),
                            n_converged=st.n_converged, total_iterations=st.total_iterations,
                            max_iterations=st.max_iterations, refined_problems=st.refined_problems,
-                           total_spmvs=st.total_spmvs, device_ms=st.device_ms,
+                           total_spmvs=st.total_spmvs, device_ms=st.device_ms, stage_spmvs=stage,
                            config=(st.config.block_threads, st.config.blocks_per_cu,
                                    st.config.contacts_per_block))
 
