@@ -78,16 +78,22 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
   for (int r = 0; r < BR; ++r) v[r] = Acc(0);
   decltype(pre(0)) rp{};
   if (act) {
-    Acc xc[BC];
-    load(ix.col, xc);
+    // Issue order matters: the row data and the block values are requested before
+    // the column gather, whose loader may branch on the gathered values (a
+    // projection) - loads placed after such branches wait for them.
     rp = pre(row);
     const T* vq = A.vals + q;
     const size_t nb = (size_t)A.nblocks;
+    T blk[BR * BC];
+#pragma unroll
+    for (int e = 0; e < BR * BC; ++e) blk[e] = vq[(size_t)e * nb];
+    Acc xc[BC];
+    load(ix.col, xc);
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
       Acc acc = Acc(0);
 #pragma unroll
-      for (int c = 0; c < BC; ++c) acc += Acc(vq[(size_t)(r * BC + c) * nb]) * xc[c];
+      for (int c = 0; c < BC; ++c) acc += Acc(blk[r * BC + c]) * xc[c];
       v[r] = acc;
     }
   }
@@ -117,9 +123,16 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
 // Tiles first, first + stride, ... of A, software-pipelined: the indices of the
 // next tile are loaded before the current tile's gathers are issued, so each tile
 // costs one dependent memory round trip instead of three.
+#ifndef SB_TILE_PIPELINE
+#define SB_TILE_PIPELINE 1
+#endif
 template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
 __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int stride, Loader& load, Pre& pre,
                                           Leader& lead, Acc* carry) {
+#if !SB_TILE_PIPELINE
+  for (int t = first; t < A.ntiles; t += stride) tile_gather<BR, BC, T, Acc>(A, t, tile_indices(A, t), load, pre, lead, carry);
+  return;
+#endif
   if (first >= A.ntiles) return;
   TileIdx cur = tile_indices(A, first);
   for (int t = first; t < A.ntiles; t += stride) {
