@@ -199,6 +199,44 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, dou
 int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
                 double* r, int on_device, double* residual, sb_stats* stats);
 
+/* ---------------------------------------------------------------- derivatives
+   Reverse-mode derivatives of L = g . r for the projected fixed-point map of the
+   dual problem, T(r) = P_mu(r - rho (W r + b + s_mu(W r + b))) (s_mu: the De Saxce
+   term of the Coulomb law, 0 for SOC), whose fixed points are the solutions of
+   the projected-gradient solvers. No counterpart in the reference (it has no
+   differentiation). Matrix-free (needs H and Minv). Host arrays. */
+enum sb_adjoint_mode {
+  SB_ADJOINT_UNROLL = 0,   /* L = g . T^K(r0): exact gradient of K steps, checkpointed */
+  SB_ADJOINT_IMPLICIT = 1  /* L = g . r* at a fixed point r0 = r*: implicit function theorem */
+};
+typedef struct sb_adjoint_options {
+  int mode;               /* sb_adjoint_mode */
+  int law;                /* sb_law (Coulomb by default) */
+  int iterations;         /* unroll: K (200) */
+  int checkpoint_every;   /* unroll: C (0: ceil(sqrt(K))) */
+  double rho;             /* step of the map; 0: 1 / lambda_max(W) by power iteration */
+  double tol;             /* implicit: relative change of the adjoint vector (1e-10) */
+  int max_iters;          /* implicit: iterations of the adjoint fixed point (20000) */
+  int grid_blocks;        /* 0 = automatic */
+} sb_adjoint_options;
+typedef struct sb_adjoint_stats {
+  double rho;             /* step used */
+  double loss;            /* g . r_K (unroll) or g . r* (implicit) */
+  int iterations;         /* unroll: K; implicit: adjoint iterations */
+  int converged;          /* implicit: relative change below tol */
+  double change;          /* implicit: last relative change */
+  double device_ms;
+} sb_adjoint_stats;
+void sb_adjoint_options_default(sb_adjoint_options* o);
+/* r0: start of the unroll or the fixed point (3 n_c). g: dL/dr (3 n_c) or NULL for
+   the forward map only. r_out (optional): T^K(r0). grad_b (3 n_c), grad_mu (n_c),
+   grad_minv (36 n_b, row-major 6x6 per body): outputs, each optional. */
+int sb_adjoint(sb_ctx* ctx, const sb_problem* P, const sb_adjoint_options* opts, const double* r0, const double* g,
+               double* r_out, double* grad_b, double* grad_mu, double* grad_minv, sb_adjoint_stats* stats);
+/* Replaces b, mu and/or Minv (NULL: unchanged) of a problem in place (H kept);
+   W, its diagonal and the row scalings are recomputed. */
+int sb_problem_update(sb_ctx* ctx, sb_problem* P, const double* b, const double* mu, const double* minv);
+
 /* ---------------------------------------------------------------- batches
    Many independent dual friction problems solved in one persistent launch: each
    workgroup takes whole problems from a queue, largest first, and solves them

@@ -1,0 +1,382 @@
+// Reverse-mode derivatives of the friction solve (no counterpart in the
+// reference, which has no differentiation).
+//
+// The differentiated map is the projected fixed-point step of the dual problem
+//   T(r) = P_mu(z),  z = r - rho (u + s_mu(u)),  u = W r + b,
+// with P_mu the projection on the cones K_mu (SOCLaw::projectOnConstraint,
+// SOCLaw.impl.hpp:47-76) and s_mu(u) = mu |u_T| e_N the De Saxce term
+// (SOCLaw.hpp:70-80; 0 for the SOC law). Its fixed points are the solutions the
+// projected-gradient solvers converge to. W = H Minv H^T is applied matrix-free.
+// For a loss L = g . r the kernels return dL/db, dL/dmu and dL/dMinv_k (6x6).
+//
+//  * unroll: r_{k+1} = T(r_k) for k < K from a given r_0, L = g . r_K; the
+//    reverse sweep visits k = K-1 .. 0 with lambda_K = g:
+//      mu_z = J_P(z_k)^T lambda_{k+1}
+//      mu_u = -rho (I + J_s(u_k))^T mu_z                    (= dL/du_k)
+//      dL/db += mu_u,  dL/dmu_i += lambda_i . dP/dmu - rho |u_T| mu_z,N
+//      dL/dMinv_k += (H^T mu_u)_k (H^T r_k)_k^T
+//      lambda_k = mu_z + W mu_u
+//    Iterates are recomputed segment by segment from checkpoints stored every C
+//    steps by the forward pass (memory (K/C + C) vectors instead of K).
+//  * implicit: at a fixed point r* = T(r*), lambda* = sum_j ((dT/dr)^T)^j g is
+//    accumulated by lambda <- g + (dT/dr)^T lambda with the Jacobians frozen at
+//    r*, until the relative change is below tol; the gradients are (dT/dtheta)^T
+//    lambda*. It converges when dT/dr is contracting on the relevant subspace
+//    (a unique, non-degenerate solution); the unroll is the fallback otherwise.
+#include <hip/hip_runtime.h>
+
+#include "sb_law.hpp"
+#include "sb_op.hpp"
+#include "sb_problem.hpp"
+#include "sb_team.hpp"
+
+namespace sb {
+
+namespace {
+
+template <typename Team>
+__device__ __forceinline__ void ld3t(const double* p, int i, double v[3]) {
+  v[0] = Team::ld(p + 3 * i);
+  v[1] = Team::ld(p + 3 * i + 1);
+  v[2] = Team::ld(p + 3 * i + 2);
+}
+template <typename Team>
+__device__ __forceinline__ void st3t(double* p, int i, const double v[3]) {
+  Team::st(p + 3 * i, v[0]);
+  Team::st(p + 3 * i + 1, v[1]);
+  Team::st(p + 3 * i + 2, v[2]);
+}
+
+// z = r - rho (u + s(u)) and |u_T| for contact i.
+__device__ __forceinline__ void step_point(bool coulomb, double mu, double rho, const double r[3], const double u[3],
+                                           double z[3], double& ut) {
+  ut = sqrt(u[1] * u[1] + u[2] * u[2]);
+  const double s0 = coulomb ? mu * ut : 0.0;
+  z[0] = r[0] - rho * (u[0] + s0);
+  z[1] = r[1] - rho * u[1];
+  z[2] = r[2] - rho * u[2];
+}
+
+// mu_z = J^T lam, mu_u = -rho (I + J_s)^T mu_z with J_s = e_N (mu u_T / |u_T|)^T.
+__device__ __forceinline__ void adjoint_point(bool coulomb, double mu, double rho, const double J[9], const double u[3],
+                                              double ut, const double lam[3], double mz[3], double mu_u[3]) {
+  mz[0] = J[0] * lam[0] + J[3] * lam[1] + J[6] * lam[2];
+  mz[1] = J[1] * lam[0] + J[4] * lam[1] + J[7] * lam[2];
+  mz[2] = J[2] * lam[0] + J[5] * lam[1] + J[8] * lam[2];
+  double a1 = 0.0, a2 = 0.0;
+  if (coulomb && ut > 0.0 && mu > 0.0) {
+    a1 = mu * u[1] / ut;
+    a2 = mu * u[2] / ut;
+  }
+  mu_u[0] = -rho * mz[0];
+  mu_u[1] = -rho * (mz[1] + a1 * mz[0]);
+  mu_u[2] = -rho * (mz[2] + a2 * mz[0]);
+}
+
+struct FwdRow {
+  double r[3], b[3], mu;
+};
+struct RevRow {
+  double r[3], b[3], lam[3], mu;
+};
+
+}  // namespace
+
+// Forward: K steps of T from r0 (checkpoints every C steps), the loss g . r_K.
+// Reverse (unless g is null): the sweep above. Implicit: Neumann iterations at r0.
+template <typename Team>
+__device__ void adjoint_body(const AdjArgs& a, Team& tm) {
+  const DevProblem<double>& P = a.P;
+  const int nc = P.nc, n3 = 3 * nc;
+  const bool coulomb = a.coulomb != 0;
+  const double rho = a.rho;
+  bool ok = true;
+  auto bodyDefault = [&](int k, double v[6]) {
+    double w[6];
+    minv_apply<double>(P.minv, P.nb, k, v, w);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) Team::st(&a.wbody[6 * k + r], w[r]);
+  };
+  // r_dst = T(r_src)
+  auto forward_step = [&](const double* src, double* dst) -> bool {
+    auto ld = [&](int j, double xc[3]) { ld3t<Team>(src, j, xc); };
+    auto pre = [&](int i) {
+      FwdRow w;
+      ld3t<Team>(src, i, w.r);
+      w.b[0] = P.b[3 * i];
+      w.b[1] = P.b[3 * i + 1];
+      w.b[2] = P.b[3 * i + 2];
+      w.mu = P.mu[i];
+      return w;
+    };
+    auto lead = [&](int i, double v[3], const FwdRow& w) {
+      const double u[3] = {v[0] + w.b[0], v[1] + w.b[1], v[2] + w.b[2]};
+      double z[3], ut;
+      step_point(coulomb, w.mu, rho, w.r, u, z, ut);
+      project<double>(w.mu, z[0], z[1], z[2]);
+      st3t<Team>(dst, i, z);
+    };
+    return apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, bodyDefault) && tm.sync();
+  };
+  auto copy = [&](double* dst, const double* src) {
+    for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(dst + k, Team::cload(src + k));
+  };
+  // zero the gradients
+  if (a.g) {
+    for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(a.grad_b + k, 0.0);
+    for (int k = tm.tid; k < nc; k += tm.nthreads) Team::st(a.grad_mu + k, 0.0);
+    for (int k = tm.tid; k < 36 * P.nb; k += tm.nthreads) Team::st(a.grad_minv + k, 0.0);
+  }
+  int iters = 0;
+  double change = 0.0;
+  bool converged = false;
+  if (a.mode == 0) {
+    // ---- forward with checkpoints
+    copy(a.ckpt, a.r0);
+    ok = tm.sync();
+    const int C = a.C;
+    for (int k = 0; k < a.K && ok; ++k) {
+      const double* src = (k % C == 0) ? a.ckpt + (size_t)(k / C) * n3 : a.seg + (size_t)(k & 1) * n3;
+      double* dst = ((k + 1) % C == 0 || k + 1 == a.K) ? a.ckpt + (size_t)((k + 1) / C + ((k + 1) % C ? 1 : 0)) * n3
+                                                       : a.seg + (size_t)((k + 1) & 1) * n3;
+      ok = forward_step(src, dst);
+    }
+    // r_K sits in checkpoint slot ceil(K / C)
+    const double* rK = a.ckpt + (size_t)((a.K + C - 1) / C) * n3;
+    if (ok) copy(a.rK, rK);
+    if (ok && a.g) {
+      double part[1] = {0.0};
+      for (int k = tm.tid; k < n3; k += tm.nthreads) part[0] += a.g[k] * Team::cload(rK + k);
+      ok = tm.template reduce<1>(part, 0u);
+      if (ok && tm.leader()) a.out[0] = tm.red[0];
+      // ---- reverse sweep, segment by segment
+      int cur = 0;
+      copy(a.lam, a.g);
+      ok = ok && tm.sync();
+      const int nseg = (a.K + C - 1) / C;
+      for (int sgi = nseg - 1; sgi >= 0 && ok; --sgi) {
+        const int k0 = sgi * C, k1 = (k0 + C < a.K) ? k0 + C : a.K;
+        // recompute r_k0 .. r_{k1-1} into seg[0 .. k1-k0)
+        copy(a.seg, a.ckpt + (size_t)sgi * n3);
+        ok = tm.sync();
+        for (int k = k0; k + 1 < k1 && ok; ++k)
+          ok = forward_step(a.seg + (size_t)(k - k0) * n3, a.seg + (size_t)(k - k0 + 1) * n3);
+        for (int k = k1 - 1; k >= k0 && ok; --k) {
+          const double* rk = a.seg + (size_t)(k - k0) * n3;
+          const double* lam = a.lam + (size_t)cur * n3;
+          double* lamN = a.lam + (size_t)(1 - cur) * n3;
+          // op A: u_k = W r_k + b, keeping (H^T r_k); local adjoint at contact level
+          {
+            auto ld = [&](int j, double xc[3]) { ld3t<Team>(rk, j, xc); };
+            auto body = [&](int kb, double v[6]) {
+#pragma unroll
+              for (int r = 0; r < 6; ++r) Team::st(&a.hr[6 * kb + r], v[r]);
+              bodyDefault(kb, v);
+            };
+            auto pre = [&](int i) {
+              RevRow w;
+              ld3t<Team>(rk, i, w.r);
+              ld3t<Team>(lam, i, w.lam);
+              w.b[0] = P.b[3 * i];
+              w.b[1] = P.b[3 * i + 1];
+              w.b[2] = P.b[3 * i + 2];
+              w.mu = P.mu[i];
+              return w;
+            };
+            auto lead = [&](int i, double v[3], const RevRow& w) {
+              const double u[3] = {v[0] + w.b[0], v[1] + w.b[1], v[2] + w.b[2]};
+              double z[3], ut, J[9], dmu[3], mz[3], muu[3];
+              step_point(coulomb, w.mu, rho, w.r, u, z, ut);
+              project_jacobian<double>(w.mu, z, J, dmu);
+              adjoint_point(coulomb, w.mu, rho, J, u, ut, w.lam, mz, muu);
+              st3t<Team>(a.mz, i, mz);
+              st3t<Team>(a.mu_u, i, muu);
+              // a row has one leader thread in every step (static tile schedule):
+              // read-modify-write through the coherent accessors
+#pragma unroll
+              for (int c = 0; c < 3; ++c) Team::st(a.grad_b + 3 * i + c, Team::cload(a.grad_b + 3 * i + c) + muu[c]);
+              Team::st(a.grad_mu + i, Team::cload(a.grad_mu + i) + w.lam[0] * dmu[0] + w.lam[1] * dmu[1] +
+                                          w.lam[2] * dmu[2] - (coulomb ? rho * ut * mz[0] : 0.0));
+            };
+            ok = apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, body) && tm.sync();
+          }
+          // op B: lambda_k = mu_z + W mu_u; dL/dMinv += (H^T mu_u)(H^T r_k)^T
+          if (ok) {
+            auto ld = [&](int j, double xc[3]) { ld3t<Team>(a.mu_u, j, xc); };
+            auto body = [&](int kb, double v[6]) {
+              double hr[6];
+#pragma unroll
+              for (int c = 0; c < 6; ++c) hr[c] = Team::ld(&a.hr[6 * kb + c]);
+              double* gm = a.grad_minv + (size_t)36 * kb;
+#pragma unroll
+              for (int r = 0; r < 6; ++r)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) Team::st(gm + 6 * r + c, Team::cload(gm + 6 * r + c) + v[r] * hr[c]);
+              bodyDefault(kb, v);
+            };
+            auto pre = [&](int i) {
+              double m[3];
+              ld3t<Team>(a.mz, i, m);
+              return FwdRow{{m[0], m[1], m[2]}, {0, 0, 0}, 0.0};
+            };
+            auto lead = [&](int i, double v[3], const FwdRow& w) {
+              const double l[3] = {w.r[0] + v[0], w.r[1] + v[1], w.r[2] + v[2]};
+              st3t<Team>(lamN, i, l);
+            };
+            ok = apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, body) && tm.sync();
+          }
+          cur = 1 - cur;
+        }
+      }
+      iters = a.K;
+      converged = ok;
+    }
+  } else {
+    // ---- implicit: frozen Jacobians at r* = r0
+    {
+      auto ld = [&](int j, double xc[3]) { ld3t<Team>(a.r0, j, xc); };
+      auto body = [&](int kb, double v[6]) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) Team::st(&a.hr[6 * kb + r], v[r]);
+        bodyDefault(kb, v);
+      };
+      auto pre = [&](int i) {
+        FwdRow w;
+        ld3t<Team>(a.r0, i, w.r);
+        w.b[0] = P.b[3 * i];
+        w.b[1] = P.b[3 * i + 1];
+        w.b[2] = P.b[3 * i + 2];
+        w.mu = P.mu[i];
+        return w;
+      };
+      auto lead = [&](int i, double v[3], const FwdRow& w) {
+        const double u[3] = {v[0] + w.b[0], v[1] + w.b[1], v[2] + w.b[2]};
+        double z[3], ut, J[9], dmu[3];
+        step_point(coulomb, w.mu, rho, w.r, u, z, ut);
+        project_jacobian<double>(w.mu, z, J, dmu);
+        double* jr = a.jac + (size_t)16 * i;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) Team::st(jr + e, J[e]);
+        Team::st(jr + 9, dmu[0]);
+        Team::st(jr + 10, dmu[1]);
+        Team::st(jr + 11, dmu[2]);
+        Team::st(jr + 12, u[0]);
+        Team::st(jr + 13, u[1]);
+        Team::st(jr + 14, u[2]);
+        Team::st(jr + 15, ut);
+      };
+      ok = apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, body) && tm.sync();
+    }
+    copy(a.lam, a.g);
+    ok = ok && tm.sync();
+    int cur = 0;
+    // mu_z, mu_u of lambda (contact-local)
+    auto local = [&](const double* lam) {
+      for (int i = tm.tid; i < nc; i += tm.nthreads) {
+        const double* jr = a.jac + (size_t)16 * i;
+        double J[9], u[3], l[3], mz[3], muu[3];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) J[e] = Team::ld(jr + e);
+        u[0] = Team::ld(jr + 12);
+        u[1] = Team::ld(jr + 13);
+        u[2] = Team::ld(jr + 14);
+        ld3t<Team>(lam, i, l);
+        adjoint_point(coulomb, P.mu[i], rho, J, u, Team::ld(jr + 15), l, mz, muu);
+        st3t<Team>(a.mz, i, mz);
+        st3t<Team>(a.mu_u, i, muu);
+      }
+    };
+    for (; ok && iters < a.max_iters; ++iters) {
+      const double* lam = a.lam + (size_t)cur * n3;
+      double* lamN = a.lam + (size_t)(1 - cur) * n3;
+      local(lam);
+      ok = tm.sync();
+      if (!ok) break;
+      double part[2] = {0.0, 0.0};
+      auto ld = [&](int j, double xc[3]) { ld3t<Team>(a.mu_u, j, xc); };
+      auto pre = [&](int i) {
+        RevRow w;
+        ld3t<Team>(a.mz, i, w.r);
+        ld3t<Team>(lam, i, w.lam);
+        w.b[0] = a.g[3 * i];
+        w.b[1] = a.g[3 * i + 1];
+        w.b[2] = a.g[3 * i + 2];
+        w.mu = 0.0;
+        return w;
+      };
+      auto lead = [&](int i, double v[3], const RevRow& w) {
+        double l[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          l[c] = w.b[c] + w.r[c] + v[c];
+          const double d = l[c] - w.lam[c];
+          part[0] += d * d;
+          part[1] += l[c] * l[c];
+        }
+        st3t<Team>(lamN, i, l);
+      };
+      ok = apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, bodyDefault) &&
+           tm.template reduce<2>(part, 0u);
+      if (!ok) break;
+      cur = 1 - cur;
+      change = sqrt(tm.red[0] / (tm.red[1] > 0.0 ? tm.red[1] : 1.0));
+      if (change < a.tol) {
+        ++iters;
+        converged = true;
+        break;
+      }
+    }
+    // gradients from lambda*
+    if (ok) {
+      const double* lam = a.lam + (size_t)cur * n3;
+      local(lam);
+      for (int i = tm.tid; i < nc; i += tm.nthreads) {
+        const double* jr = a.jac + (size_t)16 * i;
+        double l[3], mz[3], muu[3];
+        ld3t<Team>(lam, i, l);
+        ld3t<Team>(a.mz, i, mz);
+        ld3t<Team>(a.mu_u, i, muu);
+        Team::st(a.grad_b + 3 * i, muu[0]);
+        Team::st(a.grad_b + 3 * i + 1, muu[1]);
+        Team::st(a.grad_b + 3 * i + 2, muu[2]);
+        Team::st(a.grad_mu + i, l[0] * Team::ld(jr + 9) + l[1] * Team::ld(jr + 10) + l[2] * Team::ld(jr + 11) -
+                                    (coulomb ? rho * Team::ld(jr + 15) * mz[0] : 0.0));
+      }
+      ok = tm.sync();
+      if (ok) {
+        auto ld = [&](int j, double xc[3]) { ld3t<Team>(a.mu_u, j, xc); };
+        auto body = [&](int kb, double v[6]) {
+          double hr[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) hr[c] = Team::ld(&a.hr[6 * kb + c]);
+#pragma unroll
+          for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Team::st(&a.grad_minv[(size_t)36 * kb + 6 * r + c], v[r] * hr[c]);
+          bodyDefault(kb, v);
+        };
+        NoPre nopre;
+        auto lead = [&](int, double*, NoPre::type) {};
+        ok = apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, nopre, lead, body);
+      }
+      // the loss at r*
+      double part[1] = {0.0};
+      for (int k = tm.tid; k < n3; k += tm.nthreads) part[0] += a.g[k] * a.r0[k];
+      ok = ok && tm.template reduce<1>(part, 0u);
+      if (ok && tm.leader()) a.out[0] = tm.red[0];
+    }
+  }
+  if (tm.leader()) {
+    a.out[1] = (double)iters;
+    a.out[2] = change;
+    a.out[3] = converged ? 1.0 : 0.0;
+    a.out[4] = ok ? 0.0 : 1.0;
+  }
+}
+
+__global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
+  GridTeam tm(a.gs, blockIdx.x);
+  adjoint_body<GridTeam>(a, tm);
+}
+
+}  // namespace sb

@@ -254,17 +254,11 @@ std::vector<double> minv_element_major(int nb, const std::vector<double>& minv) 
   return em;
 }
 
-int upload_problem(sb_problem* P) {
+// Values that sb_problem_update may replace: W (when assembled), Minv, b, mu,
+// the diagonal blocks and the row scalings.
+int upload_values(sb_problem* P) {
   hipStream_t s = P->ctx->stream;
   const int nc = P->nc, nb = P->nb;
-  // H contact-major, H^T body-major
-  SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
-  {
-    std::vector<int> trp, tci;
-    std::vector<double> tv;
-    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv);
-    SB_HIP(P->HT.upload(sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data()), s));
-  }
   if (P->hasW) {
     std::vector<int> frp, fci;
     std::vector<double> fv;
@@ -298,6 +292,22 @@ int upload_problem(sb_problem* P) {
   SB_HIP(P->d_scal.upload(P->scal, s));
   std::vector<double> ones(nc, 1.0);
   SB_HIP(P->d_ones.upload(ones, s));
+  return SB_OK;
+}
+
+int upload_problem(sb_problem* P) {
+  hipStream_t s = P->ctx->stream;
+  const int nc = P->nc, nb = P->nb;
+  // H contact-major, H^T body-major
+  SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
+  {
+    std::vector<int> trp, tci;
+    std::vector<double> tv;
+    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv);
+    SB_HIP(P->HT.upload(sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data()), s));
+  }
+  int rc = upload_values(P);
+  if (rc) return rc;
   const size_t n3 = 3 * (size_t)nc;
   for (int k = 0; k < 3; ++k) {
     SB_HIP(P->X[k].alloc(n3));
@@ -1117,6 +1127,156 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
 }
 
 // ------------------------------------------------------------------ memory helpers
+int sb_problem_update(sb_ctx* ctx, sb_problem* P, const double* b, const double* mu, const double* minv) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!P) return SB_ERR_INVALID;
+  if (b) P->b.assign(b, b + 3 * (size_t)P->nc);
+  if (mu) P->mu.assign(mu, mu + P->nc);
+  if (minv) {
+    P->minv.assign(minv, minv + 36 * (size_t)P->nb);
+    if (P->hasW)
+      sbhost::assembleW(P->nc, P->nb, P->hrp.data(), P->hci.data(), P->hv.data(), P->minv.data(), P->wrp, P->wci,
+                        P->wv);
+  }
+  rc = upload_values(P);
+  if (rc) return rc;
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+void sb_adjoint_options_default(sb_adjoint_options* o) {
+  if (!o) return;
+  o->mode = SB_ADJOINT_UNROLL;
+  o->law = SB_LAW_COULOMB;
+  o->iterations = 200;
+  o->checkpoint_every = 0;
+  o->rho = 0.0;
+  o->tol = 1e-10;
+  o->max_iters = 20000;
+  o->grid_blocks = 0;
+}
+
+int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, const double* r0, const double* g,
+               double* r_out, double* grad_b, double* grad_mu, double* grad_minv, sb_adjoint_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r0) return SB_ERR_INVALID;
+  if (o->mode != SB_ADJOINT_UNROLL && o->mode != SB_ADJOINT_IMPLICIT) return SB_ERR_INVALID;
+  if (o->mode == SB_ADJOINT_IMPLICIT && !g) return SB_ERR_INVALID;
+  if (o->mode == SB_ADJOINT_UNROLL && o->iterations < 1) return SB_ERR_INVALID;
+  if (o->mode == SB_ADJOINT_IMPLICIT && (o->max_iters < 1 || !(o->tol > 0))) return SB_ERR_INVALID;
+  if (!(o->rho >= 0) || o->checkpoint_every < 0) return SB_ERR_INVALID;
+  if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  hipStream_t s = ctx->stream;
+  const int nc = P->nc, nb = P->nb;
+  const size_t n3 = 3 * (size_t)nc;
+  const DevProblem<double> V = P->view(true);
+  // step of the map: 1 / lambda_max(W), lambda_max by power iteration on the device
+  double rho = o->rho;
+  if (rho == 0.0) {
+    std::vector<double> x(n3), y(n3);
+    for (size_t k = 0; k < n3; ++k) x[k] = 1.0 + 0.5 * std::sin(0.7 * (double)k);
+    double lam = 0.0;
+    for (int it = 0; it < 40; ++it) {
+      double nx = 0.0;
+      for (double v : x) nx += v * v;
+      nx = std::sqrt(nx);
+      for (double& v : x) v /= nx;
+      SB_HIP(hipMemcpyAsync(P->X[0].p, x.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s));
+      SB_HIP(sb::launch_matfree(V, P->X[0].p, P->y.p, P->wbody.p, P->carry.p, s));
+      SB_HIP(hipMemcpyAsync(y.data(), P->y.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s));
+      SB_HIP(hipStreamSynchronize(s));
+      double ny = 0.0;
+      for (double v : y) ny += v * v;
+      lam = std::sqrt(ny);
+      x.swap(y);
+    }
+    if (!(lam > 0)) return SB_ERR_NAN;
+    rho = 1.0 / lam;
+  }
+  const int K = o->iterations;
+  int C = o->checkpoint_every;
+  if (C == 0) C = std::max(1, (int)std::ceil(std::sqrt((double)K)));
+  const int nseg = (K + C - 1) / C;
+  DBuf<double> r0d, gd, rK, ckpt, seg, lam, mz, muu, hr, jac, gb, gmu, gminv, out;
+  SB_HIP(r0d.upload(r0, n3, s));
+  if (g) SB_HIP(gd.upload(g, n3, s));
+  SB_HIP(rK.alloc(n3));
+  SB_HIP(lam.alloc(2 * n3));
+  SB_HIP(mz.alloc(n3));
+  SB_HIP(muu.alloc(n3));
+  SB_HIP(hr.alloc(6 * (size_t)nb));
+  SB_HIP(gb.alloc(n3));
+  SB_HIP(gmu.alloc(nc));
+  SB_HIP(gminv.alloc(36 * (size_t)nb));
+  SB_HIP(out.alloc(8));
+  if (o->mode == SB_ADJOINT_UNROLL) {
+    SB_HIP(ckpt.alloc((size_t)(nseg + 1) * n3));
+    SB_HIP(seg.alloc((size_t)std::max(C, 2) * n3));
+  } else {
+    SB_HIP(jac.alloc(16 * (size_t)nc));
+  }
+  sb::AdjArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.P = V;
+  a.mode = o->mode;
+  a.K = K;
+  a.C = C;
+  a.coulomb = o->law == SB_LAW_COULOMB;
+  a.rho = rho;
+  a.tol = o->tol;
+  a.max_iters = o->max_iters;
+  a.r0 = r0d.p;
+  a.g = g ? gd.p : nullptr;
+  a.rK = rK.p;
+  a.ckpt = ckpt.p;
+  a.seg = seg.p;
+  a.lam = lam.p;
+  a.mz = mz.p;
+  a.mu_u = muu.p;
+  a.hr = hr.p;
+  a.jac = jac.p;
+  a.wbody = P->wbody.p;
+  a.carry = P->carry.p;
+  a.grad_b = gb.p;
+  a.grad_mu = gmu.p;
+  a.grad_minv = gminv.p;
+  a.out = out.p;
+  const void* fn = (const void*)&sb::adjoint_kernel;
+  const int threads = kMaxThreads;
+  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, std::max(P->H.host.ntiles, P->HT.host.ntiles));
+  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
+  a.gs = make_sync(ctx, grid);
+  SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
+  void* args[] = {&a};
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  double res[8];
+  SB_HIP(hipMemcpyAsync(res, out.p, sizeof(res), hipMemcpyDeviceToHost, s));
+  if (r_out) SB_HIP(hipMemcpyAsync(r_out, o->mode == SB_ADJOINT_UNROLL ? rK.p : r0d.p, sizeof(double) * n3,
+                                   hipMemcpyDeviceToHost, s));
+  if (g && grad_b) SB_HIP(hipMemcpyAsync(grad_b, gb.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s));
+  if (g && grad_mu) SB_HIP(hipMemcpyAsync(grad_mu, gmu.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  if (g && grad_minv) SB_HIP(hipMemcpyAsync(grad_minv, gminv.p, sizeof(double) * 36 * nb, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  if (st) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    st->rho = rho;
+    st->loss = g ? res[0] : 0.0;
+    st->iterations = (int)res[1];
+    st->change = res[2];
+    st->converged = (int)res[3];
+    st->device_ms = ms;
+  }
+  if (res[4] != 0.0) return SB_ERR_BARRIER_TIMEOUT;
+  if (o->mode == SB_ADJOINT_IMPLICIT && res[3] == 0.0) return SB_NOT_CONVERGED;
+  return SB_OK;
+}
+
 int sb_device_alloc(sb_ctx* ctx, void** ptr, size_t bytes) {
   int rc = check_ctx(ctx);
   if (rc) return rc;

@@ -95,4 +95,39 @@ __device__ __forceinline__ T law_eval(bool coulomb, T mu, const T x[3], const T 
   return fb_canonical_sq(xh, yh);
 }
 
+// Projection on K_mu with its (generalized) Jacobian J = dP/dz (row-major 3x3)
+// and dP/dmu, for the reverse-mode (adjoint) sweeps. The branch is chosen with
+// the same tests as project(); on the boundary between branches the derivative
+// of the branch taken is returned.
+template <typename T>
+__device__ __forceinline__ void project_jacobian(T mu, const T z[3], T J[9], T dmu[3]) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) J[e] = T(0);
+  dmu[0] = dmu[1] = dmu[2] = T(0);
+  if (mu < T(0)) return;
+  const T t = sqrt(z[1] * z[1] + z[2] * z[2]);
+  if (t <= mu * z[0]) {  // inside the cone: identity
+    J[0] = J[4] = J[8] = T(1);
+    return;
+  }
+  if (mu * t <= -z[0]) return;  // inside the polar cone: zero
+  const T den = T(1) + mu * mu;
+  const T a = (z[0] + mu * t) / den;
+  const T n1 = z[1] / t, n2 = z[2] / t;
+  const T c = mu * a / t;
+  J[0] = T(1) / den;
+  J[1] = mu * n1 / den;
+  J[2] = mu * n2 / den;
+  J[3] = mu * n1 / den;
+  J[4] = mu * mu * n1 * n1 / den + c * (T(1) - n1 * n1);
+  J[5] = mu * mu * n1 * n2 / den - c * n1 * n2;
+  J[6] = mu * n2 / den;
+  J[7] = J[5];
+  J[8] = mu * mu * n2 * n2 / den + c * (T(1) - n2 * n2);
+  const T da = (t - T(2) * mu * z[0] - mu * mu * t) / (den * den);
+  dmu[0] = da;
+  dmu[1] = (a + mu * da) * n1;
+  dmu[2] = (a + mu * da) * n2;
+}
+
 }  // namespace sb

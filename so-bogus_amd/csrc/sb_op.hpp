@@ -1,0 +1,67 @@
+// One application of the Delassus operator inside a persistent solver, by a
+// team: the assembled W (3x3 blocks), or matrix-free H (Minv (H^T x)) with a
+// caller-provided body step between the H^T and the H passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sb_problem.hpp"
+#include "sb_tiles.hpp"
+
+namespace sb {
+
+// Minv_k v for the element-major [36][nb] storage.
+template <typename T>
+__device__ __forceinline__ void minv_apply(const T* minv, int nb, int k, const T v[6], T w[6]) {
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    T acc = T(0);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc += minv[(size_t)(r * 6 + c) * nb + k] * v[c];
+    w[r] = acc;
+  }
+}
+
+// M x with x given by the loader, finished by the leader on complete contact rows
+// (with the row data fetched by pre). Matrix-free: body(k, v) receives (H^T x)_k
+// and must store the body vector that the H pass reads from wbody[6k..6k+6).
+// Contains 0 (assembled, no long rows), 1 (matrix-free) or 2 (matrix-free with
+// hub bodies) internal team barriers.
+template <bool MF, typename T, typename Team, typename Loader, typename Pre, typename Leader, typename Body>
+__device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* wbody, T* carry, Team& tm, Loader& ld,
+                                               Pre& pre, Leader& lead, Body& body) {
+  if (!MF) {
+    tile_loop<3, 3, T, T>(P.W, tm.gw, tm.nwaves, ld, pre, lead, carry);
+    if (P.W.nlong > 0) {
+      if (!tm.sync()) return false;
+      for (int k = tm.gw; k < P.W.nlong; k += tm.nwaves) {
+        T v[3];
+        long_row_sum<3, T, T>(P.W, k, carry, v);
+        if (lane_id() == 0) {
+          const int row = P.W.long_row[k];
+          lead(row, v, pre(row));
+        }
+      }
+    }
+    return true;
+  }
+  NoPre nopre;
+  auto leadA = [&](int k, T v[6], NoPre::type) { body(k, v); };
+  tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry);
+  if (!tm.sync()) return false;
+  if (P.HT.nlong > 0) {
+    for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {
+      T v[6];
+      long_row_sum<6, T, T>(P.HT, k, carry, v);
+      if (lane_id() == 0) body(P.HT.long_row[k], v);
+    }
+    if (!tm.sync()) return false;
+  }
+  auto ldB = [&](int k, T xc[6]) {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) xc[r] = Team::ld(&wbody[6 * k + r]);
+  };
+  tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, carry);
+  return true;
+}
+
+}  // namespace sb

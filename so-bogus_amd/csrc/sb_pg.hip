@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sb_law.hpp"
+#include "sb_op.hpp"
 #include "sb_problem.hpp"
 #include "sb_team.hpp"
 
@@ -65,63 +66,18 @@ __device__ __forceinline__ void widen3(const T v[3], double d[3]) {
   d[2] = (double)v[2];
 }
 
-// Minv_k v for the element-major [36][nb] storage.
-template <typename T>
-__device__ __forceinline__ void minv_apply(const T* minv, int nb, int k, const T v[6], T w[6]) {
-#pragma unroll
-  for (int r = 0; r < 6; ++r) {
-    T acc = T(0);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) acc += minv[(size_t)(r * 6 + c) * nb + k] * v[c];
-    w[r] = acc;
-  }
-}
-
-// One operator application M x with x given by the loader, finished by the
-// leader on complete contact rows (with the row data fetched by pre). Contains
-// 0 (assembled, no long rows), 1 (matrix-free) or 2 (matrix-free with hub
-// bodies) internal team barriers.
+// One operator application M x (assembled W, or H (Minv (H^T x)) with the
+// default body step: Minv applied and stored for the H pass).
 template <bool MF, typename T, typename Team, typename Loader, typename Pre, typename Leader>
 __device__ __forceinline__ bool apply_op(const PGProblem<T>& pb, Team& tm, Loader& ld, Pre& pre, Leader& lead) {
   const DevProblem<T>& P = pb.P;
-  if (!MF) {
-    tile_loop<3, 3, T, T>(P.W, tm.gw, tm.nwaves, ld, pre, lead, pb.carry);
-    if (P.W.nlong > 0) {
-      if (!tm.sync()) return false;
-      for (int k = tm.gw; k < P.W.nlong; k += tm.nwaves) {
-        T v[3];
-        long_row_sum<3, T, T>(P.W, k, pb.carry, v);
-        if (lane_id() == 0) {
-          const int row = P.W.long_row[k];
-          lead(row, v, pre(row));
-        }
-      }
-    }
-    return true;
-  }
-  NoPre nopre;
-  auto leadA = [&](int k, T v[6], NoPre::type) {
+  auto body = [&](int k, T v[6]) {
     T w[6];
     minv_apply<T>(P.minv, P.nb, k, v, w);
 #pragma unroll
     for (int r = 0; r < 6; ++r) Team::st(&pb.wbody[6 * k + r], w[r]);
   };
-  tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, pb.carry);
-  if (!tm.sync()) return false;
-  if (P.HT.nlong > 0) {
-    for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {
-      T v[6];
-      long_row_sum<6, T, T>(P.HT, k, pb.carry, v);
-      if (lane_id() == 0) leadA(P.HT.long_row[k], v, NoPre::type{});
-    }
-    if (!tm.sync()) return false;
-  }
-  auto ldB = [&](int k, T xc[6]) {
-#pragma unroll
-    for (int r = 0; r < 6; ++r) xc[r] = Team::ld(&pb.wbody[6 * k + r]);
-  };
-  tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, pb.carry);
-  return true;
+  return apply_operator<MF, T>(P, pb.wbody, pb.carry, tm, ld, pre, lead, body);
 }
 
 __device__ __forceinline__ double nesterov(double& theta, double q) {

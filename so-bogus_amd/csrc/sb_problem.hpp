@@ -96,6 +96,34 @@ struct PGBatchArgs {
   uint64_t timeout_ticks;
 };
 
+// Reverse-mode derivatives of the projected fixed-point map (matrix-free, FP64).
+struct AdjArgs {
+  DevProblem<double> P;
+  int mode;             // 0: unrolled K steps from r0; 1: implicit at the fixed point r0
+  int K, C;             // unroll length, checkpoint interval
+  int coulomb;          // De Saxce term on (Coulomb law) or off (SOC law)
+  double rho;           // step of the map
+  double tol;           // implicit: relative change of lambda
+  int max_iters;        // implicit: Neumann iterations
+  const double* r0;     // [3 nc] start (unroll) or fixed point (implicit)
+  const double* g;      // [3 nc] dL/dr (null: forward only)
+  double* rK;           // [3 nc] out: r_K (unroll)
+  double* ckpt;         // [ceil(K/C) + 1][3 nc]
+  double* seg;          // [max(C, 2)][3 nc]
+  double* lam;          // [2][3 nc]
+  double* mz;           // [3 nc]
+  double* mu_u;         // [3 nc]
+  double* hr;           // [6 nb] H^T r of the current step
+  double* jac;          // implicit: [nc][16] J_P (9), dP/dmu (3), u (3), |u_T|
+  double* wbody;        // [6 nb]
+  double* carry;
+  double* grad_b;       // [3 nc] out
+  double* grad_mu;      // [nc] out
+  double* grad_minv;    // [nb][36] out (row-major 6x6 per body)
+  GridSync gs;
+  double* out;          // [0] loss [1] iterations [2] last relative change [3] converged [4] failed
+};
+
 struct GSArgs {
   int nc;
   int coulomb;

@@ -28,6 +28,7 @@ LAW_SOC, LAW_COULOMB = 0, 1
 OP_ASSEMBLED, OP_MATRIX_FREE = 0, 1
 SCALING_REFERENCE, SCALING_DIAGONAL, SCALING_NONE = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+ADJOINT_UNROLL, ADJOINT_IMPLICIT = 0, 1
 COLORING_REFERENCE, COLORING_GREEDY = 0, 1
 BSR_SYMMETRIC = 1
 PROBLEM_ASSEMBLE_W = 1
@@ -40,7 +41,8 @@ EXPORTED = [
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_gs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
-    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_memset_device",
+    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_memset_device", "sb_adjoint_options_default",
+    "sb_adjoint", "sb_problem_update",
 ]
 
 
@@ -65,6 +67,16 @@ class Stats(C.Structure):
                 ("device_ms", C.c_double), ("history", C.POINTER(C.c_double)), ("history_capacity", C.c_int),
                 ("history_length", C.c_int), ("refine_iterations", C.c_int), ("refine_spmvs", C.c_int),
                 ("mixed_residual", C.c_double)]
+
+
+class AdjointOptions(C.Structure):
+    _fields_ = [("mode", C.c_int), ("law", C.c_int), ("iterations", C.c_int), ("checkpoint_every", C.c_int),
+                ("rho", C.c_double), ("tol", C.c_double), ("max_iters", C.c_int), ("grid_blocks", C.c_int)]
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("rho", C.c_double), ("loss", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
+                ("change", C.c_double), ("device_ms", C.c_double)]
 
 
 class BatchConfig(C.Structure):
@@ -137,6 +149,10 @@ def lib():
         "sb_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
         "sb_synchronize": (C.c_int, [_vp]),
         "sb_memset_device": (C.c_int, [_vp, _vp, C.c_int, C.c_size_t]),
+        "sb_adjoint_options_default": (None, [C.POINTER(AdjointOptions)]),
+        "sb_adjoint": (C.c_int, [_vp, _vp, C.POINTER(AdjointOptions), _dp, _dp, _dp, _dp, _dp, _dp,
+                                 C.POINTER(AdjointStats)]),
+        "sb_problem_update": (C.c_int, [_vp, _vp, _dp, _dp, _dp]),
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
         "sb_last_phase_times": (C.c_int, [_vp, _dp]),
         "sb_bench_copy": (C.c_int, [_vp, C.c_size_t, C.c_int, _dp]),
@@ -427,6 +443,37 @@ class DualFrictionProblem:
             cp, nco = _l(colors), len(colors) - 1
         fn = lambda rp, dev, res, st: lib().sb_gs_solve(self.ctx.h, self.h, C.byref(o), cp, nco, rp, dev, res, st)
         return self._run(fn, r0, on_device, hist_cap)
+
+    def update(self, b=None, mu=None, minv=None):
+        """Replace b, mu and/or Minv in place (sb_problem_update)."""
+        keep = [None if v is None else _f64(v).reshape(-1) for v in (b, mu, minv)]
+        ptr = [None if v is None else _d(v) for v in keep]
+        _check(lib().sb_problem_update(self.ctx.h, self.h, *ptr), "sb_problem_update")
+
+    def adjoint(self, r0, g=None, mode=ADJOINT_UNROLL, law=LAW_COULOMB, iterations=200, checkpoint_every=0, rho=0.0,
+                tol=1e-10, max_iters=20000) -> dict:
+        """Forward map T^K(r0) (g None) or the gradients of L = g . r (sb_adjoint)."""
+        o = AdjointOptions()
+        lib().sb_adjoint_options_default(C.byref(o))
+        o.mode, o.law, o.iterations, o.checkpoint_every = mode, law, iterations, checkpoint_every
+        o.rho, o.tol, o.max_iters = rho, tol, max_iters
+        r0 = _f64(r0)
+        rout = np.zeros(3 * self.nc)
+        st = AdjointStats()
+        out = {"r": rout}
+        if g is None:
+            code = lib().sb_adjoint(self.ctx.h, self.h, C.byref(o), _d(r0), None, _d(rout), None, None, None,
+                                    C.byref(st))
+        else:
+            g = _f64(g)
+            gb, gm, gM = np.zeros(3 * self.nc), np.zeros(self.nc), np.zeros(36 * self.nb)
+            code = lib().sb_adjoint(self.ctx.h, self.h, C.byref(o), _d(r0), _d(g), _d(rout), _d(gb), _d(gm), _d(gM),
+                                    C.byref(st))
+            out.update(grad_b=gb, grad_mu=gm, grad_minv=gM.reshape(self.nb, 36))
+        _check(code, "sb_adjoint", allow_not_converged=True)
+        out.update(status=code, rho=st.rho, loss=st.loss, iterations=st.iterations, converged=bool(st.converged),
+                   change=st.change, device_ms=st.device_ms)
+        return out
 
     def bench_spmv(self, kind: int, reps: int = 20) -> float:
         ms = C.c_double()
