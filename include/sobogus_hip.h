@@ -218,6 +218,9 @@ typedef struct sb_adjoint_options {
   double tol;             /* implicit: relative change of the adjoint vector (1e-10) */
   int max_iters;          /* implicit: iterations of the adjoint fixed point (20000) */
   int grid_blocks;        /* 0 = automatic */
+  int branch_mode;        /* unroll: 0 projection branches from the cone tests; 1 also record them
+                             into branches; 2 replay them from branches (the map is then one smooth
+                             piece: finite differences see no kinks) */
 } sb_adjoint_options;
 typedef struct sb_adjoint_stats {
   double rho;             /* step used */
@@ -230,9 +233,11 @@ typedef struct sb_adjoint_stats {
 void sb_adjoint_options_default(sb_adjoint_options* o);
 /* r0: start of the unroll or the fixed point (3 n_c). g: dL/dr (3 n_c) or NULL for
    the forward map only. r_out (optional): T^K(r0). grad_b (3 n_c), grad_mu (n_c),
-   grad_minv (36 n_b, row-major 6x6 per body): outputs, each optional. */
+   grad_minv (36 n_b, row-major 6x6 per body): outputs, each optional. branches:
+   K * n_c bytes (step-major) when branch_mode is 1 or 2, else NULL. */
 int sb_adjoint(sb_ctx* ctx, const sb_problem* P, const sb_adjoint_options* opts, const double* r0, const double* g,
-               double* r_out, double* grad_b, double* grad_mu, double* grad_minv, sb_adjoint_stats* stats);
+               double* r_out, double* grad_b, double* grad_mu, double* grad_minv, unsigned char* branches,
+               sb_adjoint_stats* stats);
 /* Replaces b, mu and/or Minv (NULL: unchanged) of a problem in place (H kept);
    W, its diagonal and the row scalings are recomputed. */
 int sb_problem_update(sb_ctx* ctx, sb_problem* P, const double* b, const double* mu, const double* minv);

@@ -97,8 +97,15 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
 #pragma unroll
     for (int r = 0; r < 6; ++r) Team::st(&a.wbody[6 * k + r], w[r]);
   };
-  // r_dst = T(r_src)
-  auto forward_step = [&](const double* src, double* dst) -> bool {
+  // branch of contact i at step k: from the tests (recorded when asked) or replayed
+  auto branch_of = [&](int k, int i, double mu, const double z[3]) -> int {
+    if (a.branch_mode == 2) return a.branch[(size_t)k * nc + i];
+    const int br = project_branch<double>(mu, z);
+    if (a.branch_mode == 1) a.branch[(size_t)k * nc + i] = (unsigned char)br;
+    return br;
+  };
+  // r_dst = T(r_src), step k
+  auto forward_step = [&](const double* src, double* dst, int k) -> bool {
     auto ld = [&](int j, double xc[3]) { ld3t<Team>(src, j, xc); };
     auto pre = [&](int i) {
       FwdRow w;
@@ -113,7 +120,7 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
       const double u[3] = {v[0] + w.b[0], v[1] + w.b[1], v[2] + w.b[2]};
       double z[3], ut;
       step_point(coulomb, w.mu, rho, w.r, u, z, ut);
-      project<double>(w.mu, z[0], z[1], z[2]);
+      project_on_branch<double>(branch_of(k, i, w.mu, z), w.mu, z);
       st3t<Team>(dst, i, z);
     };
     return apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, bodyDefault) && tm.sync();
@@ -139,7 +146,7 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
       const double* src = (k % C == 0) ? a.ckpt + (size_t)(k / C) * n3 : a.seg + (size_t)(k & 1) * n3;
       double* dst = ((k + 1) % C == 0 || k + 1 == a.K) ? a.ckpt + (size_t)((k + 1) / C + ((k + 1) % C ? 1 : 0)) * n3
                                                        : a.seg + (size_t)((k + 1) & 1) * n3;
-      ok = forward_step(src, dst);
+      ok = forward_step(src, dst, k);
     }
     // r_K sits in checkpoint slot ceil(K / C)
     const double* rK = a.ckpt + (size_t)((a.K + C - 1) / C) * n3;
@@ -160,7 +167,7 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
         copy(a.seg, a.ckpt + (size_t)sgi * n3);
         ok = tm.sync();
         for (int k = k0; k + 1 < k1 && ok; ++k)
-          ok = forward_step(a.seg + (size_t)(k - k0) * n3, a.seg + (size_t)(k - k0 + 1) * n3);
+          ok = forward_step(a.seg + (size_t)(k - k0) * n3, a.seg + (size_t)(k - k0 + 1) * n3, k);
         for (int k = k1 - 1; k >= k0 && ok; --k) {
           const double* rk = a.seg + (size_t)(k - k0) * n3;
           const double* lam = a.lam + (size_t)cur * n3;
@@ -187,7 +194,8 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
               const double u[3] = {v[0] + w.b[0], v[1] + w.b[1], v[2] + w.b[2]};
               double z[3], ut, J[9], dmu[3], mz[3], muu[3];
               step_point(coulomb, w.mu, rho, w.r, u, z, ut);
-              project_jacobian<double>(w.mu, z, J, dmu);
+              project_jacobian<double>(w.mu, z, J, dmu,
+                                       a.branch_mode == 2 ? (int)a.branch[(size_t)k * nc + i] : -1);
               adjoint_point(coulomb, w.mu, rho, J, u, ut, w.lam, mz, muu);
               st3t<Team>(a.mz, i, mz);
               st3t<Team>(a.mu_u, i, muu);

@@ -1155,10 +1155,12 @@ void sb_adjoint_options_default(sb_adjoint_options* o) {
   o->tol = 1e-10;
   o->max_iters = 20000;
   o->grid_blocks = 0;
+  o->branch_mode = 0;
 }
 
 int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, const double* r0, const double* g,
-               double* r_out, double* grad_b, double* grad_mu, double* grad_minv, sb_adjoint_stats* st) {
+               double* r_out, double* grad_b, double* grad_mu, double* grad_minv, unsigned char* branches,
+               sb_adjoint_stats* st) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!Pc || !o || !r0) return SB_ERR_INVALID;
@@ -1167,6 +1169,8 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   if (o->mode == SB_ADJOINT_UNROLL && o->iterations < 1) return SB_ERR_INVALID;
   if (o->mode == SB_ADJOINT_IMPLICIT && (o->max_iters < 1 || !(o->tol > 0))) return SB_ERR_INVALID;
   if (!(o->rho >= 0) || o->checkpoint_every < 0) return SB_ERR_INVALID;
+  if (o->branch_mode < 0 || o->branch_mode > 2) return SB_ERR_INVALID;
+  if (o->branch_mode != 0 && (!branches || o->mode != SB_ADJOINT_UNROLL)) return SB_ERR_INVALID;
   if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   sb_problem* P = const_cast<sb_problem*>(Pc);
   hipStream_t s = ctx->stream;
@@ -1201,6 +1205,10 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   if (C == 0) C = std::max(1, (int)std::ceil(std::sqrt((double)K)));
   const int nseg = (K + C - 1) / C;
   DBuf<double> r0d, gd, rK, ckpt, seg, lam, mz, muu, hr, jac, gb, gmu, gminv, out;
+  DBuf<unsigned char> br;
+  const size_t nbr = (size_t)K * nc;
+  if (o->branch_mode == 2) SB_HIP(br.upload(branches, nbr, s));
+  if (o->branch_mode == 1) SB_HIP(br.alloc(nbr));
   SB_HIP(r0d.upload(r0, n3, s));
   if (g) SB_HIP(gd.upload(g, n3, s));
   SB_HIP(rK.alloc(n3));
@@ -1243,6 +1251,8 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   a.grad_b = gb.p;
   a.grad_mu = gmu.p;
   a.grad_minv = gminv.p;
+  a.branch = br.p;
+  a.branch_mode = o->branch_mode;
   a.out = out.p;
   const void* fn = (const void*)&sb::adjoint_kernel;
   const int threads = kMaxThreads;
@@ -1261,6 +1271,7 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   if (g && grad_b) SB_HIP(hipMemcpyAsync(grad_b, gb.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s));
   if (g && grad_mu) SB_HIP(hipMemcpyAsync(grad_mu, gmu.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
   if (g && grad_minv) SB_HIP(hipMemcpyAsync(grad_minv, gminv.p, sizeof(double) * 36 * nb, hipMemcpyDeviceToHost, s));
+  if (o->branch_mode == 1) SB_HIP(hipMemcpyAsync(branches, br.p, nbr, hipMemcpyDeviceToHost, s));
   SB_HIP(hipStreamSynchronize(s));
   if (st) {
     float ms = 0.f;

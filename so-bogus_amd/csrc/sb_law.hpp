@@ -95,22 +95,48 @@ __device__ __forceinline__ T law_eval(bool coulomb, T mu, const T x[3], const T 
   return fb_canonical_sq(xh, yh);
 }
 
+// Branch of the projection on K_mu taken by project(): 0 inside the cone
+// (identity), 1 inside the polar cone or mu < 0 (zero), 2 onto the cone's boundary.
+template <typename T>
+__device__ __forceinline__ int project_branch(T mu, const T z[3]) {
+  if (mu < T(0)) return 1;
+  const T t = sqrt(z[1] * z[1] + z[2] * z[2]);
+  if (t <= mu * z[0]) return 0;
+  if (mu * t <= -z[0]) return 1;
+  return 2;
+}
+// The projection formula of a given branch (same arithmetic as project()).
+template <typename T>
+__device__ __forceinline__ void project_on_branch(int branch, T mu, T z[3]) {
+  if (branch == 0) return;
+  const T t = sqrt(z[1] * z[1] + z[2] * z[2]);
+  if (branch == 1 || !(t > T(0))) {
+    z[0] = z[1] = z[2] = T(0);
+    return;
+  }
+  const T proj = (z[0] + mu * t) / (T(1) + mu * mu);
+  const T f = proj * mu / t;
+  z[0] = proj;
+  z[1] *= f;
+  z[2] *= f;
+}
+
 // Projection on K_mu with its (generalized) Jacobian J = dP/dz (row-major 3x3)
 // and dP/dmu, for the reverse-mode (adjoint) sweeps. The branch is chosen with
 // the same tests as project(); on the boundary between branches the derivative
 // of the branch taken is returned.
 template <typename T>
-__device__ __forceinline__ void project_jacobian(T mu, const T z[3], T J[9], T dmu[3]) {
+__device__ __forceinline__ void project_jacobian(T mu, const T z[3], T J[9], T dmu[3], int branch = -1) {
 #pragma unroll
   for (int e = 0; e < 9; ++e) J[e] = T(0);
   dmu[0] = dmu[1] = dmu[2] = T(0);
-  if (mu < T(0)) return;
+  if (branch < 0) branch = project_branch<T>(mu, z);
   const T t = sqrt(z[1] * z[1] + z[2] * z[2]);
-  if (t <= mu * z[0]) {  // inside the cone: identity
+  if (branch == 0) {  // inside the cone: identity
     J[0] = J[4] = J[8] = T(1);
     return;
   }
-  if (mu * t <= -z[0]) return;  // inside the polar cone: zero
+  if (branch == 1 || !(t > T(0))) return;  // inside the polar cone: zero
   const T den = T(1) + mu * mu;
   const T a = (z[0] + mu * t) / den;
   const T n1 = z[1] / t, n2 = z[2] / t;

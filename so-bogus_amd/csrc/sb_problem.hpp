@@ -120,6 +120,8 @@ struct AdjArgs {
   double* grad_b;       // [3 nc] out
   double* grad_mu;      // [nc] out
   double* grad_minv;    // [nb][36] out (row-major 6x6 per body)
+  unsigned char* branch;  // unroll: [K][nc] projection branch per step (null: not kept)
+  int branch_mode;      // 0: branches from the projection tests, 1: record them, 2: replay them
   GridSync gs;
   double* out;          // [0] loss [1] iterations [2] last relative change [3] converged [4] failed
 };

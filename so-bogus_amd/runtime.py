@@ -71,7 +71,8 @@ class Stats(C.Structure):
 
 class AdjointOptions(C.Structure):
     _fields_ = [("mode", C.c_int), ("law", C.c_int), ("iterations", C.c_int), ("checkpoint_every", C.c_int),
-                ("rho", C.c_double), ("tol", C.c_double), ("max_iters", C.c_int), ("grid_blocks", C.c_int)]
+                ("rho", C.c_double), ("tol", C.c_double), ("max_iters", C.c_int), ("grid_blocks", C.c_int),
+                ("branch_mode", C.c_int)]
 
 
 class AdjointStats(C.Structure):
@@ -150,7 +151,7 @@ def lib():
         "sb_synchronize": (C.c_int, [_vp]),
         "sb_memset_device": (C.c_int, [_vp, _vp, C.c_int, C.c_size_t]),
         "sb_adjoint_options_default": (None, [C.POINTER(AdjointOptions)]),
-        "sb_adjoint": (C.c_int, [_vp, _vp, C.POINTER(AdjointOptions), _dp, _dp, _dp, _dp, _dp, _dp,
+        "sb_adjoint": (C.c_int, [_vp, _vp, C.POINTER(AdjointOptions), _dp, _dp, _dp, _dp, _dp, _dp, _vp,
                                  C.POINTER(AdjointStats)]),
         "sb_problem_update": (C.c_int, [_vp, _vp, _dp, _dp, _dp]),
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
@@ -451,25 +452,38 @@ class DualFrictionProblem:
         _check(lib().sb_problem_update(self.ctx.h, self.h, *ptr), "sb_problem_update")
 
     def adjoint(self, r0, g=None, mode=ADJOINT_UNROLL, law=LAW_COULOMB, iterations=200, checkpoint_every=0, rho=0.0,
-                tol=1e-10, max_iters=20000) -> dict:
-        """Forward map T^K(r0) (g None) or the gradients of L = g . r (sb_adjoint)."""
+                tol=1e-10, max_iters=20000, record_branches=False, branches=None) -> dict:
+        """Forward map T^K(r0) (g None) or the gradients of L = g . r (sb_adjoint).
+        record_branches: also return the projection branch of every contact and step;
+        branches: replay them (the map then stays on one smooth piece)."""
         o = AdjointOptions()
         lib().sb_adjoint_options_default(C.byref(o))
         o.mode, o.law, o.iterations, o.checkpoint_every = mode, law, iterations, checkpoint_every
         o.rho, o.tol, o.max_iters = rho, tol, max_iters
+        br = None
+        if branches is not None:
+            br = np.ascontiguousarray(branches, dtype=np.uint8)
+            assert br.size == iterations * self.nc
+            o.branch_mode = 2
+        elif record_branches:
+            br = np.zeros(iterations * self.nc, dtype=np.uint8)
+            o.branch_mode = 1
+        brp = None if br is None else br.ctypes.data_as(_vp)
         r0 = _f64(r0)
         rout = np.zeros(3 * self.nc)
         st = AdjointStats()
         out = {"r": rout}
         if g is None:
-            code = lib().sb_adjoint(self.ctx.h, self.h, C.byref(o), _d(r0), None, _d(rout), None, None, None,
+            code = lib().sb_adjoint(self.ctx.h, self.h, C.byref(o), _d(r0), None, _d(rout), None, None, None, brp,
                                     C.byref(st))
         else:
             g = _f64(g)
             gb, gm, gM = np.zeros(3 * self.nc), np.zeros(self.nc), np.zeros(36 * self.nb)
             code = lib().sb_adjoint(self.ctx.h, self.h, C.byref(o), _d(r0), _d(g), _d(rout), _d(gb), _d(gm), _d(gM),
-                                    C.byref(st))
+                                    brp, C.byref(st))
             out.update(grad_b=gb, grad_mu=gm, grad_minv=gM.reshape(self.nb, 36))
+        if record_branches and branches is None:
+            out["branches"] = br.reshape(iterations, self.nc)
         _check(code, "sb_adjoint", allow_not_converged=True)
         out.update(status=code, rho=st.rho, loss=st.loss, iterations=st.iterations, converged=bool(st.converged),
                    change=st.change, device_ms=st.device_ms)
