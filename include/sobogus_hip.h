@@ -104,7 +104,8 @@ typedef struct sb_pg_options {
   int precision;          /* sb_precision, FP64 by default */
   double mixed_tol_factor;/* mixed: the FP32 stage stops at tol * factor (0.8) */
   int mixed_stall_iters;  /* mixed: the FP32 stage also stops after this many iterations without a 1%
-                             decrease of its best residual (100; its rounding floor was reached) */
+                             decrease of its best residual once that is below 10 tol (its rounding
+                             floor), or after 4x as many anywhere (100) */
 } sb_pg_options;
 
 /* GaussSeidel parameters (GaussSeidelBase.hpp:39-91) */

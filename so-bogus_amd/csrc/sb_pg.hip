@@ -231,7 +231,14 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
       stallRef = minRes;
       lastImprove = it;
     }
-    if (prm.stall_iters > 0 && it - lastImprove >= prm.stall_iters) break;
+    // FP32 stage of a mixed solve: hand over to FP64 once the best residual has not
+    // decreased by 1% for stall_iters iterations near the tolerance (the FP32
+    // rounding floor), or for 4 x stall_iters anywhere (an APGD plateau far above
+    // the tolerance is not a precision floor and usually ends by itself)
+    if (prm.stall_iters > 0) {
+      const int since = it - lastImprove;
+      if ((since >= prm.stall_iters && minRes < 10.0 * prm.tol) || since >= 4 * prm.stall_iters) break;
+    }
     int rxs = 0;
     while (rxs == rx || rxs == rprev) ++rxs;
     const T* Xx = pb.X[rx];
@@ -508,6 +515,14 @@ __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
   pg_body<MF, T, GridTeam>(a.prm, a.pb, tm, tsum);
 }
 
+// Teams of several workgroups invalidate their caches after each barrier and read
+// the vectors other workgroups wrote with plain loads (measured faster than
+// coherent loads without invalidation, SB_BATCH_TEAM_ACQUIRE=0).
+#ifndef SB_BATCH_TEAM_ACQUIRE
+#define SB_BATCH_TEAM_ACQUIRE 1
+#endif
+using BatchTeamSel = GridTeamT<SB_BATCH_TEAM_ACQUIRE != 0>;
+
 // Many problems. Block b belongs to the class whose block range holds it; its
 // team takes the next problem of the class's queue and solves it alone. Teams
 // never wait for other teams. A team of k > 1 blocks needs its blocks resident
@@ -548,7 +563,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
   g.partials = a.team_partials + (size_t)gt * 2 * kMaxTeamBlocks * kMaxPartials;
   g.nblocks = (unsigned)cl.k;
   g.timeout_ticks = a.timeout_ticks;
-  BatchTeam tm(g, rank);
+  BatchTeamSel tm(g, rank);
   for (;;) {
     // the team leader draws the next problem and hands it over through the barrier
     if (rank == 0 && threadIdx.x == 0) cst(&a.team_slot[gt], (int)atomicAdd(&a.queue[c], 1u));
@@ -561,7 +576,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
     if (k >= nq) break;
     const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
     uint64_t tsum[4] = {0, 0, 0, 0};
-    pg_body<MF, T, BatchTeam>(a.prm, probs[p], tm, tsum);
+    pg_body<MF, T, BatchTeamSel>(a.prm, probs[p], tm, tsum);
     if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
   }
 }

@@ -37,7 +37,8 @@ struct PGParams {
   int ls_iters;
   double ls_opt, ls_pess;
   int stall_iters;  // > 0: stop (not converged) after this many iterations without a 1% decrease of
-                    // the best residual (the FP32 stage of a mixed solve hands over to FP64)
+                    // the best residual within 10 tol (4x as many above): the FP32 stage of a mixed
+                    // solve hands over to FP64
 };
 
 // One problem's device data and workspace; T is the storage (and bulk

@@ -20,10 +20,8 @@
 namespace sb {
 
 // Acquire = true: the barrier invalidates this XCD's caches and vectors written by
-// other blocks are read with plain loads (best when the whole chip runs one
-// problem). Acquire = false: no invalidation; such vectors are read with
-// coherent loads (many small teams share each XCD's L2: an invalidation by one
-// team would evict the others' matrices).
+// other blocks are read with plain loads. Acquire = false: no invalidation; such
+// vectors are read with coherent loads.
 template <bool Acquire>
 struct GridTeamT {
   Barrier bar;
@@ -67,7 +65,6 @@ struct GridTeamT {
   }
 };
 using GridTeam = GridTeamT<true>;
-using BatchTeam = GridTeamT<false>;
 
 struct BlockTeam {
   int gw, nwaves, tid, nthreads;
