@@ -235,6 +235,7 @@ def main() -> int:
         ms = dp50.bench_spmv(kind, reps=50)
         spmv[name] = {"ms": ms, "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}
     del r50, dp50
+    zipf = None
     if not args.no_large:
         p1m = G.workload("zipf-1m")
         dp1m = R.DualFrictionProblem(ctx, p1m, assemble=False)
@@ -245,8 +246,25 @@ def main() -> int:
             ms = dp1m.bench_spmv(kind, reps=20)
             spmv[name] = {"ms": ms, "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9,
                           "frac_of_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        del dp1m
+        # the 1M-contact Zipf problem has no solution at FB 1e-6 (its hubs over-constrain the
+        # bodies); FP64 vs mixed is compared on a fixed number of APGD iterations
+        zipf = {}
+        r1m = R.DeviceArray(ctx, 3 * p1m.nc)
+        for name, prec in (("fp64", R.PRECISION_FP64), ("mixed", R.PRECISION_MIXED)):
+            o = R.pg_options(precision=prec, mixed_stall_iters=0, **{**kw, "max_iters": 100})
+            times = []
+            for _ in range(2):
+                r1m.zero()
+                s1 = dp1m.solve_pg(r1m, o, on_device=True)
+                times.append(s1.device_ms)
+            zipf[name] = {"ms": min(times), "iterations": s1.iterations, "spmvs": s1.spmvs,
+                          "us_per_operator_application": 1e3 * min(times) / s1.spmvs, "residual": s1.residual}
+        zipf["mixed_speedup_per_application"] = (zipf["fp64"]["us_per_operator_application"] /
+                                                 zipf["mixed"]["us_per_operator_application"])
+        del r1m, dp1m
         log("1M-contact SpMV: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in spmv.items() if k.endswith("1m")))
+        log(f"zipf-1m, 100 APGD iterations: FP64 {zipf['fp64']['us_per_operator_application']:.0f} us, mixed "
+            f"{zipf['mixed']['us_per_operator_application']:.0f} us per operator application")
     ms_copy = ctx.bench_copy(COPY_BYTES, reps=10)
     copy_gbs = 2 * COPY_BYTES / (ms_copy * 1e-3) / 1e9
 
@@ -288,6 +306,7 @@ def main() -> int:
                      "model": "sum over problems of operator applications x compulsory bytes of one matrix-free "
                               "W x (4-byte values in the FP32 stage, 8-byte in the FP64 stage)"},
         "spmv": spmv,
+        "zipf_1m_fixed_iterations": zipf,
         "stream_copy_GBs": copy_gbs,
     }
     if not args.no_cpu:

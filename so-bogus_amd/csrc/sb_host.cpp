@@ -44,17 +44,22 @@ Tiles buildTiles(int nrows, int ncols, int br, int bc, const int* rowptr, const 
   }
   T.nblocks = (int)srcBlock.size();
   // tiles: whole rows packed greedily into <= 64 blocks; longer rows get their own tiles
+  // A whole-row tile records -(1 + scan steps) for its own longest row; a long-row
+  // tile records the long row's index.
   T.tstart.push_back(0);
   int maxLen = 1;
-  int cur = 0;  // blocks in the open tile
+  int cur = 0;     // blocks in the open tile
+  int curMax = 1;  // longest row in the open tile
+  auto closeTile = [&](int end) {
+    T.tstart.push_back(end);
+    T.tlong.push_back(-1 - ceilLog2(curMax));
+    cur = 0;
+    curMax = 1;
+  };
   for (int i = 0; i < nrows; ++i) {
     const int len = T.rowptr[i + 1] - T.rowptr[i];
     if (len > W) {
-      if (cur > 0) {
-        T.tstart.push_back(T.rowptr[i]);
-        T.tlong.push_back(-1);
-        cur = 0;
-      }
+      if (cur > 0) closeTile(T.rowptr[i]);
       const int k = (int)T.long_row.size();
       T.long_row.push_back(i);
       T.long_t0.push_back((int)T.tstart.size() - 1);
@@ -66,18 +71,12 @@ Tiles buildTiles(int nrows, int ncols, int br, int bc, const int* rowptr, const 
       maxLen = W;
       continue;
     }
-    if (cur + len > W) {
-      T.tstart.push_back(T.rowptr[i]);
-      T.tlong.push_back(-1);
-      cur = 0;
-    }
+    if (cur + len > W) closeTile(T.rowptr[i]);
     cur += len;
+    curMax = std::max(curMax, len);
     maxLen = std::max(maxLen, len);
   }
-  if (cur > 0) {
-    T.tstart.push_back(T.nblocks);
-    T.tlong.push_back(-1);
-  }
+  if (cur > 0) closeTile(T.nblocks);
   T.ntiles = (int)T.tlong.size();
   T.scan_steps = ceilLog2(maxLen);
   // element-major values

@@ -35,7 +35,7 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
       if (!tm.sync()) return false;
       for (int k = tm.gw; k < P.W.nlong; k += tm.nwaves) {
         T v[3];
-        long_row_sum<3, T, T>(P.W, k, carry, v);
+        long_row_sum<3, T, T, Team::kCoherentReads>(P.W, k, carry, v);
         if (lane_id() == 0) {
           const int row = P.W.long_row[k];
           lead(row, v, pre(row));
@@ -51,7 +51,7 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
   if (P.HT.nlong > 0) {
     for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {
       T v[6];
-      long_row_sum<6, T, T>(P.HT, k, carry, v);
+      long_row_sum<6, T, T, Team::kCoherentReads>(P.HT, k, carry, v);
       if (lane_id() == 0) body(P.HT.long_row[k], v);
     }
     if (!tm.sync()) return false;

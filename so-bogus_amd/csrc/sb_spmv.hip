@@ -133,24 +133,55 @@ __global__ void __launch_bounds__(256) k_tiles(TileMat<double> A, LD ld, LE le, 
   const int t = blockIdx.x * wpb + wave_in_block();
   if (t >= A.ntiles) return;
   NoPre pre;
-  tile_gather<BR, BC, double, double>(A, t, tile_indices(A, t), ld, pre, le, carry);
+  // the block values are streamed once: non-temporal loads keep the gathered vectors cached
+  tile_gather<BR, BC, double, double, true>(A, t, tile_indices(A, t), ld, pre, le, carry);
 }
 
+// One block per long row: the row's carries (written by the previous kernel) are
+// summed by every thread with four tiles in flight, then across the block in a
+// fixed order.
 template <int BR, typename LE>
-__global__ void k_long(TileMat<double> A, LE le, const double* carry) {
-  const int wpb = blockDim.x / SB_WAVE;
-  const int k = blockIdx.x * wpb + wave_in_block();
-  if (k >= A.nlong) return;
+__global__ void __launch_bounds__(256) k_long(TileMat<double> A, LE le, const double* __restrict__ carry) {
+  __shared__ double s_v[BR][4];
+  const int k = blockIdx.x;
+  const int t0 = A.long_t0[k], t1 = A.long_t1[k];
   double v[BR];
-  long_row_sum<BR, double, double>(A, k, carry, v);
-  if (lane_id() == 0) le(A.long_row[k], v);
+#pragma unroll
+  for (int r = 0; r < BR; ++r) v[r] = 0.0;
+  int t = t0 + threadIdx.x;
+  for (; t + 3 * 256 < t1; t += 4 * 256) {
+    double c[4][BR];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < BR; ++r) c[j][r] = carry[(size_t)(t + j * 256) * BR + r];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < BR; ++r) v[r] += c[j][r];
+  }
+  for (; t < t1; t += 256) {
+#pragma unroll
+    for (int r = 0; r < BR; ++r) v[r] += carry[(size_t)t * BR + r];
+  }
+#pragma unroll
+  for (int r = 0; r < BR; ++r) {
+    v[r] = wave_sum(v[r]);
+    if (lane_id() == 0) s_v[r][wave_in_block()] = v[r];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < BR; ++r) v[r] = (s_v[r][0] + s_v[r][1]) + (s_v[r][2] + s_v[r][3]);
+    le(A.long_row[k], v);
+  }
 }
 
 template <int BR, int BC, typename LD, typename LE>
 static void run_tiles(const TileMat<double>& A, LD ld, LE le, double* carry, hipStream_t s) {
   const int threads = 256, wpb = threads / SB_WAVE;
   if (A.ntiles > 0) k_tiles<BR, BC><<<(A.ntiles + wpb - 1) / wpb, threads, 0, s>>>(A, ld, le, carry);
-  if (A.nlong > 0) k_long<BR><<<(A.nlong + wpb - 1) / wpb, threads, 0, s>>>(A, le, carry);
+  if (A.nlong > 0) k_long<BR><<<A.nlong, 256, 0, s>>>(A, le, carry);
 }
 
 hipError_t launch_w(const DevProblem<double>& P, const double* x, double* y, double* carry, hipStream_t s) {

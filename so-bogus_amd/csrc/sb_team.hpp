@@ -24,6 +24,8 @@ namespace sb {
 // vectors are read with coherent loads.
 template <bool Acquire>
 struct GridTeamT {
+  // vectors written by other blocks must be read with coherent loads
+  static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
   Barrier bar;
   int gw, nwaves, tid, nthreads;
   bool lead;
@@ -67,6 +69,7 @@ struct GridTeamT {
 using GridTeam = GridTeamT<true>;
 
 struct BlockTeam {
+  static constexpr bool kCoherentReads = false;
   int gw, nwaves, tid, nthreads;
   double red[kMaxPartials];
   __device__ BlockTeam() {

@@ -27,7 +27,8 @@ struct TileMat {
   const int* tstart = nullptr; // [ntiles + 1] block offsets
   const int* brow = nullptr;   // [nblocks]
   const int* bcol = nullptr;   // [nblocks]
-  const int* tlong = nullptr;  // [ntiles] -1: whole rows; >= 0: tile holds a piece of that long row
+  const int* tlong = nullptr;  // [ntiles] < 0: whole rows, -(1 + scan steps of the tile);
+                               // >= 0: the tile holds a piece of that long row
   const T* vals = nullptr;     // [BR*BC][nblocks]
   int nlong = 0;               // long rows
   const int* long_row = nullptr;  // [nlong]
@@ -43,13 +44,14 @@ struct NoPre {  // row prefetch that loads nothing
 // Per-lane indices of one tile: the block range and, for lane l, the row and
 // column of block q0 + l (row -1 past the tile's end).
 struct TileIdx {
-  int q0, q1, row, col;
+  int q0, q1, row, col, tl;
 };
 template <typename T>
 __device__ __forceinline__ TileIdx tile_indices(const TileMat<T>& A, int t) {
   TileIdx ix;
   ix.q0 = A.tstart[t];
   ix.q1 = A.tstart[t + 1];
+  ix.tl = A.tlong[t];
   const int q = ix.q0 + lane_id();
   if (q < ix.q1) {
     ix.row = A.brow[q];
@@ -66,7 +68,7 @@ __device__ __forceinline__ TileIdx tile_indices(const TileMat<T>& A, int t) {
 // active lane (so the row's own data is fetched while the gathers are in flight);
 // lead(row, v[BR], pre) consumes complete row sums in the row's last lane.
 // Partial sums of long tiles go to carry[t * BR + r].
-template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
+template <int BR, int BC, typename T, typename Acc, bool NT = false, typename Loader, typename Pre, typename Leader>
 __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const TileIdx& ix, Loader& load, Pre& pre,
                                             Leader& lead, Acc* carry) {
   const int l = lane_id();
@@ -86,7 +88,7 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
     const size_t nb = (size_t)A.nblocks;
     T blk[BR * BC];
 #pragma unroll
-    for (int e = 0; e < BR * BC; ++e) blk[e] = vq[(size_t)e * nb];
+    for (int e = 0; e < BR * BC; ++e) blk[e] = NT ? __builtin_nontemporal_load(vq + (size_t)e * nb) : vq[(size_t)e * nb];
     Acc xc[BC];
     load(ix.col, xc);
 #pragma unroll
@@ -97,7 +99,8 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
       v[r] = acc;
     }
   }
-  for (int s = 0; s < A.scan_steps; ++s) {
+  const int steps = ix.tl >= 0 ? 6 : -ix.tl - 1;  // a long tile is one row across the wave
+  for (int s = 0; s < steps; ++s) {
     const int d = 1 << s;
     const int ru = __shfl_up(row, d, SB_WAVE);
     Acc vu[BR];
@@ -111,7 +114,7 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
   const int rn = __shfl_down(row, 1, SB_WAVE);
   const bool leader = act && (l == SB_WAVE - 1 || rn != row);
   if (leader) {
-    if (A.tlong[t] >= 0) {
+    if (ix.tl >= 0) {
 #pragma unroll
       for (int r = 0; r < BR; ++r) cst(&carry[(size_t)t * BR + r], v[r]);
     } else {
@@ -145,15 +148,30 @@ __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int st
 }
 
 // Sum of the carries of long row k (one wave), fixed order: lane-strided partial
-// sums then the xor tree. Result valid in every lane.
-template <int BR, typename T, typename Acc>
+// sums (four tiles in flight per lane) then the xor tree. Result valid in every
+// lane. Coherent: the carries were written by other blocks of the same kernel and
+// no cache invalidation separates the reads from the writes; otherwise plain loads.
+template <int BR, typename T, typename Acc, bool Coherent = true>
 __device__ __forceinline__ void long_row_sum(const TileMat<T>& A, int k, const Acc* carry, Acc v[BR]) {
   const int t0 = A.long_t0[k], t1 = A.long_t1[k];
 #pragma unroll
   for (int r = 0; r < BR; ++r) v[r] = Acc(0);
-  for (int t = t0 + lane_id(); t < t1; t += SB_WAVE) {
+  auto ld = [&](size_t i) { return Coherent ? cld(carry + i) : carry[i]; };
+  int t = t0 + lane_id();
+  for (; t + 3 * SB_WAVE < t1; t += 4 * SB_WAVE) {
+    Acc c[4][BR];
 #pragma unroll
-    for (int r = 0; r < BR; ++r) v[r] += cld(&carry[(size_t)t * BR + r]);
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < BR; ++r) c[j][r] = ld((size_t)(t + j * SB_WAVE) * BR + r);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < BR; ++r) v[r] += c[j][r];
+  }
+  for (; t < t1; t += SB_WAVE) {
+#pragma unroll
+    for (int r = 0; r < BR; ++r) v[r] += ld((size_t)t * BR + r);
   }
 #pragma unroll
   for (int r = 0; r < BR; ++r) v[r] = wave_sum(v[r]);
