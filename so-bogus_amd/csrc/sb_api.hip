@@ -163,9 +163,9 @@ struct sb_problem {
   DBuf<double> d_minv, d_b, d_mu, d_diag, d_scal, d_ones;
   DBuf<float> d_minv32;
   // workspace (FP64 solvers and kernels)
-  DBuf<double> X[3], MX[3], G[2], xbest, wbody, carry, rio, hist, y, terms, local, reg, ndx;
+  DBuf<double> X[4], MX[4], G[3], xbest, wbody, carry, rio, hist, y, terms, local, reg, ndx;
   // workspace of the FP32 (mixed-precision) projected-gradient solver
-  DBuf<float> X32[3], MX32[3], G32[2], xbest32, wbody32, carry32;
+  DBuf<float> X32[4], MX32[4], G32[3], xbest32, wbody32, carry32;
   DBuf<unsigned char> skip;
   DBuf<int> colors;
   DBuf<double> out;  // [2][16] solver results (FP32 stage, FP64 stage)
@@ -203,24 +203,22 @@ struct sb_problem {
   // set by the caller.
   void fill(sb::PGProblem<double>& pb, bool withScaling) const {
     pb.P = view(withScaling);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       pb.X[k] = X[k].p;
       pb.MX[k] = MX[k].p;
     }
-    pb.G[0] = G[0].p;
-    pb.G[1] = G[1].p;
+    for (int k = 0; k < 3; ++k) pb.G[k] = G[k].p;
     pb.xbest = xbest.p;
     pb.wbody = wbody.p;
     pb.carry = carry.p;
   }
   void fill(sb::PGProblem<float>& pb, bool withScaling) const {
     pb.P = view32(withScaling);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       pb.X[k] = X32[k].p;
       pb.MX[k] = MX32[k].p;
     }
-    pb.G[0] = G32[0].p;
-    pb.G[1] = G32[1].p;
+    for (int k = 0; k < 3; ++k) pb.G[k] = G32[k].p;
     pb.xbest = xbest32.p;
     pb.wbody = wbody32.p;
     pb.carry = carry32.p;
@@ -309,12 +307,11 @@ int upload_problem(sb_problem* P) {
   int rc = upload_values(P);
   if (rc) return rc;
   const size_t n3 = 3 * (size_t)nc;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     SB_HIP(P->X[k].alloc(n3));
     SB_HIP(P->MX[k].alloc(n3));
   }
-  SB_HIP(P->G[0].alloc(n3));
-  SB_HIP(P->G[1].alloc(n3));
+  for (int k = 0; k < 3; ++k) SB_HIP(P->G[k].alloc(n3));
   SB_HIP(P->xbest.alloc(n3));
   SB_HIP(P->rio.alloc(n3));
   SB_HIP(P->y.alloc(n3));
@@ -322,12 +319,11 @@ int upload_problem(sb_problem* P) {
   SB_HIP(P->wbody.alloc(6 * (size_t)nb));
   size_t carry = std::max({P->H.carryNeed(3), P->HT.carryNeed(6), P->W.carryNeed(3), (size_t)8});
   SB_HIP(P->carry.alloc(carry));
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     SB_HIP(P->X32[k].alloc(n3));
     SB_HIP(P->MX32[k].alloc(n3));
   }
-  SB_HIP(P->G32[0].alloc(n3));
-  SB_HIP(P->G32[1].alloc(n3));
+  for (int k = 0; k < 3; ++k) SB_HIP(P->G32[k].alloc(n3));
   SB_HIP(P->xbest32.alloc(n3));
   SB_HIP(P->wbody32.alloc(6 * (size_t)nb));
   SB_HIP(P->carry32.alloc(carry));

@@ -97,7 +97,7 @@ struct GridSync {
 };
 constexpr unsigned kSyncGroups = 16;  // blocks are grouped by blockIdx % kSyncGroups
 constexpr unsigned kSyncStride = 32;  // one 128-byte line per counter
-constexpr int kMaxPartials = 8;
+constexpr int kMaxPartials = 12;
 
 __device__ __forceinline__ uint64_t realtime() { return __builtin_amdgcn_s_memrealtime(); }
 

@@ -115,7 +115,7 @@ struct InitRow {
 };
 template <typename T>
 struct TrialRow {  // own-contact data of a line-search trial row
-  T x[3], g[3], p[3], m[3];
+  T x[3], g[3], p[3], m[3], mp[3];
   double b[3], mu, s;
 };
 template <typename T>
@@ -140,7 +140,11 @@ __device__ __forceinline__ void start_point(const PGProblem<T>& pb, int i, T xc[
 // The solve of one problem by one team. tsum accumulates phase times
 // (s_memrealtime ticks): [0] operator-phase work, [1] operator-phase reduction
 // and wait, [2] update-phase work, [3] update-phase wait.
-template <bool MF, typename T, typename Team>
+// Fuse: the Nesterov point is formed speculatively inside every line-search trial
+// (no update phase; pays where a team barrier is expensive: the whole grid), or by
+// a separate element-wise phase after the accepted trial (cheaper per trial and in
+// registers: teams synchronised by __syncthreads or few blocks).
+template <bool MF, typename T, typename Team, bool Fuse>
 __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, uint64_t tsum[4]) {
   const DevProblem<T>& P = pb.P;
   const int nc = P.nc;
@@ -239,27 +243,58 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
       const int since = it - lastImprove;
       if ((since >= prm.stall_iters && minRes < 10.0 * prm.tol) || since >= 4 * prm.stall_iters) break;
     }
+    // slots: the candidate xs and the Nesterov point xn take iterate slots other than
+    // the current x and the previous iterate (which other members still read), and
+    // gradient slots other than the current one
     int rxs = 0;
     while (rxs == rx || rxs == rprev) ++rxs;
+    int rout = 0;
+    while (rout == rx || rout == rprev || rout == rxs) ++rout;
+    const int rgs = (rg + 1) % 3, rgn = (rg + 2) % 3;
     const T* Xx = pb.X[rx];
     const T* Gx = pb.G[rg];
+    // Nesterov step for this iteration (ProjectedGradient.impl.hpp:135-149): beta
+    // depends only on theta, so every trial also forms xn = xs + beta (xs - prev),
+    // M xn = M xs + beta (M xs - M prev), g(xn), its residual and energy; the
+    // iteration then adopts xn (decr2 < 0) or restarts at xs, without another pass.
+    double thetaN = theta;
+    const double beta = useNesterov ? nesterov(thetaN, 0.0) : 0.0;
+    // contact i's Nesterov point from its candidate (T arithmetic as the candidate)
+    auto nesterov_point = [&](int i, const T xst[3], const T v[3], const T pv[3], const T mp[3], double mu,
+                              double sc, const double b[3], double& rr, double& Jn) {
+      T xnt[3], Mnt[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        xnt[k] = (T)((double)xst[k] + beta * ((double)xst[k] - (double)pv[k]));
+        Mnt[k] = (T)((double)v[k] + beta * ((double)v[k] - (double)mp[k]));
+      }
+      tst3<Team, T>(pb.X[rout], i, xnt);
+      tst3<Team, T>(pb.MX[rout], i, Mnt);
+      double xn[3], Mn[3], g[3];
+      widen3<T>(xnt, xn);
+      widen3<T>(Mnt, Mn);
+      eval_contact(coulomb, mu, sc, b, xn, Mn, rr, g, Jn);
+      const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
+      tst3<Team, T>(pb.G[rgn], i, gt);
+    };
     if (!spg) {
       // ---- APGD / Descent: backtracking line search (ProjectedGradient.impl.hpp:117-133)
       alpha *= prm.ls_opt;
-      double Js = 0, decr = 0, nd = 0, decr2 = 0, resN = 0, JN = 0;
+      double Js = 0, decr = 0, nd = 0, decr2 = 0, resN = 0, JN = 0, resX = 0, JX = 0;
       auto pre = [&](int i) {
         TrialRow<T> r;
         tld3<Team, T>(Xx, i, r.x);
         tld3<Team, T>(Gx, i, r.g);
         tld3<Team, T>(pb.X[rprev], i, r.p);
         tld3<Team, T>(pb.MX[rx], i, r.m);
+        if (Fuse && useNesterov) tld3<Team, T>(pb.MX[rprev], i, r.mp);
         dld3<T>(P.b, i, r.b);
         r.mu = P.mu[i];
         r.s = rowScale(i);
         return r;
       };
       for (int ls = 0; ls < prm.ls_iters && ok; ++ls) {
-        double part[6] = {0, 0, 0, 0, 0, 0};
+        double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const T al = (T)alpha;
         auto ld = [&](int j, T xc[3]) {
           T xv[3], gv[3];
@@ -289,14 +324,20 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
           double rr, gn[3], Jn;
           eval_contact(coulomb, r.mu, r.s, r.b, xs, vd, rr, gn, Jn);
           const T gt[3] = {(T)gn[0], (T)gn[1], (T)gn[2]};
-          tst3<Team, T>(pb.G[1 - rg], i, gt);
+          tst3<Team, T>(pb.G[rgs], i, gt);
           part[4] = resMax ? fmax(part[4], rr) : part[4] + rr;
           part[5] += Jn;
+          if (Fuse && useNesterov) {
+            double rx2, Jx;
+            nesterov_point(i, xst, v, r.p, r.mp, r.mu, r.s, r.b, rx2, Jx);
+            part[6] = resMax ? fmax(part[6], rx2) : part[6] + rx2;
+            part[7] += Jx;
+          }
         };
         tp0 = realtime();
         ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
         tp1 = realtime();
-        ok = ok && tm.template reduce<6>(part, resMax << 4);
+        ok = ok && tm.template reduce<8>(part, (resMax << 4) | (resMax << 6));
         tsum[0] += tp1 - tp0;
         tsum[1] += realtime() - tp1;
         ++spmvs;
@@ -308,55 +349,48 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         decr2 = tm.red[3];
         resN = tm.red[4];
         JN = tm.red[5];
+        resX = tm.red[6];
+        JX = tm.red[7];
         if (Js < J + decr + 0.5 / alpha * nd) break;
         alpha *= prm.ls_pess;
       }
       if (!ok) break;
-      // ---- Nesterov extrapolation or restart (:135-149)
       if (useNesterov && decr2 < 0) {
-        const double beta = nesterov(theta, 0.0);
-        int rout = 0;
-        while (rout == rxs || rout == rprev) ++rout;
-        tp0 = realtime();
-        double part[2] = {0.0, 0.0};
-        for (int i = tm.tid; i < nc; i += tm.nthreads) {
-          T xst[3], pvt[3], Mst[3], Mpt[3];
-          tld3<Team, T>(pb.X[rxs], i, xst);
-          tld3<Team, T>(pb.X[rprev], i, pvt);
-          tld3<Team, T>(pb.MX[rxs], i, Mst);
-          tld3<Team, T>(pb.MX[rprev], i, Mpt);
-          double b[3], xn[3], Mn[3];
-          dld3<T>(P.b, i, b);
-          T xnt[3], Mnt[3];
-          for (int k = 0; k < 3; ++k) {
-            xnt[k] = (T)((double)xst[k] + beta * ((double)xst[k] - (double)pvt[k]));
-            Mnt[k] = (T)((double)Mst[k] + beta * ((double)Mst[k] - (double)Mpt[k]));
+        if (!Fuse) {
+          // element-wise Nesterov update of the accepted candidate
+          tp0 = realtime();
+          double p2[2] = {0.0, 0.0};
+          for (int i = tm.tid; i < nc; i += tm.nthreads) {
+            T xst[3], pvt[3], Mst[3], Mpt[3];
+            tld3<Team, T>(pb.X[rxs], i, xst);
+            tld3<Team, T>(pb.X[rprev], i, pvt);
+            tld3<Team, T>(pb.MX[rxs], i, Mst);
+            tld3<Team, T>(pb.MX[rprev], i, Mpt);
+            double b[3], rr, Jn;
+            dld3<T>(P.b, i, b);
+            nesterov_point(i, xst, Mst, pvt, Mpt, P.mu[i], rowScale(i), b, rr, Jn);
+            p2[0] = resMax ? fmax(p2[0], rr) : p2[0] + rr;
+            p2[1] += Jn;
           }
-          tst3<Team, T>(pb.X[rout], i, xnt);
-          tst3<Team, T>(pb.MX[rout], i, Mnt);
-          widen3<T>(xnt, xn);
-          widen3<T>(Mnt, Mn);
-          double rr, g[3], Ji;
-          eval_contact(coulomb, P.mu[i], rowScale(i), b, xn, Mn, rr, g, Ji);
-          const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
-          tst3<Team, T>(pb.G[rg], i, gt);
-          part[0] = resMax ? fmax(part[0], rr) : part[0] + rr;
-          part[1] += Ji;
+          tp1 = realtime();
+          ok = tm.template reduce<2>(p2, resMax);
+          tsum[2] += tp1 - tp0;
+          tsum[3] += realtime() - tp1;
+          if (!ok) break;
+          resX = tm.red[0];
+          JX = tm.red[1];
         }
-        tp1 = realtime();
-        ok = tm.template reduce<2>(part, resMax);
-        tsum[2] += tp1 - tp0;
-        tsum[3] += realtime() - tp1;
-        if (!ok) break;
-        res = tm.red[0];
-        J = tm.red[1];
+        res = resX;
+        J = JX;
         rx = rout;
         rprev = rxs;
+        rg = rgn;
+        theta = thetaN;
         ++nest;
       } else {
         rx = rxs;
         rprev = rxs;
-        rg = 1 - rg;
+        rg = rgs;
         res = resN;
         J = JN;
         theta = 1.0;
@@ -364,7 +398,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
       if (!prm.inf_norm) res /= (1.0 + nc);
     } else {
       // ---- SPG (ProjectedGradient.impl.hpp:316-346): one product per iteration.
-      double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       const T al = (T)alpha;
       auto pre = [&](int i) {
         SpgRow<T> r;
@@ -420,13 +454,18 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         double rr, gn[3], Jn;
         eval_contact(coulomb, r.mu, r.s, r.b, xs, vd, rr, gn, Jn);
         const T gt[3] = {(T)gn[0], (T)gn[1], (T)gn[2]};
-        tst3<Team, T>(pb.G[1 - rg], i, gt);
+        tst3<Team, T>(pb.G[rgs], i, gt);
         part[7] = resMax ? fmax(part[7], rr) : part[7] + rr;
+        if (Fuse) {
+          double rx2, Jx;
+          nesterov_point(i, xst, v, r.p, r.mp, r.mu, r.s, r.b, rx2, Jx);
+          part[8] = resMax ? fmax(part[8], rx2) : part[8] + rx2;
+        }
       };
       tp0 = realtime();
       ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
       tp1 = realtime();
-      ok = ok && tm.template reduce<8>(part, resMax << 7);
+      ok = ok && tm.template reduce<9>(part, (resMax << 7) | (resMax << 8));
       tsum[0] += tp1 - tp0;
       tsum[1] += realtime() - tp1;
       ++spmvs;
@@ -434,51 +473,41 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
       const double decr2 = tm.red[0];
       double wz, zz;
       if (decr2 < 0.0) {
-        const double beta = nesterov(theta, 0.0);
         wz = tm.red[1] + beta * tm.red[2] + beta * beta * tm.red[3];
         zz = tm.red[4] + 2.0 * beta * tm.red[5] + beta * beta * tm.red[6];
-        int rout = 0;
-        while (rout == rxs || rout == rprev) ++rout;
-        tp0 = realtime();
-        double p2[1] = {0.0};
-        for (int i = tm.tid; i < nc; i += tm.nthreads) {
-          T xst[3], pvt[3], Mst[3], Mpt[3];
-          tld3<Team, T>(pb.X[rxs], i, xst);
-          tld3<Team, T>(pb.X[rprev], i, pvt);
-          tld3<Team, T>(pb.MX[rxs], i, Mst);
-          tld3<Team, T>(pb.MX[rprev], i, Mpt);
-          double b[3], xn[3], Mn[3];
-          dld3<T>(P.b, i, b);
-          T xnt[3], Mnt[3];
-          for (int k = 0; k < 3; ++k) {
-            xnt[k] = (T)((double)xst[k] + beta * ((double)xst[k] - (double)pvt[k]));
-            Mnt[k] = (T)((double)Mst[k] + beta * ((double)Mst[k] - (double)Mpt[k]));
+        res = tm.red[8];
+        if (!Fuse) {
+          tp0 = realtime();
+          double p2[1] = {0.0};
+          for (int i = tm.tid; i < nc; i += tm.nthreads) {
+            T xst[3], pvt[3], Mst[3], Mpt[3];
+            tld3<Team, T>(pb.X[rxs], i, xst);
+            tld3<Team, T>(pb.X[rprev], i, pvt);
+            tld3<Team, T>(pb.MX[rxs], i, Mst);
+            tld3<Team, T>(pb.MX[rprev], i, Mpt);
+            double b[3], rr, Jn;
+            dld3<T>(P.b, i, b);
+            nesterov_point(i, xst, Mst, pvt, Mpt, P.mu[i], rowScale(i), b, rr, Jn);
+            p2[0] = resMax ? fmax(p2[0], rr) : p2[0] + rr;
           }
-          tst3<Team, T>(pb.X[rout], i, xnt);
-          tst3<Team, T>(pb.MX[rout], i, Mnt);
-          widen3<T>(xnt, xn);
-          widen3<T>(Mnt, Mn);
-          double rr, g[3], Ji;
-          eval_contact(coulomb, P.mu[i], rowScale(i), b, xn, Mn, rr, g, Ji);
-          const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
-          tst3<Team, T>(pb.G[rg], i, gt);
-          p2[0] = resMax ? fmax(p2[0], rr) : p2[0] + rr;
+          tp1 = realtime();
+          ok = tm.template reduce<1>(p2, resMax);
+          tsum[2] += tp1 - tp0;
+          tsum[3] += realtime() - tp1;
+          if (!ok) break;
+          res = tm.red[0];
         }
-        tp1 = realtime();
-        ok = tm.template reduce<1>(p2, resMax);
-        tsum[2] += tp1 - tp0;
-        tsum[3] += realtime() - tp1;
-        if (!ok) break;
-        res = tm.red[0];
         rx = rout;
         rprev = rxs;
+        rg = rgn;
+        theta = thetaN;
         ++nest;
       } else {
         wz = tm.red[1];
         zz = tm.red[4];
         rx = rxs;
         rprev = rxs;
-        rg = 1 - rg;
+        rg = rgs;
         res = tm.red[7];
         theta = 1.0;
       }
@@ -512,7 +541,7 @@ template <bool MF, typename T>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
   GridTeam tm(a.gs, blockIdx.x);
   uint64_t tsum[4] = {0, 0, 0, 0};
-  pg_body<MF, T, GridTeam>(a.prm, a.pb, tm, tsum);
+  pg_body<MF, T, GridTeam, true>(a.prm, a.pb, tm, tsum);
 }
 
 // Teams of several workgroups invalidate their caches after each barrier and read
@@ -530,6 +559,9 @@ using BatchTeamSel = GridTeamT<SB_BATCH_TEAM_ACQUIRE != 0>;
 // bounded and set the sticky error flag on timeout.
 template <bool MF, typename T, int NT>
 __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
+  // FP64: fused Nesterov step (its long problems run on multi-workgroup teams);
+  // FP32: separate update phase (fewer registers for its single-workgroup problems)
+  constexpr bool kFuse = sizeof(T) == sizeof(double);
   __shared__ int s_next;
   int c = 0;
   while (c + 1 < a.nclass && (int)blockIdx.x >= a.cls[c + 1].b0) ++c;
@@ -549,7 +581,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
       const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
       BlockTeam tm;
       uint64_t tsum[4] = {0, 0, 0, 0};
-      pg_body<MF, T, BlockTeam>(a.prm, probs[p], tm, tsum);
+      pg_body<MF, T, BlockTeam, kFuse>(a.prm, probs[p], tm, tsum);
       __syncthreads();
     }
     return;
@@ -576,7 +608,7 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
     if (k >= nq) break;
     const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
     uint64_t tsum[4] = {0, 0, 0, 0};
-    pg_body<MF, T, BatchTeamSel>(a.prm, probs[p], tm, tsum);
+    pg_body<MF, T, BatchTeamSel, kFuse>(a.prm, probs[p], tm, tsum);
     if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
   }
 }

@@ -46,9 +46,9 @@ struct PGParams {
 template <typename T>
 struct PGProblem {
   DevProblem<T> P;
-  T* X[3];
-  T* MX[3];
-  T* G[2];
+  T* X[4];   // iterate slots: current, previous, line-search candidate, Nesterov point
+  T* MX[4];  // their operator images
+  T* G[3];   // gradient slots: current, candidate, Nesterov point
   T* xbest;
   T* wbody;  // [6 nb]
   T* carry;
