@@ -129,11 +129,16 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
 #ifndef SB_TILE_PIPELINE
 #define SB_TILE_PIPELINE 1
 #endif
+// Block values of the solver's tiles read with non-temporal loads (streamed past
+// L2, which then keeps the gathered vectors).
+#ifndef SB_SOLVER_NT
+#define SB_SOLVER_NT 0
+#endif
 template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
 __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int stride, Loader& load, Pre& pre,
                                           Leader& lead, Acc* carry) {
 #if !SB_TILE_PIPELINE
-  for (int t = first; t < A.ntiles; t += stride) tile_gather<BR, BC, T, Acc>(A, t, tile_indices(A, t), load, pre, lead, carry);
+  for (int t = first; t < A.ntiles; t += stride) tile_gather<BR, BC, T, Acc, SB_SOLVER_NT>(A, t, tile_indices(A, t), load, pre, lead, carry);
   return;
 #endif
   if (first >= A.ntiles) return;
@@ -142,7 +147,7 @@ __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int st
     const int tn = t + stride;
     TileIdx nxt = cur;
     if (tn < A.ntiles) nxt = tile_indices(A, tn);
-    tile_gather<BR, BC, T, Acc>(A, t, cur, load, pre, lead, carry);
+    tile_gather<BR, BC, T, Acc, SB_SOLVER_NT>(A, t, cur, load, pre, lead, carry);
     cur = nxt;
   }
 }
