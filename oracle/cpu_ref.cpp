@@ -1501,4 +1501,173 @@ double gsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& col
   return errBest;
 }
 
+// ProductGaussSeidel::solve with a block-diagonal D = Minv and the precomputed
+// D H^T (src/Core/BlockSolvers/ProductGaussSeidel.impl.hpp:225-304, innerLoop
+// :128-218, DMtStorage<.., true> :64-86; reached through
+// PrimalFrictionProblem::solveWith(ProductGaussSeidelType&), PrimalFrictionProblem.cpp:40-59).
+// The solver keeps Mx = D H^T x (one 6-vector per body): a contact's local
+// right-hand side is lb = w_i - A_i x_i + H_i Mx, and after its update
+// Mx_k += (D_k H_ik^T) dx_i for each of its bodies. Mx is recomputed from x at
+// every evaluation. W never appears.
+double pgsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& colors, double* xio, SolveStats* st) {
+  const int n = P.nc, nb = P.nb;
+  const BSR& H = P.H;
+  const bool coulomb = o.coulomb != 0;
+  const Strategy strat = (Strategy)o.strategy;
+  const double localTol = std::pow(kEps, .75);  // SOCLaw::m_localTol (SOCLaw.impl.hpp:35)
+  // updateLocalMatrices (:147-165): A_i = sum_k H_ik D_k H_ik^T, then
+  // processLocalMatrices (GaussSeidelBase.impl.hpp:20-40)
+  std::vector<double> local(P.diag), reg(n, 0.0), scaling(n, 1.0);
+  for (int i = 0; i < n; ++i) {
+    double* d = &local[(size_t)i * 9];
+    scaling[i] = std::max(1.0, (d[0] + d[4] + d[8]) / 3);
+    if (o.autoRegularization > 0.) {
+      reg[i] = std::max(0., o.autoRegularization - minEigSym3(d));
+      d[0] += reg[i];
+      d[4] += reg[i];
+      d[8] += reg[i];
+    }
+  }
+  // DMtStorage<.., true>::compute: T_p = D_k H_p^T (6x3, row-major) for block p = (i, k)
+  const int nnz = H.nnzb();
+  std::vector<double> T((size_t)nnz * 18);
+  for (int i = 0; i < n; ++i)
+    for (int p = H.rowptr[i]; p < H.rowptr[i + 1]; ++p) {
+      const double* h = H.block(p);
+      const double* m = &P.minv[(size_t)H.colidx[p] * 36];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double acc = 0;
+          for (int l = 0; l < 6; ++l) acc += m[r * 6 + l] * h[c * 6 + l];
+          T[(size_t)p * 18 + r * 3 + c] = acc;
+        }
+    }
+  // D H^T as a body-major (row-major) matrix: the contacts of body k in ascending order
+  std::vector<int> tptr, trow, tpos;
+  buildTransposeIndex(H, tptr, trow, tpos, false);
+  Vec Mx((size_t)nb * 6, 0.0);
+  // DMtStorage::multiply (:75-80): Mx = (D H^T) x, then y += H Mx
+  auto dmtMultiply = [&](const Vec& x) {
+    for (int k = 0; k < nb; ++k) {
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int q = tptr[k]; q < tptr[k + 1]; ++q) {
+        const int i = trow[q];
+        const double* t = &T[(size_t)tpos[q] * 18];
+        for (int r = 0; r < 6; ++r) acc[r] += t[r * 3] * x[3 * i] + t[r * 3 + 1] * x[3 * i + 1] + t[r * 3 + 2] * x[3 * i + 2];
+      }
+      for (int r = 0; r < 6; ++r) Mx[(size_t)k * 6 + r] = acc[r];
+    }
+  };
+  // rowMultiply (:83-86): res += H_i Mx
+  auto rowMultiply = [&](int i, double res[3]) {
+    for (int p = H.rowptr[i]; p < H.rowptr[i + 1]; ++p) {
+      const double* h = H.block(p);
+      const double* m = &Mx[(size_t)H.colidx[p] * 6];
+      for (int r = 0; r < 3; ++r) {
+        double acc = 0;
+        for (int c = 0; c < 6; ++c) acc += h[r * 6 + c] * m[c];
+        res[r] += acc;
+      }
+    }
+  };
+  const Norm norm = o.infNorm ? Infinity : MeanSquare;
+  Vec x(xio, xio + 3 * (size_t)n), y(3 * (size_t)n), xBest;
+  const Vec& w = P.b;  // solve(): the linear-constraint operator is Zero, so w = b
+  double errBest = std::numeric_limits<double>::max();
+  unsigned spmvs = 0;
+  auto evalAndKeepBest = [&](const Vec& xx) {
+    dmtMultiply(xx);
+    ++spmvs;
+    for (int i = 0; i < n; ++i) {
+      double yi[3] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+      rowMultiply(i, yi);
+      for (int r = 0; r < 3; ++r) y[3 * i + r] = yi[r];
+    }
+    const double err = evalResidual(n, coulomb, P.mu.data(), scaling.data(), y.data(), xx.data(), norm);
+    if (err < errBest) {
+      xBest = xx;
+      errBest = err;
+    }
+    return err;
+  };
+  evalAndKeepBest(x);
+  if (o.tryZero) {  // GaussSeidelBase::tryZero (GaussSeidelBase.impl.hpp:61-77); true: Mx = 0
+    Vec zero(x.size(), 0.0);
+    const double errZero = evalResidual(n, coulomb, P.mu.data(), scaling.data(), P.b.data(), zero.data(), norm);
+    if (errZero < errBest) {
+      errBest = errZero;
+      xBest = zero;
+      x = zero;
+      std::fill(Mx.begin(), Mx.end(), 0.0);
+    } else {
+      x = xBest;
+    }
+  }
+  if (st) st->history.push_back(errBest);
+  std::vector<unsigned char> skip(n, 0);
+  double ndxRef = 0;
+  const double absSkipTol = std::min(o.skipTol, o.tol);
+  const double absSkipIters = std::min(o.skipIters, (unsigned)std::sqrt((double)n));
+  unsigned it;
+  for (it = 1; it <= o.maxIters; ++it) {
+    for (size_t c = 0; c + 1 < colors.size(); ++c) {
+      const double ndxRefColor = ndxRef;
+      double ndxRefNew = ndxRef;
+      for (int i = colors[c]; i < colors[c + 1]; ++i) {
+        if (skip[i]) {
+          --skip[i];
+          continue;
+        }
+        double lx[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+        const double* A = &local[(size_t)i * 9];
+        double lb[3];
+        for (int r = 0; r < 3; ++r) lb[r] = w[3 * i + r] - (A[r * 3] * lx[0] + A[r * 3 + 1] * lx[1] + A[r * 3 + 2] * lx[2]);
+        rowMultiply(i, lb);
+        double ldx[3] = {-lx[0], -lx[1], -lx[2]};
+        const double(*A3)[3] = reinterpret_cast<const double(*)[3]>(A);
+        const double lres = localSolve(coulomb, strat, A3, lb, lx, P.mu[i], localTol, scaling[i]);
+        const bool ok = localTol > lres;
+        for (int k = 0; k < 3; ++k) ldx[k] += lx[k];
+        if (!ok)
+          for (int k = 0; k < 3; ++k) ldx[k] *= .5;
+        for (int k = 0; k < 3; ++k) x[3 * i + k] += ldx[k];
+        // colMultiply (:79-81): Mx += (D H^T)_{:, i} dx
+        for (int p = H.rowptr[i]; p < H.rowptr[i + 1]; ++p) {
+          const double* t = &T[(size_t)p * 18];
+          double* m = &Mx[(size_t)H.colidx[p] * 6];
+          for (int r = 0; r < 6; ++r) m[r] += t[r * 3] * ldx[0] + t[r * 3 + 1] * ldx[1] + t[r * 3 + 2] * ldx[2];
+        }
+        const double s2 = scaling[i] * scaling[i];
+        const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
+        const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
+        double ref;
+        if (o.ndxRefPerColor) {
+          if (ndx2 > ndxRefNew) ndxRefNew = ndx2;
+          ref = ndxRefColor;
+        } else {
+          if (ndx2 > ndxRef) ndxRef = ndx2;
+          ref = ndxRef;
+        }
+        if (std::min(nx2, ndx2) < absSkipTol || ndx2 < o.skipTol * std::min(nx2, ref))
+          skip[i] = (unsigned char)absSkipIters;
+      }
+      if (o.ndxRefPerColor) ndxRef = ndxRefNew;
+    }
+    if (0 == (it % o.evalEvery)) {
+      const double err = evalAndKeepBest(x);
+      if (st) st->history.push_back(err);
+      if (err < o.tol) break;
+      ndxRef /= o.evalEvery;
+    }
+  }
+  if (it > o.maxIters) x = xBest;
+  std::copy(x.begin(), x.end(), xio);
+  if (st) {
+    st->iterations = std::min(it, o.maxIters);
+    st->spmvs = spmvs;
+    st->residual = errBest;
+  }
+  return errBest;
+}
+
 }  // namespace cpuref

@@ -177,6 +177,11 @@ void computeColoring(const BSR& Wlower, std::vector<int>& perm, std::vector<int>
 // pass {0, nc} for a single color.
 double gsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& colors, double* x, SolveStats* st);
 
+// ProductGaussSeidel::solve with D = Minv, D H^T precomputed
+// (src/Core/BlockSolvers/ProductGaussSeidel.impl.hpp:64-86, 128-304): the matrix-free
+// Gauss-Seidel; W is never used. Same options and color handling as gsSolve.
+double pgsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& colors, double* x, SolveStats* st);
+
 // bogus::applyPermutation (src/Core/Block/SparseBlockMatrixBase.impl.hpp:309-326): new[i] = old[perm[i]]
 // and DualFrictionProblem::applyPermutation (src/Interfaces/FrictionProblem.cpp:102-114).
 void permuteProblem(Problem& P, const std::vector<int>& perm);

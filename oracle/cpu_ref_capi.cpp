@@ -114,7 +114,7 @@ double orc_pg_solve(void* p, int variant, double tol, int maxIters, int coulomb,
   return res;
 }
 
-double orc_gs_solve(void* p, double tol, int maxIters, int evalEvery, double skipTol, int skipIters, double autoReg,
+static double gsCommon(bool product, void* p, double tol, int maxIters, int evalEvery, double skipTol, int skipIters, double autoReg,
                     int tryZero, int coulomb, int strategy, int infNorm, int ndxRefPerColor, const int* colors,
                     int ncolors, double* r, int* iters, double* hist, int histCap, int* histLen) {
   const Problem* P = static_cast<Problem*>(p);
@@ -136,10 +136,25 @@ double orc_gs_solve(void* p, double tol, int maxIters, int evalEvery, double ski
   else
     c = {0, P->nc};
   SolveStats st;
-  const double res = gsSolve(*P, o, c, r, &st);
+  const double res = product ? pgsSolve(*P, o, c, r, &st) : gsSolve(*P, o, c, r, &st);
   if (iters) *iters = (int)st.iterations;
   copyHistory(st, hist, histCap, histLen);
   return res;
+}
+
+double orc_gs_solve(void* p, double tol, int maxIters, int evalEvery, double skipTol, int skipIters, double autoReg,
+                    int tryZero, int coulomb, int strategy, int infNorm, int ndxRefPerColor, const int* colors,
+                    int ncolors, double* r, int* iters, double* hist, int histCap, int* histLen) {
+  return gsCommon(false, p, tol, maxIters, evalEvery, skipTol, skipIters, autoReg, tryZero, coulomb, strategy, infNorm,
+                  ndxRefPerColor, colors, ncolors, r, iters, hist, histCap, histLen);
+}
+
+// ProductGaussSeidel (matrix-free GS, no W)
+double orc_pgs_solve(void* p, double tol, int maxIters, int evalEvery, double skipTol, int skipIters, double autoReg,
+                     int tryZero, int coulomb, int strategy, int infNorm, int ndxRefPerColor, const int* colors,
+                     int ncolors, double* r, int* iters, double* hist, int histCap, int* histLen) {
+  return gsCommon(true, p, tol, maxIters, evalEvery, skipTol, skipIters, autoReg, tryZero, coulomb, strategy, infNorm,
+                  ndxRefPerColor, colors, ncolors, r, iters, hist, histCap, histLen);
 }
 
 int orc_coloring(void* p, int* perm, int* colors) {

@@ -57,6 +57,9 @@ def lib():
             "orc_gs_solve": (C.c_double, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                           C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
                                           _dp, _ip, _dp, C.c_int, _ip]),
+            "orc_pgs_solve": (C.c_double, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                           C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
+                                           _dp, _ip, _dp, C.c_int, _ip]),
             "orc_coloring": (C.c_int, [C.c_void_p, _ip, _ip]),
             "orc_problem_permute": (None, [C.c_void_p, _ip]),
             "orc_fb": (None, [C.c_int, C.c_double, _dp, _dp, _dp]),
@@ -175,7 +178,7 @@ class OracleProblem:
                                  int(inf_norm), _d(r), C.byref(it), C.byref(sp), _d(hist), hist_cap, C.byref(hl))
         return res, r, dict(iterations=it.value, spmvs=sp.value, history=hist[:min(hl.value, hist_cap)].copy())
 
-    def gs_solve(self, r0, tol=1e-6, max_iters=250, eval_every=25, skip_tol=1e-6, skip_iters=10, auto_reg=0.0,
+    def _gs(self, fn, r0, tol=1e-6, max_iters=250, eval_every=25, skip_tol=1e-6, skip_iters=10, auto_reg=0.0,
                  try_zero=True, coulomb=True, strategy=REV_HYBRID, inf_norm=False, ndx_ref_per_color=False,
                  colors=None, hist_cap=10000):
         r = f64(r0).copy()
@@ -187,10 +190,18 @@ class OracleProblem:
         else:
             colors = i32(colors)
             cp, nco = _i(colors), len(colors) - 1
-        res = lib().orc_gs_solve(self.h, tol, max_iters, eval_every, skip_tol, skip_iters, auto_reg, int(try_zero),
+        res = fn(self.h, tol, max_iters, eval_every, skip_tol, skip_iters, auto_reg, int(try_zero),
                                  int(coulomb), strategy, int(inf_norm), int(ndx_ref_per_color), cp, nco, _d(r),
                                  C.byref(it), _d(hist), hist_cap, C.byref(hl))
         return res, r, dict(iterations=it.value, history=hist[:min(hl.value, hist_cap)].copy())
+
+    def gs_solve(self, r0, **kw):
+        """GaussSeidel::solve on the assembled W (GaussSeidel.impl.hpp:24-205)."""
+        return self._gs(lib().orc_gs_solve, r0, **kw)
+
+    def pgs_solve(self, r0, **kw):
+        """ProductGaussSeidel::solve with D = Minv (ProductGaussSeidel.impl.hpp:64-304): W is not used."""
+        return self._gs(lib().orc_pgs_solve, r0, **kw)
 
     def coloring(self):
         perm = np.zeros(self.nc, np.int32)

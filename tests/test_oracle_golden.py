@@ -74,6 +74,46 @@ def test_small_gs_soc(O, small):
     assert res < 1e-8
 
 
+# ---------------------------------------------------------------- tests/ProductGaussSeidel.cpp:16-62
+def test_small_pgs_known_solution(O, small):
+    """Product_GS_D / Product_GS_D_precomp: D = the inverse mass, x0 = ones."""
+    prob, sol, op = small
+    res, x, _ = op.pgs_solve(np.ones(6), coulomb=True, tol=1e-8)
+    assert res < 1e-8
+    assert _is_approx(sol, x, 1e-4)
+
+
+def test_small_pgs_trivial_diagonal(G, O, small):
+    """Product_GS: D = identity, so the operator is H H^T; the residual of the
+    result, re-evaluated with y = H H^T x + b (the oracle's W of this problem), is
+    below 1e-8."""
+    import dataclasses
+    prob, sol, _ = small
+    eye = dataclasses.replace(prob, minv=np.tile(np.eye(6).reshape(36), (prob.nb, 1)))
+    op = O.OracleProblem(eye)
+    res, x, _ = op.pgs_solve(np.ones(6), coulomb=True, tol=1e-8)
+    assert res < 1e-8
+    H = eye.dense_h()
+    assert np.allclose(eye.dense_w(), H @ H.T)
+    assert op.eval(x, coulomb=True) < 1e-8
+
+
+@pytest.mark.parametrize("coulomb", [True, False])
+def test_pgs_equals_gs_on_w(G, O, coulomb):
+    """The product form (lb from H (D H^T x), no W) and the assembled-W Gauss-Seidel
+    are the same iteration: with the same coloring they take the same number of
+    sweeps and their residual histories agree to rounding."""
+    prob = G.workload("small-1k")
+    op = O.OracleProblem(prob)
+    perm, colors = op.coloring()
+    op.permute(perm)
+    r1, x1, d1 = op.gs_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=1000, colors=colors, coulomb=coulomb)
+    r2, x2, d2 = op.pgs_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=1000, colors=colors, coulomb=coulomb)
+    assert d1["iterations"] == d2["iterations"]
+    assert np.allclose(d1["history"], d2["history"], rtol=1e-9, atol=0)
+    assert np.linalg.norm(x1 - x2) <= 1e-12 * np.linalg.norm(x1)
+
+
 # ---------------------------------------------------------------- tests/ProjectedGradient.cpp:15-83
 @pytest.mark.parametrize("variant", ["PG_APGD", "PG_DESCENT", "PG_STANDARD", "PG_CONJUGATED", "PG_SPG"])
 def test_small_pg_soc(O, small, variant):
