@@ -181,8 +181,10 @@ int sb_problem_sizes(const sb_problem* P, int* n_contacts, int* n_bodies, int64_
 int sb_problem_get_w(sb_ctx* ctx, const sb_problem* P, int32_t* w_rowptr, int32_t* w_colidx, double* w_vals);
 /* Contact permutation, new[i] = old[perm[i]] (DualFrictionProblem::applyPermutation). */
 int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm);
-/* Coloring of the contacts (Coloring::update(true, W)); perm has n_contacts entries and colors
-   n_colors + 1 offsets (allocate n_contacts + 1). Requires an assembled W. */
+/* Coloring of the contacts (Coloring::update(true, W), Coloring.impl.hpp:20-62); perm has
+   n_contacts entries and colors n_colors + 1 offsets (allocate n_contacts + 1). Two contacts
+   are coupled when they share a body (the block pattern of W), so a problem without an
+   assembled W is colored from H. */
 int sb_problem_coloring(sb_ctx* ctx, const sb_problem* P, int mode, int64_t* perm, int64_t* colors, int* n_colors);
 
 /* y = W x (op = SB_OP_ASSEMBLED) or H (Minv (H^T x)) (SB_OP_MATRIX_FREE). Host or device arrays. */
@@ -199,6 +201,15 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, dou
    colors holds n_colors + 1 offsets (pass NULL / 0 for a single color). */
 int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
                 double* r, int on_device, double* residual, sb_stats* stats);
+/* Matrix-free Gauss-Seidel: ProductGaussSeidel<H, Minv>::solve (ProductGaussSeidel.impl.hpp:
+   128-304) as reached through PrimalFrictionProblem::solveWith(ProductGaussSeidelType&, r,
+   staticProblem) (PrimalFrictionProblem.cpp:40-59). Needs no assembled W; the body vector
+   Minv H^T r is kept current through the sweeps. Same options and results as sb_gs_solve.
+   The contacts of one color must share no body (sb_problem_coloring + permutation), else
+   SB_ERR_INVALID; with colors NULL the contacts are swept in their order (the reference's
+   sequential sweep), in runs of consecutive contacts that share no body. */
+int sb_pgs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
+                 double* r, int on_device, double* residual, sb_stats* stats);
 
 /* ---------------------------------------------------------------- derivatives
    Reverse-mode derivatives of L = g . r for the projected fixed-point map of the

@@ -168,6 +168,11 @@ struct sb_problem {
   DBuf<float> X32[4], MX32[4], G32[3], xbest32, wbody32, carry32;
   DBuf<unsigned char> skip;
   DBuf<int> colors;
+  // matrix-free Gauss-Seidel: Minv_k H_ik^T per H block and per-contact block ranges,
+  // rebuilt when the uploaded data changes (version)
+  DBuf<double> dmt;
+  DBuf<int> hrow;
+  int version = 0, pgsVersion = -1;
   DBuf<double> out;  // [2][16] solver results (FP32 stage, FP64 stage)
   DevProblem<double> view(bool withScaling) const {
     DevProblem<double> v;
@@ -257,6 +262,7 @@ std::vector<double> minv_element_major(int nb, const std::vector<double>& minv) 
 int upload_values(sb_problem* P) {
   hipStream_t s = P->ctx->stream;
   const int nc = P->nc, nb = P->nb;
+  ++P->version;
   if (P->hasW) {
     std::vector<int> frp, fci;
     std::vector<double> fv;
@@ -296,6 +302,7 @@ int upload_values(sb_problem* P) {
 int upload_problem(sb_problem* P) {
   hipStream_t s = P->ctx->stream;
   const int nc = P->nc, nb = P->nb;
+  ++P->version;
   // H contact-major, H^T body-major
   SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
   {
@@ -866,9 +873,13 @@ int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm
 
 int sb_problem_coloring(sb_ctx* ctx, const sb_problem* P, int mode, int64_t* perm, int64_t* colors, int* ncolors) {
   (void)ctx;
-  if (!P || !P->hasW || !perm || !colors || !ncolors || (mode != 0 && mode != 1)) return SB_ERR_INVALID;
+  if (!P || !perm || !colors || !ncolors || (mode != 0 && mode != 1)) return SB_ERR_INVALID;
   std::vector<int64_t> pm, cl;
-  sbhost::coloring(P->nc, P->wrp.data(), P->wci.data(), mode, pm, cl);
+  // the pattern of W is the contact-body incidence of H: the coloring does not need W
+  if (P->hasW)
+    sbhost::coloring(P->nc, P->wrp.data(), P->wci.data(), mode, pm, cl);
+  else
+    sbhost::coloringFromH(P->nc, P->nb, P->hrp.data(), P->hci.data(), mode, pm, cl);
   std::memcpy(perm, pm.data(), sizeof(int64_t) * pm.size());
   std::memcpy(colors, cl.data(), sizeof(int64_t) * cl.size());
   *ncolors = (int)cl.size() - 1;
@@ -1014,13 +1025,11 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   return o64[5] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
 }
 
-int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const int64_t* colors, int ncolors,
-                double* r, int on_device, double* residual, sb_stats* st) {
-  int rc = check_ctx(ctx);
-  if (rc) return rc;
-  if (!Pc || !o || !r) return SB_ERR_INVALID;
-  sb_problem* P = const_cast<sb_problem*>(Pc);
-  if (!P->hasW) return SB_ERR_INVALID;
+// Set-up shared by the two Gauss-Seidel solvers: options, color offsets, local
+// matrices with auto-regularization (GaussSeidelBase::processLocalMatrices,
+// GaussSeidelBase.impl.hpp:20-40), skip counters, r, history, kernel arguments.
+static int gs_prepare(sb_ctx* ctx, sb_problem* P, const sb_gs_options* o, const int64_t* colors, int ncolors,
+                      double* r, int on_device, sb_stats* st, sb::GSArgs& a, int& histCap, int& maxColor) {
   if (o->max_iters < 0 || o->eval_every < 1 || o->strategy < 0 || o->strategy > 3) return SB_ERR_INVALID;
   if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   const int n = P->nc;
@@ -1036,7 +1045,6 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
     col = {0, n};
   }
   hipStream_t s = ctx->stream;
-  // local matrices with optional auto-regularization (GaussSeidelBase::processLocalMatrices)
   std::vector<double> local(P->diag), reg(n, 0.0);
   if (o->auto_regularization > 0.0)
     for (int i = 0; i < n; ++i) {
@@ -1050,12 +1058,10 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   SB_HIP(P->colors.upload(col, s));
   SB_HIP(P->skip.alloc(n));
   SB_HIP(hipMemsetAsync(P->skip.p, 0, n, s));
-  const size_t n3 = 3 * (size_t)n;
-  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
-  const int histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
+  int rc;
+  if ((rc = copy_in(ctx, P->rio.p, r, 3 * (size_t)n, on_device))) return rc;
+  histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
   if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
-
-  sb::GSArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nc = n;
   a.coulomb = o->law == SB_LAW_COULOMB;
@@ -1081,23 +1087,23 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   a.hist = P->hist.p;
   a.hist_cap = histCap;
   a.out = ctx->out.p;
-  const void* fn = (const void*)&sb::gs_kernel;
-  int maxColor = 0;
+  maxColor = 0;
   for (size_t c = 0; c + 1 < col.size(); ++c) maxColor = std::max(maxColor, col[c + 1] - col[c]);
-  const int threads = block_threads_for(o->block_threads, kThreads);
-  if (threads != 256) return threads < 0 ? SB_ERR_INVALID : SB_ERR_UNSUPPORTED;  // GS kernel: 256 threads
-  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, std::max(P->W.host.ntiles, (maxColor + 63) / 64));
-  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
-  a.gs = make_sync(ctx, grid);
+  return SB_OK;
+}
+
+// Launch of a prepared Gauss-Seidel kernel, result and statistics.
+static int gs_run(sb_ctx* ctx, sb_problem* P, const void* fn, int grid, int threads, void** args, double* r,
+                  int on_device, double* residual, sb_stats* st, int histCap) {
+  hipStream_t s = ctx->stream;
   SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
-  const int* wrowptr = P->W.rowptr.p;
-  void* args[] = {&a, &wrowptr};
   SB_HIP(hipEventRecord(ctx->ev0, s));
   SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
   double out[8];
+  int rc;
   if ((rc = read_status(ctx, out, 8))) return rc;
-  if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
+  if ((rc = copy_out(ctx, r, P->rio.p, 3 * (size_t)P->nc, on_device))) return rc;
   if (st) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
@@ -1120,6 +1126,119 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   if (out[5] != 0.0) return SB_ERR_BARRIER_TIMEOUT;
   if (!std::isfinite(out[0])) return SB_ERR_NAN;
   return out[2] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
+}
+
+int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const int64_t* colors, int ncolors,
+                double* r, int on_device, double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (!P->hasW) return SB_ERR_INVALID;
+  sb::GSArgs a;
+  int histCap = 0, maxColor = 0;
+  if ((rc = gs_prepare(ctx, P, o, colors, ncolors, r, on_device, st, a, histCap, maxColor))) return rc;
+  const void* fn = (const void*)&sb::gs_kernel;
+  const int threads = block_threads_for(o->block_threads, kThreads);
+  if (threads != 256) return threads < 0 ? SB_ERR_INVALID : SB_ERR_UNSUPPORTED;  // GS kernel: 256 threads
+  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, std::max(P->W.host.ntiles, (maxColor + 63) / 64));
+  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
+  a.gs = make_sync(ctx, grid);
+  const int* wrowptr = P->W.rowptr.p;
+  void* args[] = {&a, &wrowptr};
+  return gs_run(ctx, P, fn, grid, threads, args, r, on_device, residual, st, histCap);
+}
+
+// Per-H-block Minv_k H_ik^T (DMtStorage<.., true>::compute, ProductGaussSeidel.impl.hpp:64-68)
+// in the H tiles' block order, and each contact's first block and block count.
+static int pgs_upload(sb_problem* P) {
+  if (P->pgsVersion == P->version) return SB_OK;
+  const sbhost::Tiles& T = P->H.host;
+  const int n = P->nc;
+  const size_t nq = (size_t)T.nblocks;
+  std::vector<double> dmt(18 * nq, 0.0);
+  std::vector<int> hrow(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const int q0 = T.rowptr[i], len = P->hrp[i + 1] - P->hrp[i];
+    hrow[2 * i] = q0;
+    hrow[2 * i + 1] = len;
+    for (int t = 0; t < len; ++t) {
+      const int p = P->hrp[i] + t;
+      const double* h = &P->hv[(size_t)p * 18];
+      const double* m = &P->minv[(size_t)P->hci[p] * 36];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double acc = 0.0;
+          for (int l = 0; l < 6; ++l) acc += m[r * 6 + l] * h[c * 6 + l];
+          dmt[(size_t)(r * 3 + c) * nq + q0 + t] = acc;
+        }
+    }
+  }
+  hipStream_t s = P->ctx->stream;
+  SB_HIP(P->dmt.upload(dmt, s));
+  SB_HIP(P->hrow.upload(hrow, s));
+  SB_HIP(hipStreamSynchronize(s));
+  P->pgsVersion = P->version;
+  return SB_OK;
+}
+
+int sb_pgs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const int64_t* colors, int ncolors,
+                 double* r, int on_device, double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  // Contacts of one color update the body vector in parallel, so they must share no
+  // body. Given colors are checked; without colors the natural order is cut into
+  // maximal runs of contacts that share no body, which sweeps the contacts exactly
+  // in the reference's sequential order.
+  std::vector<int64_t> runs;
+  {
+    std::vector<int> stamp(P->nb, -1);
+    if (colors && ncolors > 0) {
+      if (colors[0] != 0 || colors[ncolors] != P->nc) return SB_ERR_INVALID;
+      for (int c = 0; c < ncolors; ++c) {
+        if (colors[c + 1] < colors[c]) return SB_ERR_INVALID;
+        for (int64_t i = colors[c]; i < colors[c + 1]; ++i)
+          for (int p = P->hrp[i]; p < P->hrp[i + 1]; ++p) {
+            if (stamp[P->hci[p]] == c) return SB_ERR_INVALID;
+            stamp[P->hci[p]] = c;
+          }
+      }
+    } else {
+      runs.push_back(0);
+      int c = 0;
+      for (int i = 0; i < P->nc; ++i) {
+        bool clash = false;
+        for (int p = P->hrp[i]; p < P->hrp[i + 1]; ++p) clash |= stamp[P->hci[p]] == c;
+        if (clash) {
+          runs.push_back(i);
+          ++c;
+        }
+        for (int p = P->hrp[i]; p < P->hrp[i + 1]; ++p) stamp[P->hci[p]] = c;
+      }
+      runs.push_back(P->nc);
+      colors = runs.data();
+      ncolors = (int)runs.size() - 1;
+    }
+  }
+  sb::GSArgs a;
+  int histCap = 0, maxColor = 0;
+  if ((rc = gs_prepare(ctx, P, o, colors, ncolors, r, on_device, st, a, histCap, maxColor))) return rc;
+  if ((rc = pgs_upload(P))) return rc;
+  sb::PGSArgs e;
+  e.hrow = P->hrow.p;
+  e.dmt = P->dmt.p;
+  e.ybody = P->wbody.p;
+  const void* fn = (const void*)&sb::pgs_kernel;
+  const int threads = block_threads_for(o->block_threads, kThreads);
+  if (threads != 256) return threads < 0 ? SB_ERR_INVALID : SB_ERR_UNSUPPORTED;  // GS kernels: 256 threads
+  const int work = std::max({P->H.host.ntiles, P->HT.host.ntiles, (maxColor + 63) / 64});
+  const int grid = grid_for(ctx, fn, threads, o->grid_blocks, work);
+  if ((rc = ensure_grid_scratch(ctx, grid))) return rc;
+  a.gs = make_sync(ctx, grid);
+  void* args[] = {&a, &e};
+  return gs_run(ctx, P, fn, grid, threads, args, r, on_device, residual, st, histCap);
 }
 
 // ------------------------------------------------------------------ memory helpers

@@ -14,7 +14,9 @@
 
 #include "sb_law.hpp"
 #include "sb_local.hpp"
+#include "sb_op.hpp"
 #include "sb_problem.hpp"
+#include "sb_team.hpp"
 
 namespace sb {
 
@@ -202,6 +204,219 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
       a.out[1] = (double)(it > a.max_iters ? a.max_iters : it);
       a.out[2] = converged ? 1.0 : 0.0;
       a.out[3] = ok2 ? bar.red[0] : -1.0;
+      a.out[4] = (double)nEval;
+      a.out[5] = (ok && ok2) ? 0.0 : 1.0;
+    }
+  }
+}
+
+// Matrix-free (product) Gauss-Seidel: ProductGaussSeidel::solve with D = Minv
+// and D H^T precomputed (src/Core/BlockSolvers/ProductGaussSeidel.impl.hpp:128-304,
+// reached through PrimalFrictionProblem::solveWith(ProductGaussSeidelType&),
+// PrimalFrictionProblem.cpp:40-59). W is never formed: the solver keeps the body
+// vector y = Minv H^T x; contact i's local right-hand side is
+// lb = b_i - A_i x_i + H_i y (A_i = W_ii + regularization), and after its update
+// y_k += (Minv_k H_ik^T) dx_i for each of its bodies. Contacts of one color share
+// no body, so their y updates do not overlap and the colors run as in gs_kernel
+// (one lane per contact, a grid barrier per color, ndxRef merged per color). The
+// evaluation every eval_every sweeps recomputes y = Minv (H^T x) from x (as the
+// reference recomputes D H^T x there) with the matrix-free operator phases and
+// evaluates u = H y + b.
+__global__ void __launch_bounds__(256) pgs_kernel(GSArgs a, PGSArgs e) {
+  GridTeam tm(a.gs, blockIdx.x);
+  const int tid = tm.tid, nthreads = tm.nthreads;
+  const int n = a.nc, n3 = 3 * n, nb = a.P.nb;
+  const bool coulomb = a.coulomb != 0;
+  const unsigned resMax = a.inf_norm ? 1u : 0u;
+  const TileMat<double>& H = a.P.H;
+  const size_t nnz = (size_t)H.nblocks;
+
+  // y = Minv H^T x (kept in e.ybody); u = H y + b; err = eval(u, x) [err0 = eval(b, 0)]
+  auto eval_phase = [&](bool withZero, double& err, double& err0) -> bool {
+    double part[2] = {0.0, 0.0};
+    auto ld = [&](int j, double xc[3]) { cld3(a.x, j, xc); };
+    NoPre nopre;
+    auto body = [&](int k, double v[6]) {
+      double w[6];
+      minv_apply<double>(a.P.minv, nb, k, v, w);
+#pragma unroll
+      for (int r = 0; r < 6; ++r) cst(&e.ybody[6 * k + r], w[r]);
+    };
+    auto lead = [&](int i, double v[3], NoPre::type) {
+      double b[3], xi[3];
+      ld3(a.P.b, i, b);
+      cld3(a.x, i, xi);
+      const double s = a.P.scal[i], mu = a.P.mu[i];
+      const double y[3] = {v[0] + b[0], v[1] + b[1], v[2] + b[2]};
+      const double xs[3] = {s * xi[0], s * xi[1], s * xi[2]};
+      const double r = law_eval<double>(coulomb, mu, xs, y);
+      part[0] = resMax ? fmax(part[0], r) : part[0] + r;
+      if (withZero) {
+        const double z[3] = {0.0, 0.0, 0.0};
+        const double r0 = law_eval<double>(coulomb, mu, z, b);
+        part[1] = resMax ? fmax(part[1], r0) : part[1] + r0;
+      }
+    };
+    if (!apply_operator<true, double>(a.P, e.ybody, a.carry, tm, ld, nopre, lead, body)) return false;
+    if (!tm.reduce<2>(part, resMax | (resMax << 1))) return false;
+    err = tm.red[0];
+    err0 = tm.red[1];
+    if (!a.inf_norm) {
+      err /= (1.0 + n);
+      err0 /= (1.0 + n);
+    }
+    return true;
+  };
+
+  double errBest = 1.7976931348623157e308;
+  double err = 0.0, err0 = 0.0;
+  bool ok = eval_phase(a.try_zero != 0, err, err0);
+  if (ok) {
+    if (err < errBest) {
+      errBest = err;
+      for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
+    }
+    // tryZero (GaussSeidelBase.impl.hpp:61-77): a better zero start also zeroes y
+    if (a.try_zero && err0 < errBest) {
+      errBest = err0;
+      for (int k = tid; k < n3; k += nthreads) {
+        cst(&a.xbest[k], 0.0);
+        cst(&a.x[k], 0.0);
+      }
+      for (int k = tid; k < 6 * nb; k += nthreads) cst(&e.ybody[k], 0.0);
+    }
+    if (tm.leader() && a.hist_cap > 0) a.hist[0] = errBest;
+    ok = tm.sync();
+  }
+
+  const double absSkipTol = fmin(a.skip_tol, a.tol);
+  double ndxRef = 0.0;
+  int it = 1, nfail = 0, nEval = 1;
+  bool converged = false;
+  for (; ok && it <= a.max_iters; ++it) {
+    for (int col = 0; col < a.ncolors && ok; ++col) {
+      const int c0 = a.colors[col], c1 = a.colors[col + 1];
+      double part[1] = {0.0};
+      for (int i = c0 + tid; i < c1; i += nthreads) {
+        if (a.skip[i]) {
+          a.skip[i] = (unsigned char)(a.skip[i] - 1);
+          continue;
+        }
+        double lx[3], x0[3];
+        cld3(a.x, i, lx);
+        x0[0] = lx[0];
+        x0[1] = lx[1];
+        x0[2] = lx[2];
+        double A[3][3];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) A[k / 3][k % 3] = a.local[(size_t)9 * i + k];
+        double lb[3];
+        ld3(a.P.b, i, lb);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) lb[r] -= A[r][0] * lx[0] + A[r][1] * lx[1] + A[r][2] * lx[2];
+        // rowMultiply: lb += H_i y, keeping y_k for the update below (contact i is
+        // the only one of its color at body k)
+        const int q0 = e.hrow[2 * i], nq = e.hrow[2 * i + 1];
+        double yk[2][6];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          if (t >= nq) break;
+          const int q = q0 + t;
+          const int k = H.bcol[q];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) yk[t][c] = mld(&e.ybody[6 * k + c]);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc += H.vals[(size_t)(r * 6 + c) * nnz + q] * yk[t][c];
+            lb[r] += acc;
+          }
+        }
+        for (int t = 2; t < nq; ++t) {  // contacts on more than two bodies
+          const int q = q0 + t;
+          const int k = H.bcol[q];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc += H.vals[(size_t)(r * 6 + c) * nnz + q] * mld(&e.ybody[6 * k + c]);
+            lb[r] += acc;
+          }
+        }
+        const double ldx0[3] = {-lx[0], -lx[1], -lx[2]};
+        const double s = a.P.scal[i];
+        const double lres = local::solve(coulomb, a.strategy, A, lb, lx, a.P.mu[i], a.local_tol, s);
+        const bool lok = a.local_tol > lres;
+        double ldx[3] = {ldx0[0] + lx[0], ldx0[1] + lx[1], ldx0[2] + lx[2]};
+        if (!lok) {
+          ldx[0] *= 0.5;
+          ldx[1] *= 0.5;
+          ldx[2] *= 0.5;
+          ++nfail;
+        }
+        cst(&a.x[3 * i], x0[0] + ldx[0]);
+        cst(&a.x[3 * i + 1], x0[1] + ldx[1]);
+        cst(&a.x[3 * i + 2], x0[2] + ldx[2]);
+        // colMultiply: y_k += (Minv_k H_ik^T) dx
+        auto dmt_row = [&](int q, int r) {
+          return e.dmt[(size_t)(r * 3 + 0) * nnz + q] * ldx[0] + e.dmt[(size_t)(r * 3 + 1) * nnz + q] * ldx[1] +
+                 e.dmt[(size_t)(r * 3 + 2) * nnz + q] * ldx[2];
+        };
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          if (t >= nq) break;
+          const int q = q0 + t;
+          const int k = H.bcol[q];
+#pragma unroll
+          for (int r = 0; r < 6; ++r) cst(&e.ybody[6 * k + r], yk[t][r] + dmt_row(q, r));
+        }
+        for (int t = 2; t < nq; ++t) {
+          const int q = q0 + t;
+          const int k = H.bcol[q];
+#pragma unroll
+          for (int r = 0; r < 6; ++r) cst(&e.ybody[6 * k + r], mld(&e.ybody[6 * k + r]) + dmt_row(q, r));
+        }
+        const double s2 = s * s;
+        const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
+        const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
+        part[0] = fmax(part[0], ndx2);
+        if (fmin(nx2, ndx2) < absSkipTol || ndx2 < a.skip_tol * fmin(nx2, ndxRef))
+          a.skip[i] = (unsigned char)a.skip_iters;
+      }
+      ok = tm.reduce<1>(part, 1u);
+      if (ok) ndxRef = fmax(ndxRef, tm.red[0]);
+    }
+    if (!ok) break;
+    if (it % a.eval_every == 0) {
+      double e0;
+      ok = eval_phase(false, err, e0);
+      if (!ok) break;
+      if (tm.leader() && nEval < a.hist_cap) a.hist[nEval] = err;
+      ++nEval;
+      if (err < errBest) {
+        errBest = err;
+        for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
+      }
+      if (err < a.tol) {
+        converged = true;
+        break;
+      }
+      ndxRef /= a.eval_every;
+      ok = tm.sync();
+    }
+  }
+  // maxIters exhausted: x = x_best (ProductGaussSeidel.impl.hpp:301)
+  if (ok && !converged)
+    for (int k = tid; k < n3; k += nthreads) cst(&a.x[k], cld(&a.xbest[k]));
+  {
+    double part[1] = {(double)nfail};
+    const bool ok2 = ok && tm.reduce<1>(part, 0u);
+    if (tm.leader()) {
+      a.out[0] = errBest;
+      a.out[1] = (double)(it > a.max_iters ? a.max_iters : it);
+      a.out[2] = converged ? 1.0 : 0.0;
+      a.out[3] = ok2 ? tm.red[0] : -1.0;
       a.out[4] = (double)nEval;
       a.out[5] = (ok && ok2) ? 0.0 : 1.0;
     }

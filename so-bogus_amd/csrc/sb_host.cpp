@@ -296,4 +296,56 @@ void coloring(int n, const int* wrowptr, const int* wcolidx, int mode, std::vect
   }
 }
 
+void coloringFromH(int nc, int nb, const int* hrowptr, const int* hcolidx, int mode, std::vector<int64_t>& perm,
+                   std::vector<int64_t>& colors) {
+  std::vector<std::vector<int>> members;
+  if (mode == 0) {
+    // Coloring::compute as written: the smallest lower neighbour of contact i is the
+    // first contact of one of its bodies, when that contact precedes i
+    std::vector<int> first(nb, std::numeric_limits<int>::max());
+    for (int i = 0; i < nc; ++i)
+      for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p) first[hcolidx[p]] = std::min(first[hcolidx[p]], i);
+    std::vector<int> last;
+    for (int i = 0; i < nc; ++i) {
+      int jmin = std::numeric_limits<int>::max();
+      for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p)
+        if (first[hcolidx[p]] < i) jmin = std::min(jmin, first[hcolidx[p]]);
+      size_t c = 0;
+      for (; c < members.size(); ++c)
+        if (jmin == std::numeric_limits<int>::max() || last[c] < jmin) break;
+      if (c == members.size()) {
+        members.emplace_back();
+        last.push_back(-1);
+      }
+      members[c].push_back(i);
+      last[c] = i;
+    }
+  } else {
+    // first fit over the colors already used at the contact's bodies
+    std::vector<std::vector<int>> used(nb);
+    std::vector<int> mark;
+    for (int i = 0; i < nc; ++i) {
+      for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p)
+        for (int c : used[hcolidx[p]]) {
+          if (c >= (int)mark.size()) mark.resize(c + 1, -1);
+          mark[c] = i;
+        }
+      int c = 0;
+      while (c < (int)mark.size() && mark[c] == i) ++c;
+      if (c >= (int)members.size()) members.resize(c + 1);
+      members[c].push_back(i);
+      for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p) {
+        auto& u = used[hcolidx[p]];
+        if (std::find(u.begin(), u.end(), c) == u.end()) u.push_back(c);
+      }
+    }
+  }
+  perm.clear();
+  colors.assign(members.size() + 1, 0);
+  for (size_t c = 0; c < members.size(); ++c) {
+    colors[c + 1] = colors[c] + (int64_t)members[c].size();
+    for (int i : members[c]) perm.push_back(i);
+  }
+}
+
 }  // namespace sbhost

@@ -42,4 +42,10 @@ void diagBlocks(int nc, const int* hrowptr, const int* hcolidx, const double* hv
 void coloring(int n, const int* wrowptr, const int* wcolidx, int mode, std::vector<int64_t>& perm,
               std::vector<int64_t>& colors);
 
+// The same two colorings computed from the contact-body incidence of H (two
+// contacts are coupled iff they share a body, which is the block pattern of
+// W = H Minv H^T), for problems whose W is not assembled.
+void coloringFromH(int nc, int nb, const int* hrowptr, const int* hcolidx, int mode, std::vector<int64_t>& perm,
+                   std::vector<int64_t>& colors);
+
 }  // namespace sbhost

@@ -156,4 +156,11 @@ struct GSArgs {
   double* out;          // [0] residual [1] iterations [2] converged [3] local failures
 };
 
+// Extra inputs of the matrix-free (product) Gauss-Seidel.
+struct PGSArgs {
+  const int* hrow;      // [2 nc]: tile-order index of contact i's first H block, its number of blocks
+  const double* dmt;    // [18][H.nblocks]: Minv_k H_ik^T (6x3, row-major) per H block, element-major
+  double* ybody;        // [6 nb]: Minv H^T x, kept current through the sweeps
+};
+
 }  // namespace sb

@@ -23,6 +23,7 @@ __global__ void pg_kernel(PGGridArgs<T> a);
 template <bool MF, typename T, int NT>
 __global__ void pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs);
 __global__ void gs_kernel(GSArgs a, const int* wrowptr);
+__global__ void pgs_kernel(GSArgs a, PGSArgs e);
 __global__ void adjoint_kernel(AdjArgs a);
 
 }  // namespace sb

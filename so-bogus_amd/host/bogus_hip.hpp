@@ -246,6 +246,12 @@ class GaussSeidel {
   mutable sb_stats m_stats{};
 };
 
+// ProductGaussSeidel<HType, MInvType, true> (ProductGaussSeidel.hpp:20-137): the
+// matrix-free Gauss-Seidel on W = H Minv H^T, W never formed. The GPU sweeps the
+// contacts color by color (contacts of one color share no body); the coloring is
+// computed inside solveWith, with the mode set on coloring().
+class ProductGaussSeidel : public GaussSeidel {};
+
 // ------------------------------------------------------------------ friction problems
 template <unsigned Dimension>
 struct PrimalFrictionProblem;
@@ -293,6 +299,74 @@ struct PrimalFrictionProblem<3> {
         for (int j = 0; j < 6; ++j) MInv[k * 36 + i * 6 + j] = a[i][j + 6];
     }
   }
+
+  // b = E^T w - H (Minv f) (FrictionProblem.cpp:30-40, PrimalFrictionProblem.cpp:44-45)
+  std::vector<double> dualRhs() const {
+    const int nc = H.rowsOfBlocks(), nb = H.colsOfBlocks();
+    if (MInv.size() != (size_t)nb * 36) throw std::logic_error("call computeMInv() first");
+    const auto& rp = H.rowPointers();
+    const auto& ci = H.columnIndices();
+    const auto& hv = H.blockValues();
+    std::vector<double> mf((size_t)6 * nb, 0.0);
+    for (int k = 0; k < nb; ++k)
+      for (int r = 0; r < 6; ++r) {
+        double acc = 0;
+        for (int c = 0; c < 6; ++c) acc += MInv[(size_t)k * 36 + r * 6 + c] * f[6 * k + c];
+        mf[6 * k + r] = acc;
+      }
+    std::vector<double> b((size_t)3 * nc, 0.0);
+    for (int i = 0; i < nc; ++i) {
+      for (int r = 0; r < 3; ++r) {
+        double ew = 0;
+        if (E.empty())
+          ew = w[3 * i + r];
+        else
+          for (int c = 0; c < 3; ++c) ew += E[(size_t)9 * i + c * 3 + r] * w[3 * i + c];
+        double hm = 0;
+        for (int p = rp[i]; p < rp[i + 1]; ++p)
+          for (int c = 0; c < 6; ++c) hm += hv[(size_t)18 * p + r * 6 + c] * mf[6 * ci[p] + c];
+        b[3 * i + r] = ew - hm;
+      }
+    }
+    return b;
+  }
+
+  using ProductGaussSeidelType = ProductGaussSeidel;
+  // PrimalFrictionProblem::solveWith(pgs, r, staticProblem = false) (PrimalFrictionProblem.cpp:40-59):
+  // b from the primal data, W never assembled. The contacts are colored and permuted on a
+  // device copy of the problem; r is returned in the caller's order.
+  double solveWith(ProductGaussSeidelType& pgs, double* r, const bool staticProblem = false,
+                   Context& ctx = Context::threadDefault()) const {
+    const int nc = H.rowsOfBlocks(), nb = H.colsOfBlocks();
+    const std::vector<double> b = dualRhs();
+    sb_problem* P = nullptr;
+    sb_check(sb_problem_create(ctx.get(), &P, nc, nb, H.rowPointers().data(), H.columnIndices().data(),
+                               H.blockValues().data(), MInv.data(), b.data(), mu, 0),
+             "sb_problem_create");
+    std::unique_ptr<sb_problem, int (*)(sb_problem*)> hold(P, sb_problem_destroy);
+    std::vector<int64_t> perm(nc), colors(nc + 1);
+    int ncol = 0;
+    sb_check(sb_problem_coloring(ctx.get(), P, pgs.coloring().mode, perm.data(), colors.data(), &ncol), "coloring");
+    sb_check(sb_problem_apply_permutation(ctx.get(), P, perm.data()), "apply_permutation");
+    pgs.coloring().permutation.assign(perm.begin(), perm.end());
+    pgs.coloring().colors.assign(colors.begin(), colors.begin() + ncol + 1);
+    std::vector<double> rp((size_t)3 * nc);
+    for (int i = 0; i < nc; ++i)
+      for (int k = 0; k < 3; ++k) rp[3 * (size_t)i + k] = r[3 * (size_t)perm[i] + k];
+    const sb_gs_options o = pgs.options(staticProblem ? SB_LAW_SOC : SB_LAW_COULOMB);
+    std::vector<double> hist(4096);
+    sb_stats& st = pgs.mutableStats();
+    st.history = hist.data();
+    st.history_capacity = (int)hist.size();
+    double res = 0;
+    sb_check(sb_pgs_solve(ctx.get(), P, &o, colors.data(), ncol, rp.data(), 0, &res, &st), "sb_pgs_solve");
+    for (int k = 0; k < st.history_length && k < (int)hist.size(); ++k)
+      pgs.callback().trigger(k * o.eval_every, hist[k]);
+    st.history = nullptr;
+    for (int i = 0; i < nc; ++i)
+      for (int k = 0; k < 3; ++k) r[3 * (size_t)perm[i] + k] = rp[3 * (size_t)i + k];
+    return res;
+  }
 };
 
 template <>
@@ -313,28 +387,7 @@ struct DualFrictionProblem<3> {
     const auto& rp = primal.H.rowPointers();
     const auto& ci = primal.H.columnIndices();
     const auto& hv = primal.H.blockValues();
-    // b = E^T w - H (Minv f)
-    std::vector<double> mf((size_t)6 * nb, 0.0);
-    for (int k = 0; k < nb; ++k)
-      for (int r = 0; r < 6; ++r) {
-        double acc = 0;
-        for (int c = 0; c < 6; ++c) acc += primal.MInv[(size_t)k * 36 + r * 6 + c] * primal.f[6 * k + c];
-        mf[6 * k + r] = acc;
-      }
-    b.assign((size_t)3 * nc, 0.0);
-    for (int i = 0; i < nc; ++i) {
-      for (int r = 0; r < 3; ++r) {
-        double ew = 0;
-        if (primal.E.empty())
-          ew = primal.w[3 * i + r];
-        else
-          for (int c = 0; c < 3; ++c) ew += primal.E[(size_t)9 * i + c * 3 + r] * primal.w[3 * i + c];
-        double hm = 0;
-        for (int p = rp[i]; p < rp[i + 1]; ++p)
-          for (int c = 0; c < 6; ++c) hm += hv[(size_t)18 * p + r * 6 + c] * mf[6 * ci[p] + c];
-        b[3 * i + r] = ew - hm;
-      }
-    }
+    b = primal.dualRhs();
     mu.assign(primal.mu, primal.mu + nc);
     sb_problem* P = nullptr;
     sb_check(sb_problem_create(m_ctx.get(), &P, nc, nb, rp.data(), ci.data(), hv.data(), primal.MInv.data(),

@@ -1,5 +1,6 @@
 // The reference's own solver tests (bogus tests/GaussSeidel.cpp:19-65,
-// tests/ProjectedGradient.cpp:15-83, tests/BlockMV.cpp:14-245), written against
+// tests/ProjectedGradient.cpp:15-83, tests/ProductGaussSeidel.cpp:16-62,
+// tests/BlockMV.cpp:14-245), written against
 // the C++ facade bogus_hip.hpp so that they read like the reference's, and run
 // on the GPU path. Exit status 0 when every check passes.
 #include <cmath>
@@ -120,6 +121,34 @@ static void testProjectedGradient() {
   CHECK(res < 1.e-8);
 }
 
+// tests/ProductGaussSeidel.cpp:44-61 (Product_GS_D_precomp) through
+// PrimalFrictionProblem::solveWith(pgs, r) (PrimalFrictionProblem.cpp:40-59): no W.
+static void testProductGaussSeidel() {
+  SmallFrictionPb pb;
+  bogus::PrimalFrictionProblem<3>::ProductGaussSeidelType pgs;
+  unsigned ncb = 0;
+  pgs.callback().connect([&](unsigned, double) { ++ncb; });
+  for (int mode : {SB_COLORING_GREEDY, SB_COLORING_REFERENCE}) {
+    std::vector<double> x(6, 1.0);
+    pgs.coloring().mode = mode;
+    const double res = pb.primal.solveWith(pgs, x.data());  // staticProblem = false: Coulomb
+    std::printf("Product GS coloring %d: residual %.3e x = %.7f %.7f %.7f %.7f %.7f %.7f\n", mode, res, x[0], x[1],
+                x[2], x[3], x[4], x[5]);
+    CHECK(res < 1.e-8);
+    CHECK(isApprox(pb.sol, x, 1.e-4));
+  }
+  CHECK(ncb > 0);
+  // the same solution through the dual problem's Gauss-Seidel on the assembled W
+  bogus::DualFrictionProblem<3> dual;
+  dual.computeFrom(pb.primal);
+  std::vector<double> x(6, 1.0);
+  bogus::GaussSeidel gs;
+  dual.solveWith(gs, x.data(), false);
+  std::vector<double> y(6, 1.0);
+  pb.primal.solveWith(pgs, y.data());
+  CHECK(isApprox(x, y, 1.e-10));
+}
+
 static void testBlockMV() {
   // tests/BlockMV.cpp:14-55 SmallSBM
   bogus::SparseBlockMatrix sbm;
@@ -167,6 +196,7 @@ int main() {
     testBlockMV();
     testGaussSeidel();
     testProjectedGradient();
+    testProductGaussSeidel();
   } catch (const std::exception& e) {
     std::printf("EXCEPTION %s\n", e.what());
     return 2;

@@ -322,10 +322,57 @@ static void testAssembly() {
     for (int i = 0; i < nc; ++i)
       for (int p = wrp[i]; p < wrp[i + 1]; ++p)
         if (wci[p] != i) CHECK(colorOf[i] != colorOf[wci[p]], "coloring %d: %d and %d coupled", mode, i, wci[p]);
+    // the coloring from H's contact-body incidence is the same coloring
+    std::vector<int64_t> hperm, hcolors;
+    sbhost::coloringFromH(nc, nb, H.rowptr.data(), H.colidx.data(), mode, hperm, hcolors);
+    CHECK(hperm == perm && hcolors == colors, "coloring %d from H differs from the one from W", mode);
+  }
+}
+
+// Colorings from H equal the ones from the assembled pattern on larger contact
+// graphs: hub bodies, contacts with one body and with none.
+static void testColoringFromH() {
+  std::mt19937 g(77);
+  for (int trial = 0; trial < 4; ++trial) {
+    const int nb = 300 + 200 * trial, nc = 2000 + 500 * trial;
+    std::uniform_int_distribution<int> body(0, nb - 1), hub(0, 4), kind(0, 19);
+    std::vector<int> rp{0}, ci;
+    std::vector<double> hv;
+    for (int i = 0; i < nc; ++i) {
+      const int k = kind(g);
+      std::vector<int> bs;
+      if (k == 0) {
+      } else if (k == 1) {
+        bs = {body(g)};
+      } else {
+        int a = k < 6 ? hub(g) : body(g), b = body(g);
+        while (b == a) b = body(g);
+        bs = {std::min(a, b), std::max(a, b)};
+      }
+      for (int j : bs) {
+        ci.push_back(j);
+        hv.insert(hv.end(), 18, 1.0);
+      }
+      rp.push_back((int)ci.size());
+    }
+    std::vector<double> minv((size_t)nb * 36, 0.0);
+    for (int k = 0; k < nb; ++k)
+      for (int r = 0; r < 6; ++r) minv[(size_t)k * 36 + r * 7] = 1.0;
+    std::vector<int> wrp, wci;
+    std::vector<double> wv;
+    sbhost::assembleW(nc, nb, rp.data(), ci.data(), hv.data(), minv.data(), wrp, wci, wv);
+    for (int mode : {0, 1}) {
+      std::vector<int64_t> perm, colors, hperm, hcolors;
+      sbhost::coloring(nc, wrp.data(), wci.data(), mode, perm, colors);
+      sbhost::coloringFromH(nc, nb, rp.data(), ci.data(), mode, hperm, hcolors);
+      CHECK(hperm == perm && hcolors == colors, "trial %d mode %d: %zu vs %zu colors", trial, mode, hcolors.size(),
+            colors.size());
+    }
   }
 }
 
 int main() {
+  testColoringFromH();
   testTiling();
   testTranspose();
   testAssembly();
