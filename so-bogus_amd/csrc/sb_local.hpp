@@ -27,6 +27,9 @@ __device__ __forceinline__ bool isz(double s) { return fabs(s) < kEps; }
 
 // ---------------------------------------------------------------- LU (complete pivoting)
 // Solves A x = b; unknowns beyond the numerical rank (pivots <= N*eps*max pivot) are 0.
+// Pivot rows/columns are runtime values, but every array access uses a compile-time
+// index (row/column swaps select the matching entry), so a, c and the column
+// permutation stay in registers.
 template <int N>
 __device__ inline void lu_solve(const double Ain[N][N], const double bin[N], double x[N]) {
   double a[N][N], c[N];
@@ -45,7 +48,9 @@ __device__ inline void lu_solve(const double Ain[N][N], const double bin[N], dou
     if (k >= npiv) break;
     double big = -1.0;
     int rb = k, cb = k;
+#pragma unroll
     for (int j = k; j < N; ++j)
+#pragma unroll
       for (int i = k; i < N; ++i) {
         const double v = fabs(a[i][j]);
         if (v > big) {
@@ -59,42 +64,65 @@ __device__ inline void lu_solve(const double Ain[N][N], const double bin[N], dou
       break;
     }
     maxpiv = fmax(maxpiv, big);
-    if (rb != k) {
-      for (int j = 0; j < N; ++j) {
-        const double t = a[k][j];
-        a[k][j] = a[rb][j];
-        a[rb][j] = t;
+#pragma unroll
+    for (int r = k + 1; r < N; ++r)
+      if (r == rb) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const double t = a[k][j];
+          a[k][j] = a[r][j];
+          a[r][j] = t;
+        }
+        const double t = c[k];
+        c[k] = c[r];
+        c[r] = t;
       }
-      const double t = c[k];
-      c[k] = c[rb];
-      c[rb] = t;
-    }
-    if (cb != k) {
-      for (int i = 0; i < N; ++i) {
-        const double t = a[i][k];
-        a[i][k] = a[i][cb];
-        a[i][cb] = t;
+#pragma unroll
+    for (int q = k + 1; q < N; ++q)
+      if (q == cb) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const double t = a[i][k];
+          a[i][k] = a[i][q];
+          a[i][q] = t;
+        }
+        const int t = cperm[k];
+        cperm[k] = cperm[q];
+        cperm[q] = t;
       }
-      const int t = cperm[k];
-      cperm[k] = cperm[cb];
-      cperm[cb] = t;
-    }
+#pragma unroll
     for (int i = k + 1; i < N; ++i) {
       a[i][k] /= a[k][k];
+#pragma unroll
       for (int j = k + 1; j < N; ++j) a[i][j] -= a[i][k] * a[k][j];
     }
   }
   const double thr = maxpiv * (N * kEps);
   int rank = 0;
-  for (int i = 0; i < npiv; ++i) rank += (fabs(a[i][i]) > thr) ? 1 : 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (i < npiv) rank += (fabs(a[i][i]) > thr) ? 1 : 0;
   // forward substitution (unit lower), full size
+#pragma unroll
   for (int i = 1; i < N; ++i)
+#pragma unroll
     for (int j = 0; j < i; ++j) c[i] -= a[i][j] * c[j];
-  for (int i = rank - 1; i >= 0; --i) {
-    for (int j = i + 1; j < rank; ++j) c[i] -= a[i][j] * c[j];
-    c[i] /= a[i][i];
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    if (i < rank) {
+#pragma unroll
+      for (int j = i + 1; j < N; ++j)
+        if (j < rank) c[i] -= a[i][j] * c[j];
+      c[i] /= a[i][i];
+    }
   }
-  for (int i = 0; i < N; ++i) x[cperm[i]] = (i < rank) ? c[i] : 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double v = (i < rank) ? c[i] : 0.0;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+      if (cperm[i] == p) x[p] = v;
+  }
 }
 
 // ---------------------------------------------------------------- quartic roots
@@ -113,154 +141,227 @@ __device__ inline void eig2(double a, double b, double c, double d, double& r1, 
   }
 }
 
-// Eigenvalues of an n x n (n <= 4) upper-Hessenberg matrix h (modified in place).
-__device__ inline void hess_eig(int n, double h[4][4], double wr[4], double wi[4]) {
+// One implicit double-shift (Francis) step on the active window [LO, HI] of the
+// Hessenberg matrix h, shifts s (sum) and t (product). Every index is a
+// compile-time constant, so h stays in registers (a runtime-indexed 4x4 array
+// would live in scratch memory and put a memory round trip on every access).
+template <int LO, int HI>
+__device__ __forceinline__ void francis_step(double (&h)[4][4], double s, double t) {
+  double x = h[LO][LO] * h[LO][LO] + h[LO][LO + 1] * h[LO + 1][LO] - s * h[LO][LO] + t;
+  double y = h[LO + 1][LO] * (h[LO][LO] + h[LO + 1][LO + 1] - s);
+  double z = h[LO + 1][LO] * h[LO + 2][LO + 1];
+#pragma unroll
+  for (int k = LO; k <= HI - 2; ++k) {
+    const double alpha = -copysign(sqrt(x * x + y * y + z * z), x);
+    const double v0 = x - alpha, v1 = y, v2 = z;
+    const double vv = v0 * v0 + v1 * v1 + v2 * v2;
+    if (vv != 0.0) {
+      const double beta = 2.0 / vv;
+      const int q = (k - 1 > LO) ? k - 1 : LO;
+#pragma unroll
+      for (int j = q; j <= HI; ++j) {
+        const double d = beta * (v0 * h[k][j] + v1 * h[k + 1][j] + v2 * h[k + 2][j]);
+        h[k][j] -= d * v0;
+        h[k + 1][j] -= d * v1;
+        h[k + 2][j] -= d * v2;
+      }
+      const int r = (k + 3 < HI) ? k + 3 : HI;
+#pragma unroll
+      for (int i = LO; i <= r; ++i) {
+        const double d = beta * (h[i][k] * v0 + h[i][k + 1] * v1 + h[i][k + 2] * v2);
+        h[i][k] -= d * v0;
+        h[i][k + 1] -= d * v1;
+        h[i][k + 2] -= d * v2;
+      }
+    }
+    x = h[k + 1][k];
+    y = h[k + 2][k];
+    if (k < HI - 2) z = h[k + 3][k];
+  }
+  {
+    const double alpha = -copysign(sqrt(x * x + y * y), x);
+    const double v0 = x - alpha, v1 = y;
+    const double vv = v0 * v0 + v1 * v1;
+    if (vv != 0.0) {
+      const double beta = 2.0 / vv;
+      constexpr int k = HI - 1;
+#pragma unroll
+      for (int j = (k - 1 > LO ? k - 1 : LO); j <= HI; ++j) {
+        const double d = beta * (v0 * h[k][j] + v1 * h[k + 1][j]);
+        h[k][j] -= d * v0;
+        h[k + 1][j] -= d * v1;
+      }
+#pragma unroll
+      for (int i = LO; i <= HI; ++i) {
+        const double d = beta * (h[i][k] * v0 + h[i][k + 1] * v1);
+        h[i][k] -= d * v0;
+        h[i][k + 1] -= d * v1;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = LO + 2; i <= HI; ++i)
+#pragma unroll
+    for (int j = LO; j < i - 1; ++j) h[i][j] = 0.0;
+}
+
+// h[i][j] for runtime i, j in 0..3 by selection over the (register) entries
+__device__ __forceinline__ double hsel(const double (&h)[4][4], int i, int j) {
+  double v = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) v = (a == i && b == j) ? h[a][b] : v;
+  return v;
+}
+
+// Eigenvalues of an n x n (n <= 4) upper-Hessenberg matrix h (modified in place):
+// implicit double-shift QR with deflation (Golub & Van Loan, Alg. 7.5.1/7.5.2),
+// after scaling h by its largest |entry|; ad-hoc shifts after 10 and 20 stagnant
+// iterations; deflated 2x2 blocks by eig2. Runtime indices only select among
+// registers (hsel, the switch over the active window).
+__device__ inline void hess_eig(int n, double (&h)[4][4], double (&wr)[4], double (&wi)[4]) {
   double scale = 0.0;
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) scale = fmax(scale, fabs(h[i][j]));
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (i < n && j < n) scale = fmax(scale, fabs(h[i][j]));
   if (scale == 0.0) {
-    for (int i = 0; i < n; ++i) wr[i] = wi[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < n) wr[i] = wi[i] = 0.0;
     return;
   }
   const double inv = 1.0 / scale;
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) h[i][j] *= inv;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (i < n && j < n) h[i][j] *= inv;
+  auto setw = [&](int k, double r, double im) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      if (a == k) {
+        wr[a] = r;
+        wi[a] = im;
+      }
+  };
   int hi = n - 1, iter = 0, total = 0;
   const int maxTotal = 40 * n;
   while (hi >= 0) {
-    int lo = hi;
-    while (lo > 0) {
-      const double s = fabs(h[lo - 1][lo - 1]) + fabs(h[lo][lo]);
-      if (fabs(h[lo][lo - 1]) <= kEps * (s == 0.0 ? 1.0 : s)) {
-        h[lo][lo - 1] = 0.0;
-        break;
+    // deflation: the largest lo <= hi whose subdiagonal entry is negligible (else 0)
+    int lo = 0;
+#pragma unroll
+    for (int l = 3; l >= 1; --l) {
+      if (l <= hi && lo == 0) {
+        const double s = fabs(h[l - 1][l - 1]) + fabs(h[l][l]);
+        if (fabs(h[l][l - 1]) <= kEps * (s == 0.0 ? 1.0 : s)) {
+          h[l][l - 1] = 0.0;
+          lo = l;
+        }
       }
-      --lo;
     }
+    // the search above stops at the first negligible entry from hi downwards
     if (lo == hi) {
-      wr[hi] = h[hi][hi] * scale;
-      wi[hi] = 0.0;
+      setw(hi, hsel(h, hi, hi) * scale, 0.0);
       --hi;
       iter = 0;
       continue;
     }
     if (lo == hi - 1) {
       double r1, i1, r2, i2;
-      eig2(h[hi - 1][hi - 1], h[hi - 1][hi], h[hi][hi - 1], h[hi][hi], r1, i1, r2, i2);
-      wr[hi - 1] = r1 * scale;
-      wi[hi - 1] = i1 * scale;
-      wr[hi] = r2 * scale;
-      wi[hi] = i2 * scale;
+      eig2(hsel(h, hi - 1, hi - 1), hsel(h, hi - 1, hi), hsel(h, hi, hi - 1), hsel(h, hi, hi), r1, i1, r2, i2);
+      setw(hi - 1, r1 * scale, i1 * scale);
+      setw(hi, r2 * scale, i2 * scale);
       hi -= 2;
       iter = 0;
       continue;
     }
     if (total >= maxTotal) {
-      for (int k = lo; k <= hi; ++k) {
-        wr[k] = h[k][k] * scale;
-        wi[k] = 0.0;
-      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k >= lo && k <= hi) {
+          wr[k] = h[k][k] * scale;
+          wi[k] = 0.0;
+        }
       hi = lo - 1;
       continue;
     }
     ++iter;
     ++total;
     double s, t;
+    const double hmm = hsel(h, hi - 1, hi - 1), hhh = hsel(h, hi, hi);
     if (iter == 10 || iter == 20) {
-      const double e = fabs(h[hi][hi - 1]) + fabs(h[hi - 1][hi - 2]);
+      const double e = fabs(hsel(h, hi, hi - 1)) + fabs(hsel(h, hi - 1, hi - 2));
       s = 1.5 * e;
       t = e * e;
     } else {
-      s = h[hi - 1][hi - 1] + h[hi][hi];
-      t = h[hi - 1][hi - 1] * h[hi][hi] - h[hi - 1][hi] * h[hi][hi - 1];
+      s = hmm + hhh;
+      t = hmm * hhh - hsel(h, hi - 1, hi) * hsel(h, hi, hi - 1);
     }
-    double x = h[lo][lo] * h[lo][lo] + h[lo][lo + 1] * h[lo + 1][lo] - s * h[lo][lo] + t;
-    double y = h[lo + 1][lo] * (h[lo][lo] + h[lo + 1][lo + 1] - s);
-    double z = h[lo + 1][lo] * h[lo + 2][lo + 1];
-    for (int k = lo; k <= hi - 2; ++k) {
-      const double alpha = -copysign(sqrt(x * x + y * y + z * z), x);
-      const double v0 = x - alpha, v1 = y, v2 = z;
-      const double vv = v0 * v0 + v1 * v1 + v2 * v2;
-      if (vv != 0.0) {
-        const double beta = 2.0 / vv;
-        const int q = (k - 1 > lo) ? k - 1 : lo;
-        for (int j = q; j <= hi; ++j) {
-          const double d = beta * (v0 * h[k][j] + v1 * h[k + 1][j] + v2 * h[k + 2][j]);
-          h[k][j] -= d * v0;
-          h[k + 1][j] -= d * v1;
-          h[k + 2][j] -= d * v2;
-        }
-        const int r = (k + 3 < hi) ? k + 3 : hi;
-        for (int i = lo; i <= r; ++i) {
-          const double d = beta * (h[i][k] * v0 + h[i][k + 1] * v1 + h[i][k + 2] * v2);
-          h[i][k] -= d * v0;
-          h[i][k + 1] -= d * v1;
-          h[i][k + 2] -= d * v2;
-        }
-      }
-      x = h[k + 1][k];
-      y = h[k + 2][k];
-      if (k < hi - 2) z = h[k + 3][k];
-    }
-    {
-      const double alpha = -copysign(sqrt(x * x + y * y), x);
-      const double v0 = x - alpha, v1 = y;
-      const double vv = v0 * v0 + v1 * v1;
-      if (vv != 0.0) {
-        const double beta = 2.0 / vv;
-        const int k = hi - 1;
-        for (int j = (k - 1 > lo ? k - 1 : lo); j <= hi; ++j) {
-          const double d = beta * (v0 * h[k][j] + v1 * h[k + 1][j]);
-          h[k][j] -= d * v0;
-          h[k + 1][j] -= d * v1;
-        }
-        for (int i = lo; i <= hi; ++i) {
-          const double d = beta * (h[i][k] * v0 + h[i][k + 1] * v1);
-          h[i][k] -= d * v0;
-          h[i][k + 1] -= d * v1;
-        }
-      }
-    }
-    for (int i = lo + 2; i <= hi; ++i)
-      for (int j = lo; j < i - 1; ++j) h[i][j] = 0.0;
+    if (lo == 0 && hi == 2)
+      francis_step<0, 2>(h, s, t);
+    else if (lo == 0 && hi == 3)
+      francis_step<0, 3>(h, s, t);
+    else
+      francis_step<1, 3>(h, s, t);
   }
 }
 
 enum RootFilter { StrictlyPositive = 0, AllRootsF = 2 };
 
 // Real roots (|Im| < eps) of the monic x^4 + c3 x^3 + c2 x^2 + c1 x + c0.
-__device__ inline int quartic_real_roots(const double c[4], double roots[4], RootFilter filter) {
+// Appends v to roots[0..n) (runtime n, register-resident array).
+__device__ __forceinline__ void push_root(double (&roots)[4], int& n, double v) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    if (a == n) roots[a] = v;
+  ++n;
+}
+
+__device__ inline int quartic_real_roots(const double c[4], double (&roots)[4], RootFilter filter) {
   double h[4][4];
+#pragma unroll
   for (int i = 0; i < 4; ++i)
+#pragma unroll
     for (int j = 0; j < 4; ++j) h[i][j] = 0.0;
+#pragma unroll
   for (int i = 1; i < 4; ++i) h[i][i - 1] = 1.0;
+#pragma unroll
   for (int i = 0; i < 4; ++i) h[i][3] = -c[i];
   double wr[4], wi[4];
   hess_eig(4, h, wr, wi);
   int n = 0;
+#pragma unroll
   for (int i = 0; i < 4; ++i)
-    if (isz(wi[i]) && !(filter == StrictlyPositive && wr[i] <= 0.0)) roots[n++] = wr[i];
+    if (isz(wi[i]) && !(filter == StrictlyPositive && wr[i] <= 0.0)) push_root(roots, n, wr[i]);
   return n;
 }
 
 // Real roots of c[D] x^D + ... + c[0] with a possibly vanishing leading coefficient.
-__device__ inline int poly_real_roots_degenerate(double c[5], double roots[4]) {
+__device__ inline int poly_real_roots_degenerate(double c[5], double (&roots)[4]) {
   int D = 4;
   while (D > 0 && isz(c[D])) --D;
   if (D == 0) {
     roots[0] = 0.0;
     return isz(c[0]) ? 1 : 0;
   }
-  const double inv = 1.0 / c[D];
+  double cD = c[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    if (a == D) cD = c[a];
+  const double inv = 1.0 / cD;
   double h[4][4], wr[4], wi[4];
-  for (int i = 0; i < D; ++i)
-    for (int j = 0; j < D; ++j) h[i][j] = 0.0;
-  for (int i = 1; i < D; ++i) h[i][i - 1] = 1.0;
-  for (int i = 0; i < D; ++i) h[i][D - 1] = -(c[i] * inv);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[i][j] = (i < D && j < D) ? ((j == D - 1) ? -(c[i] * inv) : (i == j + 1 ? 1.0 : 0.0)) : 0.0;
   hess_eig(D, h, wr, wi);
   int n = 0;
-  for (int i = 0; i < D; ++i)
-    if (isz(wi[i])) roots[n++] = wr[i];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < D && isz(wi[i])) push_root(roots, n, wr[i]);
   return n;
 }
 
@@ -453,7 +554,9 @@ __device__ inline bool solve_orthogonality(bool coulomb, const double W[3][3], c
     const int n = quartic_real_roots(c, roots, StrictlyPositive);
     if (n == 0) return false;
     double alpha = roots[0];
-    for (int i = 1; i < n; ++i) alpha = fmin(alpha, roots[i]);
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+      if (i < n) alpha = fmin(alpha, roots[i]);
     const double M[2][2] = {{W00 + alpha, W01}, {W10, W11 + alpha}};
     const double rhs[2] = {-bN * bb0, -bN * bb1};
     double rt[2];
@@ -471,7 +574,9 @@ __device__ inline bool solve_orthogonality(bool coulomb, const double W[3][3], c
   double c[5] = {A + B, C - D, E, C + D, B - A};
   double roots[4];
   const int n = poly_real_roots_degenerate(c, roots);
-  for (int i = 0; i < n; ++i) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= n) break;
     const double t = roots[i];
     const double CT = (1.0 - t * t) / (1.0 + t * t);
     const double ST = 2.0 * t / (1.0 + t * t);
