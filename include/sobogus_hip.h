@@ -198,7 +198,9 @@ int sb_fb_residual(sb_ctx* ctx, const sb_problem* P, int law, int op, int scalin
 int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, double* r, int on_device,
                 double* residual, sb_stats* stats);
 /* GaussSeidel::solve on an assembled problem whose contacts are color-contiguous:
-   colors holds n_colors + 1 offsets (pass NULL / 0 for a single color). */
+   colors holds n_colors + 1 offsets; a color must hold no two coupled contacts (W_ij != 0),
+   else SB_ERR_INVALID. With colors NULL the contacts are swept in their order (the
+   reference's sequential sweep), in runs of consecutive uncoupled contacts. */
 int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
                 double* r, int on_device, double* residual, sb_stats* stats);
 /* Matrix-free Gauss-Seidel: ProductGaussSeidel<H, Minv>::solve (ProductGaussSeidel.impl.hpp:
@@ -210,6 +212,22 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, con
    sequential sweep), in runs of consecutive contacts that share no body. */
 int sb_pgs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
                  double* r, int on_device, double* residual, sb_stats* stats);
+
+/* Cadoux fixed point (DualFrictionProblem::solveCadoux / solveCadouxVel,
+   src/Interfaces/FrictionProblem.cpp:68-100, Cadoux.hpp:44-155): up to `iterations` SOC
+   solves with the inner solver, each with the affine term b + s(W r + b) (s the De Saxce
+   term mu |u_T| of the previous iterate), at a tolerance tightened to
+   tol_tighten * max(tol, min(res, 1)); res is the Coulomb residual of the iterate with
+   the inner solver's scaling. velocity = 1: solveCadouxVel, x is u and the problem is
+   read as r = W u + b. The inner options' law is ignored (SOC). stats: iterations =
+   fixed-point iterations, history = residual before and after each, spmvs = all
+   operator applications. Returns SB_OK when res < tol, SB_NOT_CONVERGED otherwise. */
+int sb_pg_solve_cadoux(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, int iterations,
+                       double tol_tighten, int velocity, double* x, int on_device, double* residual,
+                       sb_stats* stats);
+int sb_gs_solve_cadoux(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors,
+                       int n_colors, int iterations, double tol_tighten, int velocity, double* x, int on_device,
+                       double* residual, sb_stats* stats);
 
 /* ---------------------------------------------------------------- derivatives
    Reverse-mode derivatives of L = g . r for the projected fixed-point map of the

@@ -1670,4 +1670,119 @@ double pgsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& co
   return errBest;
 }
 
+// Cadoux fixed point (src/Interfaces/Cadoux.hpp:44-91; DualFrictionProblem::solveCadoux,
+// FrictionProblem.cpp:68-83, FrictionProblem.impl.hpp:76-91): a sequence of SOC problems
+// whose affine term carries the De Saxce correction of the previous iterate,
+// b_k = b + s(W r_k + b), each solved by the inner solver (warm-started) at a tolerance
+// tightened to tolTighten * max(tol, min(res, 1)); res is the Coulomb residual of the
+// iterate with the inner solver's own scaling. velocity = true is solveCadouxVel
+// (Cadoux.hpp:95-155): the problem read as r = W u + b, the unknown u shifted by
+// s_N = |u_T| max(0, 1/mu).
+double cadouxSolve(const Problem& P, const CadouxOptions& o, const std::vector<int>& colors, double* xio,
+                   SolveStats* st) {
+  const int n = P.nc;
+  const size_t n3 = 3 * (size_t)n;
+  Problem Q = P;  // its b is replaced by the inner problems' affine terms
+  // the minimizer's scaling (used by its eval): GS from the local matrices, PG by updateScalings
+  std::vector<double> scaling(n, 1.0);
+  const bool useDiag = o.useGS || o.pg.useAssembled || o.pg.diagScaling;
+  if (useDiag)
+    for (int i = 0; i < n; ++i) {
+      const double* d = &P.diag[(size_t)i * 9];
+      scaling[i] = std::max(1.0, (d[0] + d[4] + d[8]) / 3);
+    }
+  const bool assembled = o.useGS || o.pg.useAssembled;
+  const Norm norm = (o.useGS ? o.gs.infNorm : o.pg.infNorm) ? Infinity : MeanSquare;
+  const double tol = o.useGS ? o.gs.tol : o.pg.tol;
+  auto W = [&](const Vec& x, Vec& y) { applyW(P, assembled, x.data(), y.data()); };
+  unsigned spmvs = 0;
+  auto inner = [&](Vec& x, double innerTol) {
+    SolveStats is;
+    if (o.useGS) {
+      GSOptions g = o.gs;
+      g.tol = innerTol;
+      g.coulomb = 0;
+      gsSolve(Q, g, colors, x.data(), &is);
+    } else {
+      PGOptions g = o.pg;
+      g.tol = innerTol;
+      g.coulomb = 0;
+      pgSolve(Q, g, x.data(), &is);
+    }
+    spmvs += is.spmvs;
+  };
+  Vec x(xio, xio + n3), s(n3), y(n3);
+  double res;
+  unsigned it = 0;
+  if (!o.velocity) {
+    // s = W r + b; res = eval(coulomb, s, r)
+    W(x, s);
+    ++spmvs;
+    for (size_t k = 0; k < n3; ++k) s[k] += P.b[k];
+    res = evalResidual(n, true, P.mu.data(), scaling.data(), s.data(), x.data(), norm);
+    if (st) st->history.push_back(res);
+    for (it = 0; it < o.iterations; ++it) {
+      const double innerTol = o.tolTighten * std::max(tol, std::min(res, 1.));
+      for (int i = 0; i < n; ++i) {  // dualityCOV(coulomb, s, s); s += b
+        double c[3];
+        lawCOV(true, P.mu[i], &s[3 * i], c);
+        for (int k = 0; k < 3; ++k) Q.b[3 * i + k] = c[k] + P.b[3 * i + k];
+      }
+      inner(x, innerTol);
+      W(x, s);
+      ++spmvs;
+      for (size_t k = 0; k < n3; ++k) s[k] += P.b[k];
+      res = evalResidual(n, true, P.mu.data(), scaling.data(), s.data(), x.data(), norm);
+      if (st) st->history.push_back(res);
+      if (res < tol) {
+        ++it;
+        break;
+      }
+    }
+  } else {
+    // x holds u; ustar = u + s, s_N = |u_T| max(0, 1/mu)
+    auto shift = [&](const Vec& v, Vec& sv) {
+      for (int i = 0; i < n; ++i) {
+        sv[3 * i] = std::sqrt(v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]) * std::max(0., 1. / P.mu[i]);
+        sv[3 * i + 1] = sv[3 * i + 2] = 0.0;
+      }
+    };
+    Vec u = x, ustar(n3), Ws(n3);
+    shift(u, s);
+    for (size_t k = 0; k < n3; ++k) ustar[k] = u[k] + s[k];
+    W(u, y);
+    ++spmvs;
+    for (size_t k = 0; k < n3; ++k) y[k] += P.b[k];
+    res = evalResidual(n, false, P.mu.data(), scaling.data(), y.data(), ustar.data(), norm);
+    if (st) st->history.push_back(res);
+    for (it = 0; it < o.iterations; ++it) {
+      const double innerTol = o.tolTighten * std::max(tol, std::min(res, 1.));
+      W(s, Ws);  // Wsb = b - W s
+      ++spmvs;
+      for (size_t k = 0; k < n3; ++k) Q.b[k] = P.b[k] - Ws[k];
+      inner(ustar, innerTol);
+      for (size_t k = 0; k < n3; ++k) u[k] = ustar[k] - s[k];
+      shift(ustar, s);
+      for (size_t k = 0; k < n3; ++k) ustar[k] = u[k] + s[k];
+      W(u, y);
+      ++spmvs;
+      for (size_t k = 0; k < n3; ++k) y[k] += P.b[k];
+      res = evalResidual(n, false, P.mu.data(), scaling.data(), y.data(), ustar.data(), norm);
+      if (st) st->history.push_back(res);
+      if (it > 0 && res < tol) {
+        ++it;
+        break;
+      }
+    }
+    x = u;
+  }
+  std::copy(x.begin(), x.end(), xio);
+  if (st) {
+    st->iterations = it;
+    st->spmvs = spmvs;
+    st->residual = res;
+  }
+  return res;
+}
+
 }  // namespace cpuref

@@ -182,6 +182,20 @@ double gsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& col
 // Gauss-Seidel; W is never used. Same options and color handling as gsSolve.
 double pgsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& colors, double* x, SolveStats* st);
 
+struct CadouxOptions {
+  int useGS = 0;           // inner solver: GaussSeidel (gs, colors) or ProjectedGradient (pg)
+  GSOptions gs;
+  PGOptions pg;
+  unsigned iterations = 10;
+  double tolTighten = 1e-1;
+  int velocity = 0;        // 0: solveCadoux (unknown r), 1: solveCadouxVel (unknown u, r = W u + b)
+};
+// Cadoux fixed point over SOC solves (src/Interfaces/Cadoux.hpp:44-155). The inner
+// solver's law is SOC whatever its options say; st->iterations = fixed-point iterations,
+// st->history = the residual before and after each.
+double cadouxSolve(const Problem& P, const CadouxOptions& o, const std::vector<int>& colors, double* x,
+                   SolveStats* st);
+
 // bogus::applyPermutation (src/Core/Block/SparseBlockMatrixBase.impl.hpp:309-326): new[i] = old[perm[i]]
 // and DualFrictionProblem::applyPermutation (src/Interfaces/FrictionProblem.cpp:102-114).
 void permuteProblem(Problem& P, const std::vector<int>& perm);

@@ -157,6 +157,37 @@ double orc_pgs_solve(void* p, double tol, int maxIters, int evalEvery, double sk
                   ndxRefPerColor, colors, ncolors, r, iters, hist, histCap, histLen);
 }
 
+// Cadoux fixed point with the GS (use_gs = 1: colors) or APGD/SPG (use_gs = 0) inner solver.
+double orc_cadoux_solve(void* p, int useGS, int variant, int useAssembled, int diagScaling, double tol, int innerMaxIters,
+                        int iterations, double tolTighten, int velocity, const int* colors, int ncolors, double* r,
+                        int* iters, int* spmvs, double* hist, int histCap, int* histLen) {
+  const Problem* P = static_cast<Problem*>(p);
+  CadouxOptions o;
+  o.useGS = useGS;
+  o.gs.tol = tol;
+  o.gs.maxIters = (unsigned)innerMaxIters;
+  o.gs.ndxRefPerColor = 1;
+  o.pg.variant = variant;
+  o.pg.tol = tol;
+  o.pg.maxIters = (unsigned)innerMaxIters;
+  o.pg.useAssembled = useAssembled;
+  o.pg.diagScaling = diagScaling;
+  o.iterations = (unsigned)iterations;
+  o.tolTighten = tolTighten;
+  o.velocity = velocity;
+  std::vector<int> c;
+  if (colors && ncolors > 0)
+    c.assign(colors, colors + ncolors + 1);
+  else
+    c = {0, P->nc};
+  SolveStats st;
+  const double res = cadouxSolve(*P, o, c, r, &st);
+  if (iters) *iters = (int)st.iterations;
+  if (spmvs) *spmvs = (int)st.spmvs;
+  copyHistory(st, hist, histCap, histLen);
+  return res;
+}
+
 int orc_coloring(void* p, int* perm, int* colors) {
   const Problem* P = static_cast<Problem*>(p);
   std::vector<int> pm, cl;

@@ -60,6 +60,9 @@ def lib():
             "orc_pgs_solve": (C.c_double, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                            C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
                                            _dp, _ip, _dp, C.c_int, _ip]),
+            "orc_cadoux_solve": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                              C.c_int, C.c_double, C.c_int, _ip, C.c_int, _dp, _ip, _ip, _dp,
+                                              C.c_int, _ip]),
             "orc_coloring": (C.c_int, [C.c_void_p, _ip, _ip]),
             "orc_problem_permute": (None, [C.c_void_p, _ip]),
             "orc_fb": (None, [C.c_int, C.c_double, _dp, _dp, _dp]),
@@ -202,6 +205,23 @@ class OracleProblem:
     def pgs_solve(self, r0, **kw):
         """ProductGaussSeidel::solve with D = Minv (ProductGaussSeidel.impl.hpp:64-304): W is not used."""
         return self._gs(lib().orc_pgs_solve, r0, **kw)
+
+    def cadoux_solve(self, r0, use_gs=False, variant=PG_APGD, assembled=True, diag_scaling=True, tol=1e-6,
+                     inner_max_iters=300, iterations=10, tol_tighten=0.1, velocity=False, colors=None,
+                     hist_cap=1000):
+        """solveCadoux / solveCadouxVel (src/Interfaces/Cadoux.hpp:44-155) with the GS or PG inner solver."""
+        r = f64(r0).copy()
+        it, sp, hl = C.c_int(0), C.c_int(0), C.c_int(0)
+        hist = np.zeros(hist_cap)
+        if colors is None:
+            cp, nco = None, 0
+        else:
+            colors = i32(colors)
+            cp, nco = _i(colors), len(colors) - 1
+        res = lib().orc_cadoux_solve(self.h, int(use_gs), variant, int(assembled), int(diag_scaling), tol,
+                                     inner_max_iters, iterations, tol_tighten, int(velocity), cp, nco, _d(r),
+                                     C.byref(it), C.byref(sp), _d(hist), hist_cap, C.byref(hl))
+        return res, r, dict(iterations=it.value, spmvs=sp.value, history=hist[:min(hl.value, hist_cap)].copy())
 
     def coloring(self):
         perm = np.zeros(self.nc, np.int32)

@@ -164,6 +164,7 @@ struct sb_problem {
   DBuf<float> d_minv32;
   // workspace (FP64 solvers and kernels)
   DBuf<double> X[4], MX[4], G[3], xbest, wbody, carry, rio, hist, y, terms, local, reg, ndx;
+  DBuf<double> cad[5];  // Cadoux fixed point: original b, iterate, shift, operator image, velocities
   // workspace of the FP32 (mixed-precision) projected-gradient solver
   DBuf<float> X32[4], MX32[4], G32[3], xbest32, wbody32, carry32;
   DBuf<unsigned char> skip;
@@ -1135,6 +1136,39 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   if (!Pc || !o || !r) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (!P->hasW) return SB_ERR_INVALID;
+  // Contacts of one color are solved in parallel from each other's previous values, so
+  // a color must hold no coupled pair (W_ij != 0). Given colors are checked; without
+  // colors the natural order is cut into maximal runs of uncoupled contacts, which
+  // sweeps the contacts exactly in the reference's sequential order.
+  std::vector<int64_t> runs;
+  {
+    const int n = P->nc;
+    std::vector<int> colorOf(n, -1);
+    if (colors && ncolors > 0) {
+      if (colors[0] != 0 || colors[ncolors] != n) return SB_ERR_INVALID;
+      for (int c = 0; c < ncolors; ++c) {
+        if (colors[c + 1] < colors[c]) return SB_ERR_INVALID;
+        for (int64_t i = colors[c]; i < colors[c + 1]; ++i) colorOf[i] = c;
+      }
+      for (int i = 0; i < n; ++i)
+        for (int p = P->wrp[i]; p < P->wrp[i + 1]; ++p)
+          if (P->wci[p] != i && colorOf[P->wci[p]] == colorOf[i]) return SB_ERR_INVALID;
+    } else {
+      runs.push_back(0);
+      int start = 0;
+      for (int i = 0; i < n; ++i) {
+        bool clash = false;
+        for (int p = P->wrp[i]; p < P->wrp[i + 1]; ++p) clash |= P->wci[p] != i && P->wci[p] >= start;
+        if (clash) {
+          runs.push_back(i);
+          start = i;
+        }
+      }
+      runs.push_back(n);
+      colors = runs.data();
+      ncolors = (int)runs.size() - 1;
+    }
+  }
   sb::GSArgs a;
   int histCap = 0, maxColor = 0;
   if ((rc = gs_prepare(ctx, P, o, colors, ncolors, r, on_device, st, a, histCap, maxColor))) return rc;
@@ -1239,6 +1273,163 @@ int sb_pgs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, cons
   a.gs = make_sync(ctx, grid);
   void* args[] = {&a, &e};
   return gs_run(ctx, P, fn, grid, threads, args, r, on_device, residual, st, histCap);
+}
+
+// ------------------------------------------------------------------ Cadoux fixed point
+// friction_problem::solveCadoux / solveCadouxVel (src/Interfaces/Cadoux.hpp:44-155,
+// FrictionProblem.cpp:68-100): SOC solves whose affine term carries the De Saxce
+// correction of the previous iterate. Everything stays on the device; the problem's
+// b is swapped for each inner problem's and restored afterwards.
+static int cadoux_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* po, const sb_gs_options* go,
+                      const int64_t* colors, int ncolors, int iterations, double tighten, int velocity, double* x,
+                      int on_device, double* residual, sb_stats* st) {
+  const bool gs = go != nullptr;
+  const int n = P->nc;
+  const size_t n3 = 3 * (size_t)n;
+  hipStream_t s = ctx->stream;
+  const int op = gs ? SB_OP_ASSEMBLED : po->op;
+  if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  if (iterations < 0 || !(tighten > 0.0)) return SB_ERR_INVALID;
+  // the minimizer's own scaling, as its eval() uses it
+  const bool useScal = gs || po->scaling == SB_SCALING_DIAGONAL ||
+                       (po->scaling == SB_SCALING_REFERENCE && op == SB_OP_ASSEMBLED);
+  const int infNorm = gs ? go->inf_norm : po->inf_norm;
+  const double tol = gs ? go->tol : po->tol;
+  for (auto& b : P->cad) SB_HIP(b.alloc(n3));
+  double *b0 = P->cad[0].p, *cx = P->cad[1].p, *cs = P->cad[2].p, *cw = P->cad[3].p, *cu = P->cad[4].p;
+  SB_HIP(hipMemcpyAsync(b0, P->d_b.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  int rc;
+  if ((rc = copy_in(ctx, cx, x, n3, on_device))) return rc;
+  const DevProblem<double> V = P->view(true);
+  int spmvs = 0;
+  auto applyW = [&](const double* in, double* out) -> hipError_t {
+    ++spmvs;
+    return op == SB_OP_ASSEMBLED ? sb::launch_w(V, in, out, P->carry.p, s)
+                                 : sb::launch_matfree(V, in, out, P->wbody.p, P->carry.p, s);
+  };
+  // FB residual of (xa, W(.) + b0), the minimizer's norm and scaling
+  std::vector<double> terms(n);
+  auto evalRes = [&](bool coulomb, const double* xa, const double* wy, double& res) -> int {
+    SB_HIP(sb::launch_fb_terms(n, coulomb ? 1 : 0, P->d_mu.p, useScal ? P->d_scal.p : nullptr, wy, b0, xa,
+                               P->terms.p, s));
+    SB_HIP(hipMemcpyAsync(terms.data(), P->terms.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    SB_HIP(hipStreamSynchronize(s));
+    double acc = 0.0;
+    if (infNorm) {
+      for (double v : terms) acc = std::max(acc, v);
+    } else {
+      for (double v : terms) acc += v;
+      acc /= (1.0 + n);
+    }
+    res = acc;
+    return std::isfinite(acc) ? SB_OK : SB_ERR_NAN;
+  };
+  // the inner SOC solve of xi at tolerance t, with d_b already set; b restored after
+  auto inner = [&](double* xi, double t) -> int {
+    sb_stats is;
+    std::memset(&is, 0, sizeof(is));
+    double r = 0.0;
+    int irc;
+    if (gs) {
+      sb_gs_options g = *go;
+      g.law = SB_LAW_SOC;
+      g.tol = t;
+      irc = sb_gs_solve(ctx, P, &g, colors, ncolors, xi, 1, &r, &is);
+    } else {
+      sb_pg_options g = *po;
+      g.law = SB_LAW_SOC;
+      g.tol = t;
+      irc = sb_pg_solve(ctx, P, &g, xi, 1, &r, &is);
+    }
+    spmvs += is.spmvs + is.refine_spmvs;
+    const hipError_t e = hipMemcpyAsync(P->d_b.p, b0, n3 * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (irc < 0) return irc;
+    return e == hipSuccess ? SB_OK : SB_ERR_HIP;
+  };
+  std::vector<double> hist;
+  double res = 0.0;
+  int it = 0;
+  SB_HIP(hipEventRecord(ctx->ev0, s));
+  if (!velocity) {
+    // s = W r + b; res = eval(Coulomb, s, r)
+    SB_HIP(applyW(cx, cw));
+    if ((rc = evalRes(true, cx, cw, res))) return rc;
+    hist.push_back(res);
+    while (it < iterations) {
+      const double t = tighten * std::max(tol, std::min(res, 1.0));
+      SB_HIP(sb::launch_cadoux_rhs(n, P->d_mu.p, cw, b0, P->d_b.p, s));  // b + s(W r + b)
+      if ((rc = inner(cx, t))) return rc;
+      SB_HIP(applyW(cx, cw));
+      if ((rc = evalRes(true, cx, cw, res))) return rc;
+      hist.push_back(res);
+      ++it;
+      if (res < tol) break;
+    }
+  } else {
+    // x holds u; ustar = u + s(u), s_N = |u_T| max(0, 1/mu); res = eval(SOC, W u + b, ustar)
+    SB_HIP(hipMemcpyAsync(cu, cx, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SB_HIP(sb::launch_vel_shift(n, P->d_mu.p, cu, cs, s));
+    SB_HIP(sb::launch_lincomb(n3, 1.0, cu, 1.0, cs, cx, s));
+    SB_HIP(applyW(cu, cw));
+    if ((rc = evalRes(false, cx, cw, res))) return rc;
+    hist.push_back(res);
+    while (it < iterations) {
+      const double t = tighten * std::max(tol, std::min(res, 1.0));
+      SB_HIP(applyW(cs, cw));
+      SB_HIP(sb::launch_lincomb(n3, 1.0, b0, -1.0, cw, P->d_b.p, s));  // b - W s
+      if ((rc = inner(cx, t))) return rc;
+      SB_HIP(sb::launch_lincomb(n3, 1.0, cx, -1.0, cs, cu, s));  // u = ustar - s
+      SB_HIP(sb::launch_vel_shift(n, P->d_mu.p, cx, cs, s));
+      SB_HIP(sb::launch_lincomb(n3, 1.0, cu, 1.0, cs, cx, s));  // ustar = u + s
+      SB_HIP(applyW(cu, cw));
+      if ((rc = evalRes(false, cx, cw, res))) return rc;
+      hist.push_back(res);
+      ++it;
+      if (it > 1 && res < tol) break;
+    }
+  }
+  SB_HIP(hipEventRecord(ctx->ev1, s));
+  if ((rc = copy_out(ctx, x, velocity ? cu : cx, n3, on_device))) return rc;
+  SB_HIP(hipStreamSynchronize(s));
+  const bool conv = res < tol;
+  if (st) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    st->iterations = it;
+    st->spmvs = spmvs;
+    st->ls_trials = 0;
+    st->nesterov_steps = 0;
+    st->converged = conv ? 1 : 0;
+    st->local_failures = 0;
+    st->residual = res;
+    st->device_ms = ms;
+    st->history_length = (int)hist.size();
+    if (st->history && st->history_capacity > 0)
+      for (int k = 0; k < st->history_capacity && k < (int)hist.size(); ++k) st->history[k] = hist[k];
+  }
+  if (residual) *residual = res;
+  return conv ? SB_OK : SB_NOT_CONVERGED;
+}
+
+int sb_pg_solve_cadoux(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, int iterations,
+                       double tol_tighten, int velocity, double* x, int on_device, double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !x) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if ((rc = check_pg_options(o, P->hasW))) return rc;
+  return cadoux_run(ctx, P, o, nullptr, nullptr, 0, iterations, tol_tighten, velocity, x, on_device, residual, st);
+}
+
+int sb_gs_solve_cadoux(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const int64_t* colors,
+                       int ncolors, int iterations, double tol_tighten, int velocity, double* x, int on_device,
+                       double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !x) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  return cadoux_run(ctx, P, nullptr, o, colors, ncolors, iterations, tol_tighten, velocity, x, on_device, residual,
+                    st);
 }
 
 // ------------------------------------------------------------------ memory helpers

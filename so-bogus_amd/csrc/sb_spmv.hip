@@ -8,6 +8,8 @@
 // atomics, deterministic results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sb_law.hpp"
 #include "sb_problem.hpp"
 #include "sb_spmv.hpp"
@@ -225,6 +227,56 @@ hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* 
                            const double* b, const double* r, double* terms, hipStream_t s) {
   (void)hipGetLastError();  // clear a stale error: report this launch only
   if (nc > 0) k_fb_terms<<<(nc + 255) / 256, 256, 0, s>>>(nc, coulomb, mu, scal, u, b, r, terms);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Cadoux fixed point
+// bout = b0 + s(Wx + b0): the affine term of the next SOC problem, s the De Saxce
+// correction (mu |u_T|, 0, 0) of u = W x + b0 (Cadoux.hpp:72-76).
+__global__ void __launch_bounds__(256) k_cadoux_rhs(int nc, const double* __restrict__ mu,
+                                                    const double* __restrict__ wx, const double* __restrict__ b0,
+                                                    double* __restrict__ bout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const double u1 = wx[3 * i + 1] + b0[3 * i + 1], u2 = wx[3 * i + 2] + b0[3 * i + 2];
+  bout[3 * i] = b0[3 * i] + cov_normal<double>(true, mu[i], u1, u2);
+  bout[3 * i + 1] = b0[3 * i + 1];
+  bout[3 * i + 2] = b0[3 * i + 2];
+}
+// s = (|v_T| max(0, 1/mu), 0, 0) (Cadoux.hpp:119-123, 139-143)
+__global__ void __launch_bounds__(256) k_vel_shift(int nc, const double* __restrict__ mu, const double* __restrict__ v,
+                                                   double* __restrict__ sv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  sv[3 * i] = sqrt(v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]) * fmax(0.0, 1.0 / mu[i]);
+  sv[3 * i + 1] = 0.0;
+  sv[3 * i + 2] = 0.0;
+}
+// out = a x + c y
+__global__ void __launch_bounds__(256) k_lincomb(size_t n, double a, const double* __restrict__ x, double c,
+                                                 const double* __restrict__ y, double* __restrict__ out) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    out[k] = a * x[k] + c * y[k];
+}
+
+hipError_t launch_cadoux_rhs(int nc, const double* mu, const double* wx, const double* b0, double* bout,
+                             hipStream_t s) {
+  (void)hipGetLastError();
+  if (nc > 0) k_cadoux_rhs<<<(nc + 255) / 256, 256, 0, s>>>(nc, mu, wx, b0, bout);
+  return hipGetLastError();
+}
+hipError_t launch_vel_shift(int nc, const double* mu, const double* v, double* sv, hipStream_t s) {
+  (void)hipGetLastError();
+  if (nc > 0) k_vel_shift<<<(nc + 255) / 256, 256, 0, s>>>(nc, mu, v, sv);
+  return hipGetLastError();
+}
+hipError_t launch_lincomb(size_t n, double a, const double* x, double c, const double* y, double* out,
+                          hipStream_t s) {
+  (void)hipGetLastError();
+  if (n > 0) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    k_lincomb<<<grid, 256, 0, s>>>(n, a, x, c, y, out);
+  }
   return hipGetLastError();
 }
 

@@ -13,6 +13,11 @@ hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, dou
 hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* w, double* carry, hipStream_t s);
 hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* y, double* wbody, double* carry,
                           hipStream_t s);
+hipError_t launch_cadoux_rhs(int nc, const double* mu, const double* wx, const double* b0, double* bout,
+                             hipStream_t s);
+hipError_t launch_vel_shift(int nc, const double* mu, const double* v, double* sv, hipStream_t s);
+hipError_t launch_lincomb(size_t n, double a, const double* x, double c, const double* y, double* out,
+                          hipStream_t s);
 hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* scal, const double* u,
                            const double* b, const double* r, double* terms, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -444,6 +444,26 @@ struct DualFrictionProblem<3> {
     return res;
   }
 
+  // solveCadoux / solveCadouxVel (FrictionProblem.hpp:177-195, Cadoux.hpp:44-155): fixed-point
+  // iterations of SOC solves with the given solver; the callback receives the residual
+  // before and after each iteration.
+  double solveCadoux(GaussSeidelType& gs, double* r, const unsigned fpIterations,
+                     const SignalType* callback = nullptr) const {
+    return cadoux(&gs, nullptr, r, fpIterations, callback, 0);
+  }
+  double solveCadoux(ProjectedGradientType& pg, double* r, const unsigned fpIterations,
+                     const SignalType* callback = nullptr) const {
+    return cadoux(nullptr, &pg, r, fpIterations, callback, 0);
+  }
+  double solveCadouxVel(GaussSeidelType& gs, double* u, const unsigned fpIterations,
+                        const SignalType* callback = nullptr) const {
+    return cadoux(&gs, nullptr, u, fpIterations, callback, 1);
+  }
+  double solveCadouxVel(ProjectedGradientType& pg, double* u, const unsigned fpIterations,
+                        const SignalType* callback = nullptr) const {
+    return cadoux(nullptr, &pg, u, fpIterations, callback, 1);
+  }
+
   double evalWith(const GaussSeidelType&, const double* r, const bool staticProblem = false) const {
     return eval(r, staticProblem, SB_OP_ASSEMBLED, SB_SCALING_REFERENCE);
   }
@@ -475,6 +495,38 @@ struct DualFrictionProblem<3> {
     permuteDevice(m_inv);
     m_perm.clear();
     m_inv.clear();
+  }
+  double cadoux(GaussSeidelType* gs, ProjectedGradientType* pg, double* x, unsigned fpIterations,
+                const SignalType* callback, int velocity) const {
+    const int nc = (int)mu.size();
+    std::vector<double> xp(x, x + 3 * (size_t)nc);
+    if (permuted()) permuteVector(m_perm, xp);
+    std::vector<double> hist((size_t)fpIterations + 1);
+    sb_stats st{};
+    st.history = hist.data();
+    st.history_capacity = (int)hist.size();
+    double res = 0;
+    const double tighten = 1e-1;  // Cadoux.hpp: tolTighten default
+    if (gs) {
+      const sb_gs_options o = gs->options(SB_LAW_SOC);
+      std::vector<int64_t> colors;
+      if (gs->coloring().colors.size() >= 2 && (size_t)gs->coloring().colors.back() == (size_t)nc)
+        colors.assign(gs->coloring().colors.begin(), gs->coloring().colors.end());
+      sb_check(sb_gs_solve_cadoux(m_ctx.get(), m_P.get(), &o, colors.empty() ? nullptr : colors.data(),
+                                  colors.empty() ? 0 : (int)colors.size() - 1, (int)fpIterations, tighten, velocity,
+                                  xp.data(), 0, &res, &st),
+               "sb_gs_solve_cadoux");
+    } else {
+      const sb_pg_options o = pg->options(SB_LAW_SOC);
+      sb_check(sb_pg_solve_cadoux(m_ctx.get(), m_P.get(), &o, (int)fpIterations, tighten, velocity, xp.data(), 0,
+                                  &res, &st),
+               "sb_pg_solve_cadoux");
+    }
+    if (callback)
+      for (int k = 0; k < st.history_length && k < (int)hist.size(); ++k) callback->trigger((unsigned)k, hist[k]);
+    if (permuted()) permuteVector(m_inv, xp);
+    std::copy(xp.begin(), xp.end(), x);
+    return res;
   }
   bool permuted() const { return !m_perm.empty(); }
   const std::vector<std::size_t>& permutation() const { return m_perm; }

@@ -39,7 +39,7 @@ EXPORTED = [
     "sb_ctx_info", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
-    "sb_gs_solve", "sb_pgs_solve", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
+    "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
     "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_memset_device", "sb_adjoint_options_default",
     "sb_adjoint", "sb_problem_update",
@@ -146,6 +146,10 @@ def lib():
                                   C.POINTER(Stats)]),
         "sb_pgs_solve": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, _vp, C.c_int, _dp,
                                    C.POINTER(Stats)]),
+        "sb_pg_solve_cadoux": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), C.c_int, C.c_double, C.c_int, _vp,
+                                         C.c_int, _dp, C.POINTER(Stats)]),
+        "sb_gs_solve_cadoux": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, C.c_int, C.c_double,
+                                         C.c_int, _vp, C.c_int, _dp, C.POINTER(Stats)]),
         "sb_device_alloc": (C.c_int, [_vp, C.POINTER(_vp), C.c_size_t]),
         "sb_device_free": (C.c_int, [_vp, _vp]),
         "sb_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
@@ -458,6 +462,28 @@ class DualFrictionProblem:
             colors = _i64(colors)
             cp, nco = _l(colors), len(colors) - 1
         fn = lambda rp, dev, res, st: lib().sb_pgs_solve(self.ctx.h, self.h, C.byref(o), cp, nco, rp, dev, res, st)
+        return self._run(fn, r0, on_device, hist_cap)
+
+    def solve_cadoux(self, r0, solver="pg", opts=None, colors=None, iterations=10, tol_tighten=0.1,
+                     velocity=False, on_device=False, hist_cap=0, **kw) -> SolveResult:
+        """DualFrictionProblem::solveCadoux (velocity=False) / solveCadouxVel (True) with the
+        ProjectedGradient (solver="pg") or colored GaussSeidel ("gs") inner solver. The
+        result's iterations are fixed-point iterations and its history the residual
+        before and after each."""
+        vel = int(bool(velocity))
+        if solver == "gs":
+            o = opts if opts is not None else gs_options(**kw)
+            if colors is None:
+                cp, nco = None, 0
+            else:
+                colors = _i64(colors)
+                cp, nco = _l(colors), len(colors) - 1
+            fn = lambda rp, dev, res, st: lib().sb_gs_solve_cadoux(self.ctx.h, self.h, C.byref(o), cp, nco,
+                                                                   iterations, tol_tighten, vel, rp, dev, res, st)
+        else:
+            o = opts if opts is not None else pg_options(**kw)
+            fn = lambda rp, dev, res, st: lib().sb_pg_solve_cadoux(self.ctx.h, self.h, C.byref(o), iterations,
+                                                                   tol_tighten, vel, rp, dev, res, st)
         return self._run(fn, r0, on_device, hist_cap)
 
     def update(self, b=None, mu=None, minv=None):

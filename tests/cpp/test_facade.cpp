@@ -149,6 +149,37 @@ static void testProductGaussSeidel() {
   CHECK(isApprox(x, y, 1.e-10));
 }
 
+// DualFrictionProblem::solveCadoux / solveCadouxVel (FrictionProblem.hpp:177-195) on the
+// two-contact fixture: the fixed point is the Coulomb solution `sol`.
+static void testCadoux() {
+  SmallFrictionPb pb;
+  bogus::DualFrictionProblem<3> dual;
+  dual.computeFrom(pb.primal);
+  bogus::GaussSeidel gs;
+  gs.setTol(1.e-8);
+  bogus::ProjectedGradient pg;
+  pg.setTol(1.e-8);
+  pg.setMaxIters(1000);
+  unsigned ncb = 0;
+  bogus::Signal<unsigned, double> cb;
+  cb.connect([&](unsigned, double) { ++ncb; });
+  std::vector<double> x(6, 0.0);
+  double res = dual.solveCadoux(gs, x.data(), 50, &cb);
+  std::printf("Cadoux GS: residual %.3e\n", res);
+  CHECK(res < 1.e-8);
+  CHECK(isApprox(pb.sol, x, 1.e-4));
+  CHECK(ncb >= 2);
+  std::fill(x.begin(), x.end(), 0.0);
+  res = dual.solveCadoux(pg, x.data(), 50);
+  std::printf("Cadoux PG: residual %.3e\n", res);
+  CHECK(res < 1.e-8);
+  CHECK(isApprox(pb.sol, x, 1.e-4));
+  std::vector<double> u(6, 0.0);
+  res = dual.solveCadouxVel(gs, u.data(), 50);
+  std::printf("CadouxVel GS: residual %.3e\n", res);
+  CHECK(res < 1.e-8);
+}
+
 static void testBlockMV() {
   // tests/BlockMV.cpp:14-55 SmallSBM
   bogus::SparseBlockMatrix sbm;
@@ -197,6 +228,7 @@ int main() {
     testGaussSeidel();
     testProjectedGradient();
     testProductGaussSeidel();
+    testCadoux();
   } catch (const std::exception& e) {
     std::printf("EXCEPTION %s\n", e.what());
     return 2;

@@ -225,3 +225,27 @@ def test_product_gs_rejects_coupled_color(R, G, ctx):
     dp = R.DualFrictionProblem(ctx, prob, assemble=False)
     with pytest.raises(RuntimeError):
         dp.solve_pgs(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, colors=np.array([0, prob.nc]))
+
+
+def test_gs_natural_order(R, O, G, ctx):
+    """Assembled-W GS without colors: the contacts are swept in their own order, in
+    runs of uncoupled contacts (the reference's sequential GaussSeidel)."""
+    print("case: 1k GS, natural order", flush=True)
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    g = dp.solve_gs(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, tol=1e-8, max_iters=1000, hist_cap=100)
+    op = O.OracleProblem(prob)
+    runs = _natural_runs(prob)
+    ores, ox, oinfo = op.gs_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=1000, coulomb=True,
+                                  ndx_ref_per_color=True, colors=runs)
+    print(f"   {len(runs) - 1} runs; gpu {g.iterations} it; oracle (runs) {oinfo['iterations']}", flush=True)
+    assert g.iterations == oinfo["iterations"]
+    assert np.allclose(g.history, oinfo["history"][:len(g.history)], rtol=1e-6, atol=1e-14)
+    assert np.linalg.norm(g.r - ox) <= 1e-6 * np.linalg.norm(ox)
+
+
+def test_gs_rejects_coupled_color(R, G, ctx):
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    with pytest.raises(RuntimeError):
+        dp.solve_gs(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, colors=np.array([0, prob.nc]))

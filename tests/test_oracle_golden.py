@@ -114,6 +114,42 @@ def test_pgs_equals_gs_on_w(G, O, coulomb):
     assert np.linalg.norm(x1 - x2) <= 1e-12 * np.linalg.norm(x1)
 
 
+# ---------------------------------------------------------------- Cadoux.hpp:44-155 (no reference test)
+@pytest.mark.parametrize("use_gs", [True, False])
+def test_small_cadoux_reaches_coulomb_solution(O, small, use_gs):
+    """The fixed point of the SOC solves with the De Saxce-corrected affine term is the
+    Coulomb solution: on the two-contact fixture it reaches the known Coulomb `sol`
+    (tests/GaussSeidel.cpp:38-64) to 1e-4, and the returned residual is the oracle's
+    independent Coulomb evaluation of the result."""
+    prob, sol, op = small
+    res, x, d = op.cadoux_solve(np.zeros(6), use_gs=use_gs, tol=1e-8, iterations=50)
+    assert res < 1e-8 and d["iterations"] < 50
+    assert _is_approx(sol, x, 1e-4)
+    assert op.eval(x, coulomb=True) == pytest.approx(res, rel=1e-12, abs=1e-300)
+    assert np.all(np.diff(d["history"][:3]) < 0)
+
+
+@pytest.mark.parametrize("use_gs", [True, False])
+def test_small_cadoux_velocity_form(O, small, use_gs):
+    """solveCadouxVel: the problem read as r = W u + b. The returned residual equals an
+    independent numpy evaluation from u: u* = u + (|u_T| / mu, 0, 0), r = W u + b
+    (dense W), SOC Fischer-Burmeister of (s u*, r) averaged over n + 1."""
+    prob, sol, op = small
+    res, u, d = op.cadoux_solve(np.zeros(6), use_gs=use_gs, tol=1e-8, iterations=100, velocity=True)
+    assert res < 1e-8
+    W = prob.dense_w()
+    ustar = u.copy()
+    for i in range(2):
+        ustar[3 * i] += np.hypot(u[3 * i + 1], u[3 * i + 2]) / prob.mu[i]
+    r = W @ u + prob.b
+    scal = [max(1.0, np.trace(W[3 * i:3 * i + 3, 3 * i:3 * i + 3]) / 3) for i in range(2)]
+    acc = 0.0
+    for i in range(2):
+        fb = O.fb(prob.mu[i], scal[i] * ustar[3 * i:3 * i + 3], r[3 * i:3 * i + 3], coulomb=False)
+        acc += float(np.dot(fb, fb))
+    assert acc / 3 == pytest.approx(res, rel=1e-9, abs=1e-20)
+
+
 # ---------------------------------------------------------------- tests/ProjectedGradient.cpp:15-83
 @pytest.mark.parametrize("variant", ["PG_APGD", "PG_DESCENT", "PG_STANDARD", "PG_CONJUGATED", "PG_SPG"])
 def test_small_pg_soc(O, small, variant):
