@@ -73,7 +73,10 @@ enum sb_precision {
                              the FP32 result verifies the tolerance and refines if needed */
 };
 /* Problem creation flags */
-enum sb_problem_flags { SB_PROBLEM_ASSEMBLE_W = 1 };
+enum sb_problem_flags {
+  SB_PROBLEM_ASSEMBLE_W = 1,
+  SB_BATCH_CALLER_ORDER = 2  /* sb_batch_create: store each problem in the caller's contact and body order */
+};
 /* BSR flags (bogus::flags) */
 enum sb_bsr_flags { SB_BSR_SYMMETRIC = 1 /* only blocks with col <= row are given */ };
 /* Coloring modes */
@@ -302,7 +305,10 @@ typedef struct sb_batch_stats {
                              FP32 stage (0 in FP64 precision) and of the FP64 stage */
 } sb_batch_stats;
 
-/* Problems k = 0..n_problems-1 as in sb_problem_create (flags: SB_PROBLEM_ASSEMBLE_W). */
+/* Problems k = 0..n_problems-1 as in sb_problem_create (flags: SB_PROBLEM_ASSEMBLE_W). Each
+   problem is stored in a locality order (bodies renumbered by first touch, contacts sorted
+   by their bodies: the operator's gathers touch fewer cache lines) unless
+   SB_BATCH_CALLER_ORDER is set; r and every per-problem result stay in the caller's order. */
 int sb_batch_create(sb_ctx* ctx, sb_batch** B, int n_problems, const int* n_contacts, const int* n_bodies,
                     const int32_t* const* h_rowptr, const int32_t* const* h_colidx, const double* const* h_vals,
                     const double* const* minv, const double* const* b, const double* const* mu, int flags);
@@ -315,7 +321,7 @@ int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, const sb
 /* Autotuner: times every launch geometry the kernels allow on this batch (each
    capped at probe_iters iterations per problem, r0 = 0), records each problem's
    iteration count in one full solve (long-running problems then get larger
-   teams), times the best geometry of each team sizing on full solves, and keeps
+   teams), times the two best geometries of each team sizing on full solves, and keeps
    the fastest for later solves of this batch. Solves are reproducible between
    calls to this function: only it changes the schedule. */
 int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, int probe_iters, sb_batch_config* best,

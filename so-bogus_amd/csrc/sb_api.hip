@@ -1721,6 +1721,11 @@ struct sb_batch {
   DBuf<double> teamPartials;  // [teams][2][kMaxTeamBlocks][kMaxPartials]
   DBuf<int> teamSlot;         // [teams]
   DBuf<int> error;
+  // locality order: gperm[internal contact] = caller's contact (global indices); stage
+  // holds r in the caller's order when it comes from host memory
+  bool reordered = false;
+  DBuf<int> gperm;
+  DBuf<double> stage;
   int maxTeams = 0;
   int descScaling = -1;       // scaling mode the descriptor arrays were built for
   bool tuned = false;
@@ -2026,7 +2031,14 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
   hipStream_t s = ctx->stream;
   SB_HIP(hipMemcpyAsync(B->d_order.p, order.data(), sizeof(int) * B->n, hipMemcpyHostToDevice, s));
   const size_t n3 = 3 * (size_t)B->totalNc;
-  if (r) {
+  if (r && B->reordered) {
+    const double* src = r;
+    if (!on_device) {
+      if ((rc = copy_in(ctx, B->stage.p, r, n3, 0))) return rc;
+      src = B->stage.p;
+    }
+    SB_HIP(sb::launch_permute3(B->totalNc, B->gperm.p, src, B->xio.p, false, s));
+  } else if (r) {
     if ((rc = copy_in(ctx, B->xio.p, r, n3, on_device))) return rc;
   } else {
     SB_HIP(hipMemsetAsync(B->xio.p, 0, n3 * sizeof(double), s));
@@ -2038,7 +2050,12 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
   int err = 0;
   SB_HIP(hipMemcpyAsync(outs.data(), B->outs.p, sizeof(double) * outs.size(), hipMemcpyDeviceToHost, s));
   SB_HIP(hipMemcpyAsync(&err, B->error.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  if (r && (rc = copy_out(ctx, r, B->xio.p, n3, on_device))) return rc;
+  if (r && B->reordered) {
+    SB_HIP(sb::launch_permute3(B->totalNc, B->gperm.p, B->xio.p, on_device ? r : B->stage.p, true, s));
+    if (!on_device && (rc = copy_out(ctx, r, B->stage.p, n3, 0))) return rc;
+  } else if (r && (rc = copy_out(ctx, r, B->xio.p, n3, on_device))) {
+    return rc;
+  }
   SB_HIP(hipStreamSynchronize(s));
   int nconv = 0, totalIt = 0, maxIt = 0, refined = 0, failed = 0;
   int64_t spmvs = 0;
@@ -2101,9 +2118,49 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
   B->off.resize(n);
   B->cost.resize(n);
   B->iterHint.assign(n, 0);
+  B->reordered = (flags & SB_BATCH_CALLER_ORDER) == 0;
+  const int pflags = flags & SB_PROBLEM_ASSEMBLE_W;
+  std::vector<int> gperm;
   for (int k = 0; k < n; ++k) {
-    if ((rc = create_problem(ctx, B->probs[k], ncs[k], nbs[k], rps[k], cis[k], hvs[k], minvs[k], bs[k], mus[k], flags)))
-      return rc;
+    if (!B->reordered) {
+      if ((rc = create_problem(ctx, B->probs[k], ncs[k], nbs[k], rps[k], cis[k], hvs[k], minvs[k], bs[k], mus[k],
+                               pflags)))
+        return rc;
+    } else {
+      const int nc = ncs[k], nb = nbs[k];
+      if (nc <= 0 || nb <= 0 || !rps[k] || !cis[k] || !hvs[k] || !minvs[k] || !bs[k] || !mus[k] || rps[k][0] != 0)
+        return SB_ERR_INVALID;
+      for (int i = 0; i < nc; ++i)
+        if (rps[k][i + 1] < rps[k][i]) return SB_ERR_INVALID;
+      for (int p = 0; p < rps[k][nc]; ++p)
+        if (cis[k][p] < 0 || cis[k][p] >= nb) return SB_ERR_INVALID;
+      std::vector<int> cperm, bmap;
+      sbhost::localityOrder(nc, nb, rps[k], cis[k], cperm, bmap);
+      std::vector<int32_t> rp(nc + 1, 0), ci;
+      std::vector<double> hv, mi((size_t)36 * nb), b((size_t)3 * nc), mu(nc);
+      ci.reserve(rps[k][nc]);
+      hv.reserve((size_t)18 * rps[k][nc]);
+      std::vector<std::pair<int, int>> row;
+      for (int i = 0; i < nc; ++i) {
+        const int o = cperm[i];
+        row.clear();
+        for (int p = rps[k][o]; p < rps[k][o + 1]; ++p) row.push_back({bmap[cis[k][p]], p});
+        std::sort(row.begin(), row.end());
+        for (auto& e : row) {
+          ci.push_back(e.first);
+          hv.insert(hv.end(), hvs[k] + (size_t)18 * e.second, hvs[k] + (size_t)18 * e.second + 18);
+        }
+        rp[i + 1] = (int32_t)ci.size();
+        for (int c = 0; c < 3; ++c) b[3 * (size_t)i + c] = bs[k][3 * (size_t)o + c];
+        mu[i] = mus[k][o];
+        gperm.push_back((int)(B->totalNc + o));
+      }
+      for (int j = 0; j < nb; ++j)
+        std::memcpy(&mi[(size_t)36 * bmap[j]], minvs[k] + (size_t)36 * j, 36 * sizeof(double));
+      if ((rc = create_problem(ctx, B->probs[k], nc, nb, rp.data(), ci.data(), hv.data(), mi.data(), b.data(),
+                               mu.data(), pflags)))
+        return rc;
+    }
     B->off[k] = 3 * B->totalNc;
     B->totalNc += ncs[k];
     const sb_problem* P = B->probs[k].get();
@@ -2116,6 +2173,10 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
   SB_HIP(B->error.alloc(1));
   SB_HIP(B->d32.alloc(n));
   SB_HIP(B->d64.alloc(n));
+  if (B->reordered) {
+    SB_HIP(B->gperm.upload(gperm, s));
+    SB_HIP(B->stage.alloc(3 * (size_t)B->totalNc));
+  }
   SB_HIP(hipStreamSynchronize(s));
   *out = B.release();
   return SB_OK;
@@ -2193,7 +2254,8 @@ int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int prob
   if ((rc = timeRun(o, screened[0].second, true, ms0))) return rc;
   double bestMs = 1e300;
   sb_batch_config bestCfg = screened[0].second;
-  for (int cpb : cpbList) {  // the best screened shape of each team sizing
+  for (int cpb : cpbList) {  // the two best screened shapes of each team sizing
+    int taken = 0;
     for (const auto& sc : screened) {
       if (sc.second.contacts_per_block != cpb) continue;
       double ms;
@@ -2202,7 +2264,7 @@ int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int prob
         bestMs = ms;
         bestCfg = sc.second;
       }
-      break;
+      if (++taken == 2) break;
     }
   }
   B->tuned = true;

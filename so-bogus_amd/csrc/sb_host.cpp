@@ -348,4 +348,39 @@ void coloringFromH(int nc, int nb, const int* hrowptr, const int* hcolidx, int m
   }
 }
 
+void localityOrder(int nc, int nb, const int* hrowptr, const int* hcolidx, std::vector<int>& cperm,
+                   std::vector<int>& bmap) {
+  const int none = std::numeric_limits<int>::max();
+  auto keys = [&](const std::vector<int>& map, std::vector<std::pair<int, int>>& k) {
+    k.resize(nc);
+    for (int i = 0; i < nc; ++i) {
+      int lo = none, hi = none;
+      for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p) {
+        const int j = map.empty() ? hcolidx[p] : map[hcolidx[p]];
+        lo = (lo == none) ? j : std::min(lo, j);
+        hi = (hi == none) ? j : std::max(hi, j);
+      }
+      k[i] = {lo, hi};
+    }
+  };
+  auto order = [&](const std::vector<std::pair<int, int>>& k, std::vector<int>& perm) {
+    perm.resize(nc);
+    for (int i = 0; i < nc; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return k[a] < k[b]; });
+  };
+  std::vector<std::pair<int, int>> k;
+  std::vector<int> perm0;
+  keys({}, k);
+  order(k, perm0);
+  bmap.assign(nb, -1);
+  int next = 0;
+  for (int i : perm0)
+    for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p)
+      if (bmap[hcolidx[p]] < 0) bmap[hcolidx[p]] = next++;
+  for (int j = 0; j < nb; ++j)
+    if (bmap[j] < 0) bmap[j] = next++;
+  keys(bmap, k);
+  order(k, cperm);
+}
+
 }  // namespace sbhost

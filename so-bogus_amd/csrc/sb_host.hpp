@@ -48,4 +48,12 @@ void coloring(int n, const int* wrowptr, const int* wcolidx, int mode, std::vect
 void coloringFromH(int nc, int nb, const int* hrowptr, const int* hcolidx, int mode, std::vector<int64_t>& perm,
                    std::vector<int64_t>& colors);
 
+// Locality order of a contact problem: bodies renumbered by first touch along the
+// contacts sorted by (lowest body, highest body), then the contacts sorted by their
+// renumbered (lowest, highest) body; contacts without bodies go last. cperm[new] = old
+// contact, bmap[old] = new body. Consecutive contacts then share or neighbour their
+// first body, so the gathers of both operator passes touch fewer cache lines.
+void localityOrder(int nc, int nb, const int* hrowptr, const int* hcolidx, std::vector<int>& cperm,
+                   std::vector<int>& bmap);
+
 }  // namespace sbhost

@@ -259,6 +259,28 @@ __global__ void __launch_bounds__(256) k_lincomb(size_t n, double a, const doubl
     out[k] = a * x[k] + c * y[k];
 }
 
+// Contact vectors between two orders: gather dst[i] = src[perm[i]] or scatter
+// dst[perm[i]] = src[i], 3 doubles per contact.
+__global__ void __launch_bounds__(256) k_permute3(int64_t n, const int* __restrict__ perm,
+                                                  const double* __restrict__ src, double* __restrict__ dst,
+                                                  int scatter) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = perm[i];
+    const int64_t from = scatter ? i : j, to = scatter ? j : i;
+    dst[3 * to] = src[3 * from];
+    dst[3 * to + 1] = src[3 * from + 1];
+    dst[3 * to + 2] = src[3 * from + 2];
+  }
+}
+hipError_t launch_permute3(int64_t n, const int* perm, const double* src, double* dst, bool scatter, hipStream_t s) {
+  (void)hipGetLastError();
+  if (n > 0) {
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    k_permute3<<<grid, 256, 0, s>>>(n, perm, src, dst, scatter ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_cadoux_rhs(int nc, const double* mu, const double* wx, const double* b0, double* bout,
                              hipStream_t s) {
   (void)hipGetLastError();

@@ -13,6 +13,7 @@ hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, dou
 hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* w, double* carry, hipStream_t s);
 hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* y, double* wbody, double* carry,
                           hipStream_t s);
+hipError_t launch_permute3(int64_t n, const int* perm, const double* src, double* dst, bool scatter, hipStream_t s);
 hipError_t launch_cadoux_rhs(int nc, const double* mu, const double* wx, const double* b0, double* bout,
                              hipStream_t s);
 hipError_t launch_vel_shift(int nc, const double* mu, const double* v, double* sv, hipStream_t s);

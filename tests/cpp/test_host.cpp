@@ -371,7 +371,89 @@ static void testColoringFromH() {
   }
 }
 
+// The locality order is a pair of permutations under which H M^-1 H^T is the same
+// operator: (P W P^T)(P x) = P (W x), checked exactly on integer data.
+static void testLocalityOrder() {
+  std::mt19937 g(5);
+  const int nb = 400, nc = 1500;
+  std::uniform_int_distribution<int> body(0, nb - 1), kind(0, 29), val(-3, 3);
+  std::vector<int> rp{0}, ci;
+  std::vector<double> hv;
+  for (int i = 0; i < nc; ++i) {
+    const int k = kind(g);
+    std::vector<int> bs;
+    if (k == 1) bs = {body(g)};
+    if (k > 1) {
+      int a = body(g), b = body(g);
+      while (b == a) b = body(g);
+      bs = {std::min(a, b), std::max(a, b)};
+    }
+    for (int j : bs) {
+      ci.push_back(j);
+      for (int e = 0; e < 18; ++e) hv.push_back(val(g));
+    }
+    rp.push_back((int)ci.size());
+  }
+  std::vector<double> minv((size_t)nb * 36);
+  for (auto& v : minv) v = val(g);
+  std::vector<int> cperm, bmap;
+  sbhost::localityOrder(nc, nb, rp.data(), ci.data(), cperm, bmap);
+  std::vector<int> seen(nc, 0), bseen(nb, 0);
+  for (int i : cperm) seen[i]++;
+  for (int j : bmap) bseen[j]++;
+  for (int i = 0; i < nc; ++i) CHECK(seen[i] == 1, "contact %d appears %d times", i, seen[i]);
+  for (int j = 0; j < nb; ++j) CHECK(bseen[j] == 1, "body %d appears %d times", j, bseen[j]);
+  // permuted H (columns renumbered and sorted within each row) and Minv
+  std::vector<int> rp2{0}, ci2;
+  std::vector<double> hv2;
+  for (int i = 0; i < nc; ++i) {
+    const int o = cperm[i];
+    std::vector<std::pair<int, int>> row;
+    for (int p = rp[o]; p < rp[o + 1]; ++p) row.push_back({bmap[ci[p]], p});
+    std::sort(row.begin(), row.end());
+    for (auto& e : row) {
+      ci2.push_back(e.first);
+      hv2.insert(hv2.end(), hv.begin() + (size_t)e.second * 18, hv.begin() + (size_t)e.second * 18 + 18);
+    }
+    rp2.push_back((int)ci2.size());
+  }
+  std::vector<double> minv2((size_t)nb * 36);
+  for (int j = 0; j < nb; ++j)
+    for (int e = 0; e < 36; ++e) minv2[(size_t)bmap[j] * 36 + e] = minv[(size_t)j * 36 + e];
+  // consecutive contacts: lowest bodies non-decreasing
+  int prevLo = -1;
+  for (int i = 0; i < nc; ++i) {
+    if (rp2[i + 1] == rp2[i]) continue;
+    CHECK(ci2[rp2[i]] >= prevLo, "contact %d out of order", i);
+    prevLo = ci2[rp2[i]];
+  }
+  auto applyW = [&](const std::vector<int>& r, const std::vector<int>& c, const std::vector<double>& h,
+                    const std::vector<double>& m, const std::vector<double>& x) {
+    std::vector<double> z((size_t)nb * 6, 0.0), w((size_t)nb * 6, 0.0), y((size_t)nc * 3, 0.0);
+    for (int i = 0; i < nc; ++i)
+      for (int p = r[i]; p < r[i + 1]; ++p)
+        for (int a = 0; a < 6; ++a)
+          for (int rr = 0; rr < 3; ++rr) z[(size_t)c[p] * 6 + a] += h[(size_t)p * 18 + rr * 6 + a] * x[3 * i + rr];
+    for (int j = 0; j < nb; ++j)
+      for (int a = 0; a < 6; ++a)
+        for (int b2 = 0; b2 < 6; ++b2) w[(size_t)j * 6 + a] += m[(size_t)j * 36 + a * 6 + b2] * z[(size_t)j * 6 + b2];
+    for (int i = 0; i < nc; ++i)
+      for (int p = r[i]; p < r[i + 1]; ++p)
+        for (int rr = 0; rr < 3; ++rr)
+          for (int a = 0; a < 6; ++a) y[3 * i + rr] += h[(size_t)p * 18 + rr * 6 + a] * w[(size_t)c[p] * 6 + a];
+    return y;
+  };
+  std::vector<double> x((size_t)nc * 3), x2((size_t)nc * 3);
+  for (auto& v : x) v = val(g);
+  for (int i = 0; i < nc; ++i)
+    for (int k = 0; k < 3; ++k) x2[3 * i + k] = x[3 * cperm[i] + k];
+  const std::vector<double> y = applyW(rp, ci, hv, minv, x), y2 = applyW(rp2, ci2, hv2, minv2, x2);
+  for (int i = 0; i < nc; ++i)
+    for (int k = 0; k < 3; ++k) CHECK(y2[3 * i + k] == y[3 * cperm[i] + k], "W x differs at contact %d", i);
+}
+
 int main() {
+  testLocalityOrder();
   testColoringFromH();
   testTiling();
   testTranspose();

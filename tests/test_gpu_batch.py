@@ -127,3 +127,31 @@ def test_batch_multi_workgroup_teams(R, O, ctx, small_batch):
     _check_each(O, small_batch, res, B.split(res.r), 1e-6, 5000)
     again = B.solve(**kw)
     assert np.array_equal(again.r, res.r)
+
+
+def test_batch_locality_order_is_transparent(R, O, ctx, small_batch):
+    """The batch stores each problem in its locality order by default; r goes in and
+    comes out in the caller's order (host and device r), the same as a batch kept in
+    the caller's order up to summation order, and every result is oracle-verified."""
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-6,
+              max_iters=5000, precision=R.PRECISION_FP64)
+    B = R.Batch(ctx, small_batch)
+    C = R.Batch(ctx, small_batch, caller_order=True)
+    rb = B.solve(config=(512, 1, -1), **kw)
+    rc = C.solve(config=(512, 1, -1), **kw)
+    _check_each(O, small_batch, rb, B.split(rb.r), 1e-6, 5000)
+    assert rb.n_converged >= len(small_batch) - 2
+    # (APGD trajectories depend on summation order: a problem that stalls in one order
+    # may converge in the other)
+    for k, (a, c) in enumerate(zip(B.split(rb.r), C.split(rc.r))):
+        if rb.converged[k] and rc.converged[k]:
+            assert np.linalg.norm(a - c) <= 5e-2 * max(np.linalg.norm(c), 1e-12), k
+    # a warm start from the caller-order result, device-resident: converges at once
+    d = R.DeviceArray(ctx, 3 * B.total_contacts)
+    d.upload(rc.r)
+    warm = B.solve(d, R.pg_options(**kw), config=(512, 1, -1), on_device=True)
+    assert all(warm.iterations[k] == 0 for k in range(len(small_batch)) if rc.converged[k])
+    back = d.download()
+    for k, (a, c) in enumerate(zip(B.split(back), C.split(rc.r))):
+        if rc.converged[k]:
+            assert np.linalg.norm(a - c) <= 1e-3 * max(np.linalg.norm(c), 1e-12), k
