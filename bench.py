@@ -16,7 +16,8 @@ from the FP32 result, verifies the tolerance and refines where it is not met. Th
 FP64 batch, the plain FP64 path (512 single-problem solves one after the other),
 the 50,000-contact single problem (FP64 and mixed; and to FB 1e-8 with APGD and
 the colored Gauss-Seidel solvers, assembled W and matrix-free, against the
-oracle on the host), the batch with r in host memory, and the stand-alone block
+oracle on the host), the gradient of a 200-step unroll at 100,000 contacts (cost
+and a finite-difference check), the batch with r in host memory, and the stand-alone block
 SpMV bandwidth at 50k and at 1M contacts are reported beside it, with the CPU
 oracle timed on this host's cores on a sample of the batch.
 
@@ -162,6 +163,46 @@ def tight_50k(ctx, R, G, cpu_threads: int) -> dict:
     return out
 
 
+def adjoint_100k(ctx, R, G, n_dirs: int = 8) -> dict:
+    """The 100,000-contact derivative workload: mixed-precision solve, then the
+    gradient of L = g . T^K(r*) (K = 200, SOC law, checkpointed unroll) in b, mu and
+    Minv, its cost against the forward K steps, and central differences
+    (h = 1e-6, FP64, projection branches of the unperturbed run replayed) on
+    n_dirs random directions."""
+    prob = G.workload("adjoint-100k")
+    dp = R.DualFrictionProblem(ctx, prob, assemble=False)
+    sol = dp.solve_pg(np.zeros(3 * prob.nc), variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE,
+                      scaling=R.SCALING_DIAGONAL, tol=1e-6, max_iters=20000, precision=R.PRECISION_MIXED)
+    g = np.random.default_rng(55).standard_normal(3 * prob.nc)
+    K = 200
+    ad = dp.adjoint(sol.r, g, iterations=K, law=R.LAW_SOC, record_branches=True)
+    ad = dp.adjoint(sol.r, g, iterations=K, law=R.LAW_SOC, record_branches=True)
+    fwd = dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=R.LAW_SOC)
+    fwd = dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=R.LAW_SOC)
+    rng = np.random.default_rng(56)
+    h, errs = 1e-6, []
+    for _ in range(n_dirs):
+        db = rng.standard_normal(3 * prob.nc)
+        dmu = 0.1 * rng.standard_normal(prob.nc)
+        A = rng.standard_normal((prob.nb, 6, 6)) * 0.05
+        dM = (A + A.transpose(0, 2, 1)).reshape(prob.nb, 36)
+        an = ad["grad_b"] @ db + ad["grad_mu"] @ dmu + np.sum(ad["grad_minv"] * dM)
+        lv = []
+        for t in (h, -h):
+            dp.update(b=prob.b + t * db, mu=prob.mu + t * dmu, minv=prob.minv.reshape(prob.nb, 36) + t * dM)
+            lv.append(float(g @ dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=R.LAW_SOC,
+                                           branches=ad["branches"])["r"]))
+        errs.append(abs((lv[0] - lv[1]) / (2 * h) - an) / max(abs(an), 1e-12))
+    dp.update(b=prob.b, mu=prob.mu, minv=prob.minv.reshape(prob.nb, 36))
+    out = {"contacts": prob.nc, "bodies": prob.nb, "forward_solve_ms": sol.device_ms,
+           "forward_solve_iterations": sol.iterations, "unroll_steps": K, "gradient_ms": ad["device_ms"],
+           "forward_steps_ms": fwd["device_ms"], "gradient_vs_forward": ad["device_ms"] / fwd["device_ms"],
+           "fd_directions": n_dirs, "fd_max_rel_error": float(max(errs)), "law": "soc"}
+    log(f"adjoint-100k: gradient of {K} steps {ad['device_ms']:.1f} ms ({out['gradient_vs_forward']:.2f}x the "
+        f"forward steps), FD max rel error {out['fd_max_rel_error']:.1e} on {n_dirs} directions")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +336,7 @@ def main() -> int:
                         "contact_solves_per_s": p50.nc / (np.median(times[1:]) * 1e-3)}
     log(f"contacts-50k: FP64 {single['fp64']['ms']:.2f} ms, mixed {single['mixed']['ms']:.2f} ms per solve")
     tight = tight_50k(ctx, R, G, 0 if args.no_cpu else (args.cpu_threads or min(16, len(os.sched_getaffinity(0)))))
+    adj = adjoint_100k(ctx, R, G)
     nc50, nb50, _, wb50 = dp50.sizes()
     spmv = {}
     for kind, name, nbytes in ((0, "assembled_wx_50k", wx_assembled_bytes(wb50, nc50)),
@@ -369,6 +411,7 @@ def main() -> int:
         "speedup_mixed_batch_vs_plain_fp64": ratemx / seq_rate,
         "contacts_50k": single,
         "contacts_50k_tol_1e-8": tight,
+        "adjoint_100k": adj,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "pg_batch_kernel (FP32 stage + FP64 verification stage, one graph)",
