@@ -216,6 +216,20 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, con
 int sb_pgs_solve(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* opts, const int64_t* colors, int n_colors,
                  double* r, int on_device, double* residual, sb_stats* stats);
 
+/* Device-side set-up for an unchanged pattern (the per-step update of a simulation whose
+   contact set is the same): new H block values (3x6 row-major, in the problem's current
+   contact and block order), inverse masses (6x6 row-major per body), b and mu, any of them
+   NULL to keep it; host (on_device = 0) or device arrays. The H and H^T tiles, Minv, the
+   diagonal blocks, row scalings and an assembled W = H Minv H^T are recomputed on the GPU
+   (the work of setMatrix / computeFrom, FrictionProblem.cpp:30-40, without the symbolic
+   phase). SB_ERR_INVALID when W was supplied by sb_problem_set_w and H or Minv changes. */
+int sb_problem_update_values(sb_ctx* ctx, sb_problem* P, const double* h_vals, const double* minv, const double* b,
+                             const double* mu, int on_device);
+/* Minv_k = M_k^-1 for n_bodies 6x6 row-major blocks on the GPU (PrimalFrictionProblem::
+   computeMInv, PrimalFrictionProblem.cpp:28-38; Gauss-Jordan with partial pivoting).
+   SB_ERR_INVALID if a block is singular. */
+int sb_mass_inverse(sb_ctx* ctx, int n_bodies, const double* M, double* minv, int on_device);
+
 /* Cadoux fixed point (DualFrictionProblem::solveCadoux / solveCadouxVel,
    src/Interfaces/FrictionProblem.cpp:68-100, Cadoux.hpp:44-155): up to `iterations` SOC
    solves with the inner solver, each with the affine term b + s(W r + b) (s the De Saxce

@@ -165,6 +165,13 @@ struct sb_problem {
   // workspace (FP64 solvers and kernels)
   DBuf<double> X[4], MX[4], G[3], xbest, wbody, carry, rio, hist, y, terms, local, reg, ndx;
   DBuf<double> cad[5];  // Cadoux fixed point: original b, iterate, shift, operator image, velocities
+  // device set-up (sb_problem_update_values): the H^T tiles' source blocks, the W tiles'
+  // product terms, the BSR index of H, and the current H and Minv values (row-major);
+  // hostStale: the host copies (hv, minv, b, mu, diag, scal, wv) lag the device
+  std::vector<int> htsrc;
+  bool devSetup = false, hostStale = false, devRawValid = false, wFromH = true;
+  DBuf<int> d_hsrc, d_htsrc, d_wtptr, d_wterm, d_hrp, d_hci;
+  DBuf<double> d_hraw, d_mraw;
   // workspace of the FP32 (mixed-precision) projected-gradient solver
   DBuf<float> X32[4], MX32[4], G32[3], xbest32, wbody32, carry32;
   DBuf<unsigned char> skip;
@@ -264,6 +271,7 @@ int upload_values(sb_problem* P) {
   hipStream_t s = P->ctx->stream;
   const int nc = P->nc, nb = P->nb;
   ++P->version;
+  P->devRawValid = false;  // the host copies are the current values
   if (P->hasW) {
     std::vector<int> frp, fci;
     std::vector<double> fv;
@@ -304,13 +312,18 @@ int upload_problem(sb_problem* P) {
   hipStream_t s = P->ctx->stream;
   const int nc = P->nc, nb = P->nb;
   ++P->version;
+  P->devSetup = false;  // index maps follow the (possibly new) pattern
   // H contact-major, H^T body-major
   SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
   {
-    std::vector<int> trp, tci;
+    std::vector<int> trp, tci, tsrc;
     std::vector<double> tv;
-    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv);
-    SB_HIP(P->HT.upload(sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data()), s));
+    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv, &tsrc);
+    sbhost::Tiles ht = sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data());
+    P->htsrc.assign(ht.src.size(), -1);
+    for (size_t q = 0; q < ht.src.size(); ++q)
+      if (ht.src[q] >= 0) P->htsrc[q] = tsrc[ht.src[q]];
+    SB_HIP(P->HT.upload(std::move(ht), s));
   }
   int rc = upload_values(P);
   if (rc) return rc;
@@ -492,6 +505,65 @@ __global__ void k_split_row(const int* lrp, const int* lci, const double* lv, in
   res[2] = acc[2];
 }
 
+}  // namespace
+
+namespace {
+// Host copies of the values after device-side updates (paths that rebuild or read
+// them on the host: permutation, W export, host updates, GS local matrices, the
+// matrix-free GS's per-block products).
+int sync_host(sb_problem* P) {
+  if (!P->hostStale) return SB_OK;
+  hipStream_t s = P->ctx->stream;
+  const int nc = P->nc, nb = P->nb;
+  SB_HIP(hipMemcpyAsync(P->hv.data(), P->d_hraw.p, sizeof(double) * P->hv.size(), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(P->minv.data(), P->d_mraw.p, sizeof(double) * 36 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(P->b.data(), P->d_b.p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(P->mu.data(), P->d_mu.p, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost, s));
+  P->diag.resize((size_t)9 * nc);
+  P->scal.resize(nc);
+  SB_HIP(hipMemcpyAsync(P->diag.data(), P->d_diag.p, sizeof(double) * 9 * (size_t)nc, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(P->scal.data(), P->d_scal.p, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  if (P->hasW && P->wFromH)
+    sbhost::assembleW(nc, nb, P->hrp.data(), P->hci.data(), P->hv.data(), P->minv.data(), P->wrp, P->wci, P->wv);
+  P->hostStale = false;
+  return SB_OK;
+}
+
+// Index maps of the device set-up, built from the host structures of the current
+// pattern: the source block of every H and H^T tile block, the BSR index of H, and
+// the product terms (p_i, p_j) of every expanded W tile block.
+int setup_maps(sb_problem* P) {
+  if (P->devSetup) return SB_OK;
+  hipStream_t s = P->ctx->stream;
+  SB_HIP(P->d_hsrc.upload(P->H.host.src, s));
+  SB_HIP(P->d_htsrc.upload(P->htsrc, s));
+  SB_HIP(P->d_hrp.upload(P->hrp, s));
+  SB_HIP(P->d_hci.upload(P->hci, s));
+  if (P->hasW) {
+    const sbhost::Tiles& T = P->W.host;
+    std::vector<int> tptr(1, 0), term;
+    for (int q = 0; q < T.nblocks; ++q) {
+      if (T.src[q] >= 0) {
+        // block (i, j): terms of the lower block (max, min) in the order the host assembly sums them
+        const int i = std::max(T.brow[q], T.bcol[q]), j = std::min(T.brow[q], T.bcol[q]);
+        for (int a = P->hrp[i]; a < P->hrp[i + 1]; ++a)
+          for (int b = P->hrp[j]; b < P->hrp[j + 1]; ++b)
+            if (P->hci[a] == P->hci[b]) {
+              term.push_back(a);
+              term.push_back(b);
+            }
+      }
+      tptr.push_back((int)term.size() / 2);
+    }
+    if (term.empty()) term = {0, 0};
+    SB_HIP(P->d_wtptr.upload(tptr, s));
+    SB_HIP(P->d_wterm.upload(term, s));
+  }
+  SB_HIP(hipStreamSynchronize(s));
+  P->devSetup = true;
+  return SB_OK;
+}
 }  // namespace
 
 namespace {
@@ -766,10 +838,12 @@ int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* wrp, const int32
     for (int p = wrp[i]; p < wrp[i + 1]; ++p)
       if (wci[p] < 0 || wci[p] > i) return SB_ERR_INVALID;  // lower triangle only
   }
+  if ((rc = sync_host(P))) return rc;
   P->wrp.assign(wrp, wrp + nc + 1);
   P->wci.assign(wci, wci + wrp[nc]);
   P->wv.assign(wv, wv + (size_t)wrp[nc] * 9);
   P->hasW = true;
+  P->wFromH = false;
   return upload_problem(P);
 }
 
@@ -789,9 +863,12 @@ int sb_problem_sizes(const sb_problem* P, int* nc, int* nb, int64_t* hblocks, in
   return SB_OK;
 }
 
-int sb_problem_get_w(sb_ctx* ctx, const sb_problem* P, int32_t* wrp, int32_t* wci, double* wv) {
+int sb_problem_get_w(sb_ctx* ctx, const sb_problem* Pc, int32_t* wrp, int32_t* wci, double* wv) {
   (void)ctx;
-  if (!P || !P->hasW) return SB_ERR_INVALID;
+  if (!Pc || !Pc->hasW) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  const int rc = sync_host(P);
+  if (rc) return rc;
   if (wrp) std::memcpy(wrp, P->wrp.data(), sizeof(int32_t) * P->wrp.size());
   if (wci) std::memcpy(wci, P->wci.data(), sizeof(int32_t) * P->wci.size());
   if (wv) std::memcpy(wv, P->wv.data(), sizeof(double) * P->wv.size());
@@ -802,6 +879,7 @@ int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!P || !perm) return SB_ERR_INVALID;
+  if ((rc = sync_host(P))) return rc;
   const int n = P->nc;
   std::vector<int> inv(n, -1);
   for (int i = 0; i < n; ++i) {
@@ -1046,6 +1124,8 @@ static int gs_prepare(sb_ctx* ctx, sb_problem* P, const sb_gs_options* o, const 
     col = {0, n};
   }
   hipStream_t s = ctx->stream;
+  int rc;
+  if ((rc = sync_host(P))) return rc;
   std::vector<double> local(P->diag), reg(n, 0.0);
   if (o->auto_regularization > 0.0)
     for (int i = 0; i < n; ++i) {
@@ -1059,7 +1139,6 @@ static int gs_prepare(sb_ctx* ctx, sb_problem* P, const sb_gs_options* o, const 
   SB_HIP(P->colors.upload(col, s));
   SB_HIP(P->skip.alloc(n));
   SB_HIP(hipMemsetAsync(P->skip.p, 0, n, s));
-  int rc;
   if ((rc = copy_in(ctx, P->rio.p, r, 3 * (size_t)n, on_device))) return rc;
   histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
   if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
@@ -1187,6 +1266,8 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
 // in the H tiles' block order, and each contact's first block and block count.
 static int pgs_upload(sb_problem* P) {
   if (P->pgsVersion == P->version) return SB_OK;
+  const int rc = sync_host(P);
+  if (rc) return rc;
   const sbhost::Tiles& T = P->H.host;
   const int n = P->nc;
   const size_t nq = (size_t)T.nblocks;
@@ -1437,6 +1518,7 @@ int sb_problem_update(sb_ctx* ctx, sb_problem* P, const double* b, const double*
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!P) return SB_ERR_INVALID;
+  if ((rc = sync_host(P))) return rc;
   if (b) P->b.assign(b, b + 3 * (size_t)P->nc);
   if (mu) P->mu.assign(mu, mu + P->nc);
   if (minv) {
@@ -1621,6 +1703,82 @@ int sb_memcpy_d2h(sb_ctx* ctx, void* dst, const void* src, size_t bytes) {
   SB_HIP(hipStreamSynchronize(ctx->stream));
   return SB_OK;
 }
+// Device-side refresh of a problem's values for an unchanged pattern: new H block
+// values (row-major 3x6, in the problem's current contact order and BSR block
+// order), inverse masses (row-major 6x6 per body), b, mu; NULL leaves a part as it
+// is. Tiles of H and H^T, Minv (FP64 and FP32 copies), the diagonal blocks and row
+// scalings and, with an assembled W, W's values are recomputed by kernels from
+// index maps built once per pattern. The host copies are refreshed lazily, by
+// the host paths that need them.
+int sb_problem_update_values(sb_ctx* ctx, sb_problem* P, const double* hvals, const double* minv, const double* b,
+                             const double* mu, int on_device) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!P) return SB_ERR_INVALID;
+  if ((hvals || minv) && P->hasW && !P->wFromH) return SB_ERR_INVALID;  // W was supplied, not H Minv H^T
+  hipStream_t s = ctx->stream;
+  const int nc = P->nc, nb = P->nb;
+  const size_t nh = P->hv.size();
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if ((rc = setup_maps(P))) return rc;
+  if (!P->devRawValid) {  // current values from the host copies
+    SB_HIP(P->d_hraw.upload(P->hv, s));
+    SB_HIP(P->d_mraw.upload(P->minv, s));
+    P->devRawValid = true;
+  }
+  if (hvals) SB_HIP(hipMemcpyAsync(P->d_hraw.p, hvals, sizeof(double) * nh, kind, s));
+  if (minv) SB_HIP(hipMemcpyAsync(P->d_mraw.p, minv, sizeof(double) * 36 * (size_t)nb, kind, s));
+  if (b) SB_HIP(hipMemcpyAsync(P->d_b.p, b, sizeof(double) * 3 * (size_t)nc, kind, s));
+  if (mu) SB_HIP(hipMemcpyAsync(P->d_mu.p, mu, sizeof(double) * (size_t)nc, kind, s));
+  if (hvals) {
+    SB_HIP(sb::launch_tile_values(3, 6, false, P->H.host.nblocks, P->d_hsrc.p, P->d_hraw.p, P->H.vals.p,
+                                  P->H.vals32.p, s));
+    SB_HIP(sb::launch_tile_values(6, 3, true, P->HT.host.nblocks, P->d_htsrc.p, P->d_hraw.p, P->HT.vals.p,
+                                  P->HT.vals32.p, s));
+  }
+  if (minv) SB_HIP(sb::launch_minv_layout(nb, P->d_mraw.p, P->d_minv.p, P->d_minv32.p, s));
+  if (hvals || minv) {
+    SB_HIP(sb::launch_diag_scal(nc, P->d_hrp.p, P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->d_diag.p, P->d_scal.p, s));
+    if (P->hasW)
+      SB_HIP(sb::launch_w_numeric(P->W.host.nblocks, P->W.brow.p, P->W.bcol.p, P->d_wtptr.p, P->d_wterm.p,
+                                  P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->W.vals.p, P->W.vals32.p, s));
+  }
+  SB_HIP(hipStreamSynchronize(s));
+  ++P->version;
+  P->hostStale = true;
+  return SB_OK;
+}
+
+// Minv_k = M_k^-1 for n_bodies 6x6 row-major blocks (PrimalFrictionProblem::computeMInv,
+// PrimalFrictionProblem.cpp:28-38), on the device; SB_ERR_INVALID if a block is singular.
+int sb_mass_inverse(sb_ctx* ctx, int n_bodies, const double* M, double* minv, int on_device) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (n_bodies < 0 || (n_bodies > 0 && (!M || !minv))) return SB_ERR_INVALID;
+  if (n_bodies == 0) return SB_OK;
+  hipStream_t s = ctx->stream;
+  const size_t n = 36 * (size_t)n_bodies;
+  DBuf<double> dM, dI;
+  DBuf<int> err;
+  const double* pM = M;
+  double* pI = minv;
+  if (!on_device) {
+    SB_HIP(dM.alloc(n));
+    SB_HIP(dI.alloc(n));
+    SB_HIP(hipMemcpyAsync(dM.p, M, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    pM = dM.p;
+    pI = dI.p;
+  }
+  SB_HIP(err.alloc(1));
+  SB_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  SB_HIP(sb::launch_mass_inverse(n_bodies, pM, pI, err.p, s));
+  int e = 0;
+  SB_HIP(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (!on_device) SB_HIP(hipMemcpyAsync(minv, dI.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  return e ? SB_ERR_INVALID : SB_OK;
+}
+
 int sb_memset_device(sb_ctx* ctx, void* dst, int value, size_t bytes) {
   int rc = check_ctx(ctx);
   if (rc) return rc;

@@ -43,6 +43,7 @@ Tiles buildTiles(int nrows, int ncols, int br, int bc, const int* rowptr, const 
     T.rowptr[i + 1] = (int)srcBlock.size();
   }
   T.nblocks = (int)srcBlock.size();
+  T.src = srcBlock;
   // tiles: whole rows packed greedily into <= 64 blocks; longer rows get their own tiles
   // A whole-row tile records -(1 + scan steps) for its own longest row; a long-row
   // tile records the long row's index.
@@ -95,19 +96,22 @@ Tiles buildTiles(int nrows, int ncols, int br, int bc, const int* rowptr, const 
 }
 
 void transposeBSR(int nrows, int ncols, int br, int bc, const int* rowptr, const int* colidx, const double* vals,
-                  std::vector<int>& trowptr, std::vector<int>& tcolidx, std::vector<double>& tvals) {
+                  std::vector<int>& trowptr, std::vector<int>& tcolidx, std::vector<double>& tvals,
+                  std::vector<int>* tsrc) {
   const int nnz = rowptr[nrows];
   trowptr.assign(ncols + 1, 0);
   for (int p = 0; p < nnz; ++p) trowptr[colidx[p] + 1]++;
   for (int j = 0; j < ncols; ++j) trowptr[j + 1] += trowptr[j];
   tcolidx.assign(nnz, 0);
   tvals.assign((size_t)nnz * br * bc, 0.0);
+  if (tsrc) tsrc->assign(nnz, -1);
   std::vector<int> fill(trowptr.begin(), trowptr.end() - 1);
   for (int i = 0; i < nrows; ++i)
     for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
       const int j = colidx[p];
       const int q = fill[j]++;
       tcolidx[q] = i;
+      if (tsrc) (*tsrc)[q] = p;
       const double* src = vals + (size_t)p * br * bc;
       double* dst = &tvals[(size_t)q * br * bc];
       for (int r = 0; r < br; ++r)

@@ -12,6 +12,7 @@ struct Tiles {
   int nblocks = 0, ntiles = 0, scan_steps = 0;
   std::vector<int> tstart, brow, bcol, tlong, long_row, long_t0, long_t1;
   std::vector<int> rowptr;   // per-row block offsets in tile order
+  std::vector<int> src;      // per tile-order block: its block in the source BSR, -1 for an empty row's zero block
   std::vector<double> vals;  // [br*bc][nblocks]
 };
 
@@ -21,8 +22,10 @@ Tiles buildTiles(int nrows, int ncols, int br, int bc, const int* rowptr, const 
                  bool transposeBlocks = false);
 
 // Transposes a BSR (blocks become bc x br, rows <-> cols).
+// tsrc (optional): for each transposed block, the index of its source block.
 void transposeBSR(int nrows, int ncols, int br, int bc, const int* rowptr, const int* colidx, const double* vals,
-                  std::vector<int>& trowptr, std::vector<int>& tcolidx, std::vector<double>& tvals);
+                  std::vector<int>& trowptr, std::vector<int>& tcolidx, std::vector<double>& tvals,
+                  std::vector<int>* tsrc = nullptr);
 
 // Expands a symmetric lower-triangle BSR of square blocks into both triangles.
 void expandSymmetric(int n, int bs, const int* rowptr, const int* colidx, const double* vals, std::vector<int>& frowptr,

@@ -40,7 +40,8 @@ EXPORTED = [
     "sb_ctx_info", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
-    "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
+    "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_problem_update_values",
+    "sb_mass_inverse", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
     "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_memset_device", "sb_adjoint_options_default",
     "sb_adjoint", "sb_problem_update",
@@ -147,6 +148,8 @@ def lib():
                                   C.POINTER(Stats)]),
         "sb_pgs_solve": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, _vp, C.c_int, _dp,
                                    C.POINTER(Stats)]),
+        "sb_problem_update_values": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+        "sb_mass_inverse": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
         "sb_pg_solve_cadoux": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), C.c_int, C.c_double, C.c_int, _vp,
                                          C.c_int, _dp, C.POINTER(Stats)]),
         "sb_gs_solve_cadoux": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, C.c_int, C.c_double,
@@ -487,6 +490,19 @@ class DualFrictionProblem:
                                                                    tol_tighten, vel, rp, dev, res, st)
         return self._run(fn, r0, on_device, hist_cap)
 
+    def update_values(self, hvals=None, minv=None, b=None, mu=None):
+        """Device-side refresh for an unchanged pattern (sb_problem_update_values): new H
+        block values (BSR order of this problem, 18 per block), Minv, b, mu; numpy arrays
+        (host) or DeviceArray (device), all of one kind; None keeps a part."""
+        parts = [hvals, minv, b, mu]
+        dev = any(isinstance(v, DeviceArray) for v in parts if v is not None)
+        if dev:
+            ptrs = [None if v is None else v.ptr for v in parts]
+        else:
+            keep = [None if v is None else _f64(v).reshape(-1) for v in parts]
+            ptrs = [None if v is None else v.ctypes.data_as(_vp) for v in keep]
+        _check(lib().sb_problem_update_values(self.ctx.h, self.h, *ptrs, 1 if dev else 0), "sb_problem_update_values")
+
     def update(self, b=None, mu=None, minv=None):
         """Replace b, mu and/or Minv in place (sb_problem_update)."""
         keep = [None if v is None else _f64(v).reshape(-1) for v in (b, mu, minv)]
@@ -631,6 +647,15 @@ class Batch:
         _check(lib().sb_batch_autotune(self.ctx.h, self.h, C.byref(o), probe_iters, C.byref(best), C.byref(ms)),
                "sb_batch_autotune")
         return (best.block_threads, best.blocks_per_cu, best.contacts_per_block), ms.value
+
+
+def mass_inverse(ctx: Context, M) -> np.ndarray:
+    """Minv_k = M_k^-1 for [n_bodies, 36] row-major 6x6 blocks, on the GPU (sb_mass_inverse)."""
+    M = _f64(M).reshape(-1, 36)
+    out = np.zeros_like(M)
+    _check(lib().sb_mass_inverse(ctx.h, M.shape[0], M.ctypes.data_as(_vp), out.ctypes.data_as(_vp), 0),
+           "sb_mass_inverse")
+    return out
 
 
 def gpu_available() -> bool:

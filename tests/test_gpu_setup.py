@@ -1,0 +1,122 @@
+"""Problem set-up on the GPU (sb_problem_update_values, sb_mass_inverse) against the
+host set-up of a freshly created problem with the same values.
+
+The reference recomputes W = H Minv H^T, the diagonal blocks and the scalings on the
+host at every computeFrom / setMatrix (FrictionProblem.cpp:30-40,
+SparseMatrixMatrixProduct.impl.hpp:295-349) and inverts the masses with per-body LU
+(PrimalFrictionProblem.cpp:28-38). Here a problem keeps its pattern and the values
+are refreshed by kernels; the bar: every product, residual and solve of the updated
+problem equals the freshly created problem's (products to 1e-14 relative: the same
+sums, possibly contracted differently; solves to the same iteration count where the
+arithmetic is the same, else at tolerance).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _perturbed(G, prob, seed):
+    """Same pattern, new values: H blocks, Minv (symmetric), b and mu perturbed."""
+    rng = np.random.default_rng(seed)
+    scale = 1.0 + 0.1 * rng.random(prob.nb)  # keeps every Minv block symmetric positive definite
+    return dataclasses.replace(
+        prob,
+        hvals=prob.hvals * (1.0 + 0.1 * rng.standard_normal(prob.hvals.shape)),
+        minv=(prob.minv.reshape(prob.nb, 36) * scale[:, None]).reshape(prob.minv.shape),
+        b=prob.b + 0.05 * rng.standard_normal(prob.b.shape),
+        mu=np.clip(prob.mu + 0.05 * rng.standard_normal(prob.mu.shape), 0.05, 1.5))
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_update_values_matches_fresh_problem(R, G, ctx, on_device):
+    print(f"case: device set-up, on_device={on_device}", flush=True)
+    p0 = G.workload("small-1k")
+    p1 = _perturbed(G, p0, 11)
+    dp = R.DualFrictionProblem(ctx, p0, assemble=True)
+    fresh = R.DualFrictionProblem(ctx, p1, assemble=True)
+    if on_device:
+        arrs = []
+        for a in (p1.hvals, p1.minv, p1.b, p1.mu):
+            d = R.DeviceArray(ctx, np.asarray(a).size)
+            d.upload(np.asarray(a, dtype=np.float64).reshape(-1))
+            arrs.append(d)
+        dp.update_values(*arrs)
+    else:
+        dp.update_values(p1.hvals, p1.minv, p1.b, p1.mu)
+    x = np.random.default_rng(5).standard_normal(3 * p0.nc)
+    for op in (R.OP_ASSEMBLED, R.OP_MATRIX_FREE):
+        a, b = dp.apply_w(x, op=op), fresh.apply_w(x, op=op)
+        assert np.linalg.norm(a - b) <= 1e-14 * np.linalg.norm(b), op
+        for law in (R.LAW_COULOMB, R.LAW_SOC):
+            for sc in (R.SCALING_REFERENCE, R.SCALING_DIAGONAL):
+                ra = dp.eval_with(x, law=law, op=op, scaling=sc)
+                rb = fresh.eval_with(x, law=law, op=op, scaling=sc)
+                assert ra == pytest.approx(rb, rel=1e-12), (op, law, sc)
+    # the assembled W read back (host copy refreshed from the device values)
+    wa, wb = dp.w_lower(), fresh.w_lower()
+    assert np.array_equal(wa[0], wb[0]) and np.array_equal(wa[1], wb[1])
+    assert np.allclose(wa[2], wb[2], rtol=1e-13, atol=1e-13 * np.abs(wb[2]).max())
+    # solves: APGD (device data only) and GS (local matrices from the refreshed diagonal)
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-8,
+              max_iters=5000)
+    s1, s2 = dp.solve_pg(np.zeros(3 * p0.nc), **kw), fresh.solve_pg(np.zeros(3 * p0.nc), **kw)
+    assert s1.converged and s2.converged
+    assert abs(s1.iterations - s2.iterations) <= max(5, s2.iterations // 4)
+    g1 = dp.solve_gs(np.zeros(3 * p0.nc), law=R.LAW_SOC, tol=1e-8, max_iters=2000)
+    g2 = fresh.solve_gs(np.zeros(3 * p0.nc), law=R.LAW_SOC, tol=1e-8, max_iters=2000)
+    assert g1.converged and g2.converged and abs(g1.iterations - g2.iterations) <= 25
+
+
+def test_update_values_then_permutation_and_pgs(R, G, O, ctx):
+    """After a device update, the host paths (coloring + permutation, the matrix-free
+    GS's per-block products) see the new values."""
+    print("case: device set-up, then permutation and product GS", flush=True)
+    p0 = G.workload("small-1k")
+    p1 = _perturbed(G, p0, 12)
+    dp = R.DualFrictionProblem(ctx, p0, assemble=False)
+    dp.update_values(p1.hvals, p1.minv, p1.b, p1.mu)
+    perm, colors = dp.coloring(R.COLORING_GREEDY)
+    dp.apply_permutation(perm)
+    g = dp.solve_pgs(np.zeros(3 * p0.nc), law=R.LAW_SOC, tol=1e-8, max_iters=2000, colors=colors)
+    op = O.OracleProblem(p1, assemble=False)
+    op.permute(perm.astype(np.int32))
+    ores, ox, od = op.pgs_solve(np.zeros(3 * p0.nc), tol=1e-8, max_iters=2000, coulomb=False, colors=colors,
+                                ndx_ref_per_color=True)
+    assert g.converged and g.iterations == od["iterations"]
+    assert np.linalg.norm(g.r - ox) <= 1e-6 * np.linalg.norm(ox)
+
+
+def test_partial_updates(R, G, ctx):
+    """b / mu alone leave the matrices; Minv alone refreshes W and the scalings."""
+    p0 = G.workload("small-1k")
+    p1 = _perturbed(G, p0, 13)
+    x = np.random.default_rng(6).standard_normal(3 * p0.nc)
+    dp = R.DualFrictionProblem(ctx, p0, assemble=True)
+    dp.update_values(b=p1.b, mu=p1.mu)
+    ref = R.DualFrictionProblem(ctx, dataclasses.replace(p0, b=p1.b, mu=p1.mu), assemble=True)
+    assert np.array_equal(dp.apply_w(x), ref.apply_w(x))
+    assert dp.eval_with(x, law=R.LAW_COULOMB) == pytest.approx(ref.eval_with(x, law=R.LAW_COULOMB), rel=1e-14)
+    dp.update_values(minv=p1.minv)
+    ref = R.DualFrictionProblem(ctx, dataclasses.replace(p0, b=p1.b, mu=p1.mu, minv=p1.minv), assemble=True)
+    for op in (R.OP_ASSEMBLED, R.OP_MATRIX_FREE):
+        a, b = dp.apply_w(x, op=op), ref.apply_w(x, op=op)
+        assert np.linalg.norm(a - b) <= 1e-14 * np.linalg.norm(b)
+
+
+def test_mass_inverse(R, ctx):
+    rng = np.random.default_rng(7)
+    nb = 5000
+    A = rng.standard_normal((nb, 6, 6))
+    M = A @ A.transpose(0, 2, 1) + 6.0 * np.eye(6)  # symmetric positive definite mass blocks
+    Minv = R.mass_inverse(ctx, M.reshape(nb, 36)).reshape(nb, 6, 6)
+    ref = np.linalg.inv(M)
+    assert np.abs(Minv - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(M @ Minv - np.eye(6)).max() <= 1e-12
+    # a zero leading pivot: the elimination must swap rows
+    Mz = 2.0 * np.eye(6)[[1, 0, 2, 3, 4, 5]]
+    assert np.array_equal(R.mass_inverse(ctx, Mz.reshape(1, 36)).reshape(6, 6), Mz.T / 4.0)
+    with pytest.raises(RuntimeError):
+        R.mass_inverse(ctx, np.zeros((1, 36)))
