@@ -17,7 +17,8 @@ FP64 batch, the plain FP64 path (512 single-problem solves one after the other),
 the 50,000-contact single problem (FP64 and mixed; and to FB 1e-8 with APGD and
 the colored Gauss-Seidel solvers, assembled W and matrix-free, against the
 oracle on the host), the gradient of a 200-step unroll at 100,000 contacts (cost
-and a finite-difference check), the batch with r in host memory, and the stand-alone block
+and a finite-difference check), the per-step set-up cost (host creation against the
+device value refresh), the batch with r in host memory, and the stand-alone block
 SpMV bandwidth at 50k and at 1M contacts are reported beside it, with the CPU
 oracle timed on this host's cores on a sample of the batch.
 
@@ -203,6 +204,41 @@ def adjoint_100k(ctx, R, G, n_dirs: int = 8) -> dict:
     return out
 
 
+def setup_times(ctx, R, G) -> dict:
+    """Per-step set-up cost of a contact problem whose pattern is unchanged: creating
+    the problem on the host (tiling, transposition, W assembly, upload) against
+    refreshing its values on the device (sb_problem_update_values: H and H^T tiles,
+    Minv, diagonal blocks, scalings, W), with the new values in host memory (PCIe
+    included) or already in HBM."""
+    out = {}
+    for name, assemble in (("contacts-50k", True), ("zipf-1m", False)):
+        p = G.workload(name)
+        t0 = time.perf_counter()
+        dp = R.DualFrictionProblem(ctx, p, assemble=assemble)
+        ctx.synchronize()
+        create = time.perf_counter() - t0
+        arrs = []
+        for a in (p.hvals, p.minv, p.b, p.mu):
+            d = R.DeviceArray(ctx, np.asarray(a).size)
+            d.upload(np.asarray(a, dtype=np.float64).reshape(-1))
+            arrs.append(d)
+        host, dev = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            dp.update_values(p.hvals, p.minv, p.b, p.mu)
+            host.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            dp.update_values(*arrs)
+            dev.append(time.perf_counter() - t0)
+        out[name] = {"assembled_w": assemble, "create_on_host_ms": 1e3 * create,
+                     "update_from_host_ms": 1e3 * float(np.median(host)),
+                     "update_on_device_ms": 1e3 * float(np.median(dev))}
+        log(f"set-up {name}: create {1e3 * create:.1f} ms, device update {out[name]['update_on_device_ms']:.2f} ms "
+            f"({out[name]['update_from_host_ms']:.2f} ms with the values in host memory)")
+        del dp, arrs
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,6 +373,7 @@ def main() -> int:
     log(f"contacts-50k: FP64 {single['fp64']['ms']:.2f} ms, mixed {single['mixed']['ms']:.2f} ms per solve")
     tight = tight_50k(ctx, R, G, 0 if args.no_cpu else (args.cpu_threads or min(16, len(os.sched_getaffinity(0)))))
     adj = adjoint_100k(ctx, R, G)
+    setup = setup_times(ctx, R, G)
     nc50, nb50, _, wb50 = dp50.sizes()
     spmv = {}
     for kind, name, nbytes in ((0, "assembled_wx_50k", wx_assembled_bytes(wb50, nc50)),
@@ -412,6 +449,7 @@ def main() -> int:
         "contacts_50k": single,
         "contacts_50k_tol_1e-8": tight,
         "adjoint_100k": adj,
+        "setup": setup,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "pg_batch_kernel (FP32 stage + FP64 verification stage, one graph)",
