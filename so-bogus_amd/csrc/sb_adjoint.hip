@@ -199,12 +199,17 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
               adjoint_point(coulomb, w.mu, rho, J, u, ut, w.lam, mz, muu);
               st3t<Team>(a.mz, i, mz);
               st3t<Team>(a.mu_u, i, muu);
-              // a row has one leader thread in every step (static tile schedule):
-              // read-modify-write through the coherent accessors
+              // a row has one leader thread in every step (static tile schedule), and
+              // team barriers separate the steps: every old value is loaded first (no
+              // store between the loads, so they are in flight together), then stored
+              double gb[3];
 #pragma unroll
-              for (int c = 0; c < 3; ++c) Team::st(a.grad_b + 3 * i + c, Team::cload(a.grad_b + 3 * i + c) + muu[c]);
-              Team::st(a.grad_mu + i, Team::cload(a.grad_mu + i) + w.lam[0] * dmu[0] + w.lam[1] * dmu[1] +
-                                          w.lam[2] * dmu[2] - (coulomb ? rho * ut * mz[0] : 0.0));
+              for (int c = 0; c < 3; ++c) gb[c] = Team::ld(a.grad_b + 3 * i + c);
+              const double gmu = Team::ld(a.grad_mu + i);
+#pragma unroll
+              for (int c = 0; c < 3; ++c) Team::st(a.grad_b + 3 * i + c, gb[c] + muu[c]);
+              Team::st(a.grad_mu + i, gmu + w.lam[0] * dmu[0] + w.lam[1] * dmu[1] + w.lam[2] * dmu[2] -
+                                          (coulomb ? rho * ut * mz[0] : 0.0));
             };
             ok = apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, body) && tm.sync();
           }
@@ -216,10 +221,15 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
 #pragma unroll
               for (int c = 0; c < 6; ++c) hr[c] = Team::ld(&a.hr[6 * kb + c]);
               double* gm = a.grad_minv + (size_t)36 * kb;
+              // one leader per body row in every step, steps separated by team barriers:
+              // the 36 old values are loaded together, then updated
+              double g[36];
+#pragma unroll
+              for (int e = 0; e < 36; ++e) g[e] = Team::ld(gm + e);
 #pragma unroll
               for (int r = 0; r < 6; ++r)
 #pragma unroll
-                for (int c = 0; c < 6; ++c) Team::st(gm + 6 * r + c, Team::cload(gm + 6 * r + c) + v[r] * hr[c]);
+                for (int c = 0; c < 6; ++c) Team::st(gm + 6 * r + c, g[6 * r + c] + v[r] * hr[c]);
               bodyDefault(kb, v);
             };
             auto pre = [&](int i) {
