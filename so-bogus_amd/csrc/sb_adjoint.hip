@@ -126,7 +126,7 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
     return apply_operator<true, double>(P, a.wbody, a.carry, tm, ld, pre, lead, bodyDefault) && tm.sync();
   };
   auto copy = [&](double* dst, const double* src) {
-    for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(dst + k, Team::cload(src + k));
+    team_copy(tm, dst, src, n3);
   };
   // zero the gradients
   if (a.g) {

@@ -225,7 +225,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
     if (tm.leader() && it < pb.hist_cap) pb.hist[it] = res;
     if (it == 0 || res < minRes) {
       minRes = res;
-      for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(pb.xbest + k, Team::cload(pb.X[rx] + k));
+      team_copy(tm, pb.xbest, pb.X[rx], n3);
     }
     if (res < prm.tol) {
       converged = true;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
   // x = x_best (ProjectedGradient.impl.hpp:152); the same thread wrote xbest[k] or
   // a barrier separates the writer
   if (ok)
-    for (int k = tm.tid; k < n3; k += tm.nthreads) Team::st(pb.x_io + k, (double)Team::cload(pb.xbest + k));
+    team_copy(tm, pb.x_io, pb.xbest, n3);
   if (tm.leader()) {
     pb.out[0] = minRes;
     pb.out[1] = (double)it;

@@ -68,6 +68,25 @@ struct GridTeamT {
 };
 using GridTeam = GridTeamT<true>;
 
+// dst[k] = src[k] (converted) for k = tid, tid + nthreads, ... < n, by a team; src
+// was written before the team's last barrier (or by the same thread). Groups of
+// eight loads are issued before their stores, so they are in flight together (a
+// load/store pair per iteration would wait one memory round trip each).
+template <typename Team, typename TD, typename TS>
+__device__ __forceinline__ void team_copy(const Team& tm, TD* dst, const TS* src, int n) {
+  constexpr int U = 8;
+  const int st = tm.nthreads;
+  int k = tm.tid;
+  for (; k + (U - 1) * st < n; k += U * st) {
+    TS v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = Team::ld(src + k + u * st);
+#pragma unroll
+    for (int u = 0; u < U; ++u) Team::st(dst + k + u * st, (TD)v[u]);
+  }
+  for (; k < n; k += st) Team::st(dst + k, (TD)Team::ld(src + k));
+}
+
 struct BlockTeam {
   static constexpr bool kCoherentReads = false;
   int gw, nwaves, tid, nthreads;
