@@ -317,6 +317,9 @@ typedef struct sb_batch_stats {
   int* converged;         /* optional caller array [n_problems] */
   int* stage_spmvs;       /* optional caller array [n_problems][2]: operator applications of the
                              FP32 stage (0 in FP64 precision) and of the FP64 stage */
+  double* phase_us;       /* optional caller array [n_problems][2][4]: per stage (FP32, FP64), the
+                             team's time in operator phases (work, then reduction and barrier
+                             wait) and in update phases (work, wait), in microseconds */
 } sb_batch_stats;
 
 /* Problems k = 0..n_problems-1 as in sb_problem_create (flags: SB_PROBLEM_ASSEMBLE_W). Each

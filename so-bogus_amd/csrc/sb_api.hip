@@ -2235,6 +2235,11 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
         st->stage_spmvs[2 * k + 1] = (int)o64[2];
       }
       if (st->iterations) st->iterations[k] = it;
+      if (st->phase_us)
+        for (int e = 0; e < 4; ++e) {
+          st->phase_us[8 * k + e] = pl.mixed ? o32[7 + e] : 0.0;
+          st->phase_us[8 * k + 4 + e] = o64[7 + e];
+        }
       if (st->residuals) st->residuals[k] = o64[0];
       if (st->converged) st->converged[k] = conv ? 1 : 0;
     }

@@ -90,7 +90,8 @@ class BatchStats(C.Structure):
     _fields_ = [("n_converged", C.c_int), ("total_iterations", C.c_int), ("max_iterations", C.c_int),
                 ("refined_problems", C.c_int), ("total_spmvs", C.c_int64), ("device_ms", C.c_double),
                 ("config", BatchConfig), ("iterations", C.POINTER(C.c_int)), ("residuals", C.POINTER(C.c_double)),
-                ("converged", C.POINTER(C.c_int)), ("stage_spmvs", C.POINTER(C.c_int))]
+                ("converged", C.POINTER(C.c_int)), ("stage_spmvs", C.POINTER(C.c_int)),
+                ("phase_us", C.POINTER(C.c_double))]
 
 
 _dp = C.POINTER(C.c_double)
@@ -568,6 +569,7 @@ class BatchResult:
     device_ms: float
     stage_spmvs: np.ndarray         # [n][2] operator applications: FP32 stage, FP64 stage
     config: tuple[int, int, int]
+    phase_us: np.ndarray = field(default_factory=lambda: np.zeros(0))  # [n][stage][op work, op wait, update work, update wait]
 
 
 class Batch:
@@ -620,8 +622,10 @@ class Batch:
         res = np.zeros(self.n)
         conv = np.zeros(self.n, dtype=np.int32)
         stage = np.zeros((self.n, 2), dtype=np.int32)
+        phase = np.zeros((self.n, 2, 4))
         st = BatchStats()
         st.stage_spmvs = stage.ctypes.data_as(C.POINTER(C.c_int))
+        st.phase_us = _d(phase)
         st.iterations = its.ctypes.data_as(C.POINTER(C.c_int))
         st.residuals = _d(res)
         st.converged = conv.ctypes.data_as(C.POINTER(C.c_int))
@@ -636,7 +640,7 @@ class Batch:
         return BatchResult(status=code, r=rout, residuals=res, iterations=its, converged=conv.astype(bool),
                            n_converged=st.n_converged, total_iterations=st.total_iterations,
                            max_iterations=st.max_iterations, refined_problems=st.refined_problems,
-                           total_spmvs=st.total_spmvs, device_ms=st.device_ms, stage_spmvs=stage,
+                           total_spmvs=st.total_spmvs, device_ms=st.device_ms, stage_spmvs=stage, phase_us=phase,
                            config=(st.config.block_threads, st.config.blocks_per_cu,
                                    st.config.contacts_per_block))
 
