@@ -52,20 +52,36 @@ __device__ __forceinline__ T wave_max(T v) {
 // fence), and loads of such values in the next phase are then ordinary cached
 // loads. cld() (agent-scope load) is kept for reads that may not follow a
 // barrier of the reading block.
+//
+// Every access goes through the global address space explicitly: the solvers'
+// arrays live in device memory, but pointers read from memory (the per-problem
+// descriptors of the batched kernel) carry no address space, and the compiler
+// would otherwise emit flat instructions, which are slower and are waited on
+// together with LDS and scalar accesses.
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const gmem<T>*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(gmem<T>*)p = v;
+}
 __device__ __forceinline__ void cst(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((gmem<double>*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double cld(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((gmem<double>*)const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void cst(float* p, float v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((gmem<float>*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ float cld(const float* p) {
-  return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((gmem<float>*)const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void cst(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((gmem<int>*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Load of a value another block wrote in an earlier phase of the same kernel.
 template <typename T>
@@ -73,7 +89,7 @@ __device__ __forceinline__ T mld(const T* p) {
 #if SB_COHERENT_GATHER
   return cld(p);
 #else
-  return *p;
+  return gld(p);
 #endif
 }
 
@@ -114,7 +130,7 @@ struct Barrier {
   // block increments the top counter; the last group writes the new epoch into
   // every group's release word, which that group's blocks poll. Contention on any
   // one address is limited to a group (or the kSyncGroups group leaders).
-  __device__ bool sync() {
+  __device__ __forceinline__ bool sync() {
     __shared__ int s_ok;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -164,7 +180,7 @@ struct Barrier {
   // and leaves the results in red[] of every block. Block partials are combined
   // in the same fixed order in every block, so all blocks see the same bits.
   template <int NV>
-  __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {
+  __device__ __forceinline__ bool reduce(const double (&v)[NV], unsigned max_mask) {
     __shared__ double s_p[NV][16];
     __shared__ double s_r[NV];
     // Two partials buffers: a block can start writing the buffer of reduction n+2

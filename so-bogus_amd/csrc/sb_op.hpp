@@ -16,7 +16,7 @@ __device__ __forceinline__ void minv_apply(const T* minv, int nb, int k, const T
   for (int r = 0; r < 6; ++r) {
     T acc = T(0);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc += minv[(size_t)(r * 6 + c) * nb + k] * v[c];
+    for (int c = 0; c < 6; ++c) acc += gld(minv + (size_t)(r * 6 + c) * nb + k) * v[c];
     w[r] = acc;
   }
 }
@@ -37,7 +37,7 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
         T v[3];
         long_row_sum<3, T, T, Team::kCoherentReads>(P.W, k, carry, v);
         if (lane_id() == 0) {
-          const int row = P.W.long_row[k];
+          const int row = gld(P.W.long_row + k);
           lead(row, v, pre(row));
         }
       }
@@ -52,7 +52,7 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
     for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {
       T v[6];
       long_row_sum<6, T, T, Team::kCoherentReads>(P.HT, k, carry, v);
-      if (lane_id() == 0) body(P.HT.long_row[k], v);
+      if (lane_id() == 0) body(gld(P.HT.long_row + k), v);
     }
     if (!tm.sync()) return false;
   }

@@ -43,9 +43,9 @@ namespace {
 
 template <typename T>
 __device__ __forceinline__ void dld3(const double* p, int i, double v[3]) {
-  v[0] = p[3 * i];
-  v[1] = p[3 * i + 1];
-  v[2] = p[3 * i + 2];
+  v[0] = gld(p + 3 * i);
+  v[1] = gld(p + 3 * i + 1);
+  v[2] = gld(p + 3 * i + 2);
 }
 template <typename Team, typename T>
 __device__ __forceinline__ void tld3(const T* p, int i, T v[3]) {
@@ -129,7 +129,7 @@ template <typename T>
 __device__ __forceinline__ void start_point(const PGProblem<T>& pb, int i, T xc[3]) {
   double v[3];
   dld3<T>(pb.x_io, i, v);
-  project<double>(pb.P.mu[i], v[0], v[1], v[2]);
+  project<double>(gld(pb.P.mu + i), v[0], v[1], v[2]);
   xc[0] = (T)v[0];
   xc[1] = (T)v[1];
   xc[2] = (T)v[2];
@@ -158,14 +158,14 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
   double minRes = -1.0, res = 0.0, J = 0.0, alpha = 1.0, theta = 1.0;
   int rx = 0, rprev = 0, rg = 0;
   uint64_t tp0 = 0, tp1 = 0;
-  auto rowScale = [&](int i) { return prm.use_scaling ? P.scal[i] : 1.0; };
+  auto rowScale = [&](int i) { return prm.use_scaling ? gld(P.scal + i) : 1.0; };
 
   auto initPre = [&](int i) {
     InitRow r;
     T xr[3];
     start_point<T>(pb, i, xr);
     widen3<T>(xr, r.x);
-    r.mu = P.mu[i];
+    r.mu = gld(P.mu + i);
     dld3<T>(P.b, i, r.b);
     r.s = rowScale(i);
     return r;
@@ -222,7 +222,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
   int lastImprove = 0;
   while (ok) {
     // ---- test_residual at the current x (iteration `it`)
-    if (tm.leader() && it < pb.hist_cap) pb.hist[it] = res;
+    if (tm.leader() && it < pb.hist_cap) gst(pb.hist + it, res);
     if (it == 0 || res < minRes) {
       minRes = res;
       team_copy(tm, pb.xbest, pb.X[rx], n3);
@@ -289,7 +289,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         tld3<Team, T>(pb.MX[rx], i, r.m);
         if (Fuse && useNesterov) tld3<Team, T>(pb.MX[rprev], i, r.mp);
         dld3<T>(P.b, i, r.b);
-        r.mu = P.mu[i];
+        r.mu = gld(P.mu + i);
         r.s = rowScale(i);
         return r;
       };
@@ -300,7 +300,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
           T xv[3], gv[3];
           tld3<Team, T>(Xx, j, xv);
           tld3<Team, T>(Gx, j, gv);
-          proj_step<T>((T)P.mu[j], al, xv, gv, xc);
+          proj_step<T>((T)gld(P.mu + j), al, xv, gv, xc);
         };
         auto lead = [&](int i, T v[3], const TrialRow<T>& r) {
           T xst[3];
@@ -368,7 +368,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
             tld3<Team, T>(pb.MX[rprev], i, Mpt);
             double b[3], rr, Jn;
             dld3<T>(P.b, i, b);
-            nesterov_point(i, xst, Mst, pvt, Mpt, P.mu[i], rowScale(i), b, rr, Jn);
+            nesterov_point(i, xst, Mst, pvt, Mpt, gld(P.mu + i), rowScale(i), b, rr, Jn);
             p2[0] = resMax ? fmax(p2[0], rr) : p2[0] + rr;
             p2[1] += Jn;
           }
@@ -408,7 +408,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         tld3<Team, T>(pb.MX[rx], i, r.m);
         tld3<Team, T>(pb.MX[rprev], i, r.mp);
         dld3<T>(P.b, i, r.b);
-        r.mu = P.mu[i];
+        r.mu = gld(P.mu + i);
         r.s = rowScale(i);
         return r;
       };
@@ -416,7 +416,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         T xv[3], gv[3];
         tld3<Team, T>(Xx, j, xv);
         tld3<Team, T>(Gx, j, gv);
-        proj_step<T>((T)P.mu[j], al, xv, gv, xc);
+        proj_step<T>((T)gld(P.mu + j), al, xv, gv, xc);
       };
       auto lead = [&](int i, T v[3], const SpgRow<T>& r) {
         T xst[3];
@@ -487,7 +487,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
             tld3<Team, T>(pb.MX[rprev], i, Mpt);
             double b[3], rr, Jn;
             dld3<T>(P.b, i, b);
-            nesterov_point(i, xst, Mst, pvt, Mpt, P.mu[i], rowScale(i), b, rr, Jn);
+            nesterov_point(i, xst, Mst, pvt, Mpt, gld(P.mu + i), rowScale(i), b, rr, Jn);
             p2[0] = resMax ? fmax(p2[0], rr) : p2[0] + rr;
           }
           tp1 = realtime();
@@ -523,14 +523,14 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
   if (ok)
     team_copy(tm, pb.x_io, pb.xbest, n3);
   if (tm.leader()) {
-    pb.out[0] = minRes;
-    pb.out[1] = (double)it;
-    pb.out[2] = (double)spmvs;
-    pb.out[3] = (double)lsTrials;
-    pb.out[4] = (double)nest;
-    pb.out[5] = converged ? 1.0 : 0.0;
-    pb.out[6] = ok ? 0.0 : 1.0;
-    for (int k = 0; k < 4; ++k) pb.out[7 + k] = 0.01 * (double)tsum[k];  // microseconds (100 MHz clock)
+    gst(pb.out + 0, minRes);
+    gst(pb.out + 1, (double)it);
+    gst(pb.out + 2, (double)spmvs);
+    gst(pb.out + 3, (double)lsTrials);
+    gst(pb.out + 4, (double)nest);
+    gst(pb.out + 5, converged ? 1.0 : 0.0);
+    gst(pb.out + 6, ok ? 0.0 : 1.0);
+    for (int k = 0; k < 4; ++k) gst(pb.out + 7 + k, 0.01 * (double)tsum[k]);  // microseconds (100 MHz clock)
   }
 }
 

@@ -44,9 +44,9 @@ struct GridTeamT {
     lead = rank == 0 && threadIdx.x == 0;
   }
   __device__ bool leader() const { return lead; }
-  __device__ bool sync() { return bar.sync(); }
+  __device__ __forceinline__ bool sync() { return bar.sync(); }
   template <int NV>
-  __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {
+  __device__ __forceinline__ bool reduce(const double (&v)[NV], unsigned max_mask) {
     if (!bar.reduce<NV>(v, max_mask)) return false;
 #pragma unroll
     for (int k = 0; k < NV; ++k) red[k] = bar.red[k];
@@ -100,14 +100,14 @@ struct BlockTeam {
   __device__ bool leader() const { return threadIdx.x == 0; }
   // Workgroup barrier; __syncthreads orders global-memory accesses of the
   // workgroup (all its waves share one CU and its L1).
-  __device__ bool sync() {
+  __device__ __forceinline__ bool sync() {
     __syncthreads();
     return true;
   }
   // Fixed-order reduction: xor tree inside each wave, then the per-wave values
   // in wave order. Every thread ends with the same bits in red[].
   template <int NV>
-  __device__ bool reduce(const double (&v)[NV], unsigned max_mask) {
+  __device__ __forceinline__ bool reduce(const double (&v)[NV], unsigned max_mask) {
     __shared__ double s_p[NV][16];  // up to 1024 threads
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -127,15 +127,15 @@ struct BlockTeam {
   }
   template <typename T>
   __device__ static void st(T* p, T v) {
-    *p = v;
+    gst(p, v);
   }
   template <typename T>
   __device__ static T ld(const T* p) {
-    return *p;
+    return gld(p);
   }
   template <typename T>
   __device__ static T cload(const T* p) {
-    return *p;
+    return gld(p);
   }
 };
 

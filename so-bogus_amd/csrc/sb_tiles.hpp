@@ -49,13 +49,13 @@ struct TileIdx {
 template <typename T>
 __device__ __forceinline__ TileIdx tile_indices(const TileMat<T>& A, int t) {
   TileIdx ix;
-  ix.q0 = A.tstart[t];
-  ix.q1 = A.tstart[t + 1];
-  ix.tl = A.tlong[t];
+  ix.q0 = gld(A.tstart + t);
+  ix.q1 = gld(A.tstart + t + 1);
+  ix.tl = gld(A.tlong + t);
   const int q = ix.q0 + lane_id();
   if (q < ix.q1) {
-    ix.row = A.brow[q];
-    ix.col = A.bcol[q];
+    ix.row = gld(A.brow + q);
+    ix.col = gld(A.bcol + q);
   } else {
     ix.row = -1;
     ix.col = 0;
@@ -88,7 +88,8 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
     const size_t nb = (size_t)A.nblocks;
     T blk[BR * BC];
 #pragma unroll
-    for (int e = 0; e < BR * BC; ++e) blk[e] = NT ? __builtin_nontemporal_load(vq + (size_t)e * nb) : vq[(size_t)e * nb];
+    for (int e = 0; e < BR * BC; ++e)
+      blk[e] = NT ? __builtin_nontemporal_load((const gmem<T>*)(vq + (size_t)e * nb)) : gld(vq + (size_t)e * nb);
     Acc xc[BC];
     load(ix.col, xc);
 #pragma unroll
@@ -158,10 +159,10 @@ __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int st
 // no cache invalidation separates the reads from the writes; otherwise plain loads.
 template <int BR, typename T, typename Acc, bool Coherent = true>
 __device__ __forceinline__ void long_row_sum(const TileMat<T>& A, int k, const Acc* carry, Acc v[BR]) {
-  const int t0 = A.long_t0[k], t1 = A.long_t1[k];
+  const int t0 = gld(A.long_t0 + k), t1 = gld(A.long_t1 + k);
 #pragma unroll
   for (int r = 0; r < BR; ++r) v[r] = Acc(0);
-  auto ld = [&](size_t i) { return Coherent ? cld(carry + i) : carry[i]; };
+  auto ld = [&](size_t i) { return Coherent ? cld(carry + i) : gld(carry + i); };
   int t = t0 + lane_id();
   for (; t + 3 * SB_WAVE < t1; t += 4 * SB_WAVE) {
     Acc c[4][BR];
