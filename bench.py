@@ -253,8 +253,8 @@ def main() -> int:
                     help="short run for rocprofv3: tuned mixed batch solved 3 times + calibration copies, no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py: this program runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is not implemented)", file=sys.stderr)
         return 2
 
     _load_package()

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. This is the correctness oracle: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
-// as the checker (or the timed CPU baseline), never as the thing measured or
+// to check results (or as the timed CPU baseline), never as the thing measured or
 // shipped. The product path (so-bogus_amd/) never links or calls it.
 //
 // Every routine cites the reference file:line it restates. Reference paths are

@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/_build/libcpuref.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
-checker, and by bench.py's cpu_baseline leg. The product path never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to check
+results, and by bench.py's cpu_baseline leg. The product path never imports it.
 """
 from __future__ import annotations
 
