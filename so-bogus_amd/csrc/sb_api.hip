@@ -1875,6 +1875,7 @@ struct sb_batch {
   DBuf<sb::PGProblem<float>> d32;
   DBuf<sb::PGProblem<double>> d64;
   DBuf<int> d_order;          // [n] problems grouped by class
+  DBuf<int> bmap;             // [8 * CUs] class, team and rank of each block of the grid
   DBuf<unsigned> syncw;       // [kQueueWords + teams * kTeamSyncWords]: class queues, team barriers
   DBuf<double> teamPartials;  // [teams][2][kMaxTeamBlocks][kMaxPartials]
   DBuf<int> teamSlot;         // [teams]
@@ -2064,8 +2065,35 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
     b0 += teams[c] * ks[c];
     t0 += teams[c];
   }
-  pl.grid = b0;
-  pl.nteams = t0;
+  pl.nteams = t0;  // the grid stays whole: blocks no team uses leave at once
+}
+
+// Places the teams on the grid. Workgroups are dispatched to the XCDs round-robin
+// (block b runs on XCD b % 8), so a team of k blocks b = 8 s + x, s = s0 .. s0 + k - 1,
+// shares one XCD's L2 (the batch kernel checks this on the device and falls back
+// to L2-invalidating barriers for a team that does not). Teams are taken in
+// decreasing size, each onto the XCD with the fewest blocks in use: the block
+// counts are powers of two and every XCD's share of the grid is a multiple of the
+// largest, so every team fits. bmap[b] = class << 20 | team << 4 | rank, -1 idle.
+constexpr int kXcds = 8;
+std::vector<int> batch_block_map(const BatchPlan& pl) {
+  int kmax = 1;
+  for (int c = 0; c < pl.nclass; ++c) kmax = std::max(kmax, pl.cls[c].k);
+  const int nx = (pl.grid % kXcds == 0 && (pl.grid / kXcds) % kmax == 0) ? kXcds : 1;
+  const int per = pl.grid / nx;
+  std::vector<int> bmap(pl.grid, -1), used(nx, 0);
+  for (int c = 0; c < pl.nclass; ++c) {
+    const sb::BatchClass& cl = pl.cls[c];
+    for (int t = 0; t < cl.nteams; ++t) {
+      int x = 0;
+      for (int y = 1; y < nx; ++y)
+        if (used[y] < used[x]) x = y;
+      if (used[x] + cl.k > per) return {};  // the teams need more blocks than the grid has
+      for (int r = 0; r < cl.k; ++r) bmap[(size_t)(used[x] + r) * nx + x] = (c << 20) | (t << 4) | r;
+      used[x] += cl.k;
+    }
+  }
+  return bmap;
 }
 
 int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, BatchPlan& pl,
@@ -2123,6 +2151,7 @@ void batch_args(sb::PGBatchArgs<T>& a, const sb_batch* B, const BatchPlan& pl, c
   a.prm = prm;
   a.nprob = B->n;
   a.order = B->d_order.p;
+  a.bmap = B->bmap.p;
   a.nclass = pl.nclass;
   for (int c = 0; c < pl.nclass; ++c) a.cls[c] = pl.cls[c];
   a.queue = B->syncw.p;
@@ -2151,6 +2180,12 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
   const sb::PGProblem<float>* p32 = B->d32.p;
   const sb::PGProblem<double>* p64 = B->d64.p;
   const int nwords = (int)(kQueueWords + (size_t)pl.nteams * sb::kTeamSyncWords);
+  {
+    const std::vector<int> bmap = batch_block_map(pl);
+    if (bmap.empty() || (int)bmap.size() > 8 * B->ctx->ncu) return SB_ERR_INVALID;
+    SB_HIP(hipMemcpyAsync(B->bmap.p, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice, s));
+    SB_HIP(hipStreamSynchronize(s));
+  }
   SB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
   if (pl.mixed) {
@@ -2331,6 +2366,7 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
   }
   hipStream_t s = ctx->stream;
   SB_HIP(B->d_order.alloc(n));
+  SB_HIP(B->bmap.alloc((size_t)8 * ctx->ncu));
   SB_HIP(B->xio.alloc(3 * (size_t)B->totalNc));
   SB_HIP(B->outs.alloc((size_t)32 * n));
   SB_HIP(B->error.alloc(1));

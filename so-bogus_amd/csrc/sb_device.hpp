@@ -117,11 +117,23 @@ constexpr int kMaxPartials = 12;
 
 __device__ __forceinline__ uint64_t realtime() { return __builtin_amdgcn_s_memrealtime(); }
 
+// Index of the XCD (accelerator complex die) this wave runs on (HW_REG_XCC_ID).
+__device__ __forceinline__ unsigned xcd_id() { return __builtin_amdgcn_s_getreg((15 << 11) | 20) & 15u; }
+
+// Invalidates this CU's vector L1 only (buffer_inv sc0). After a barrier this is
+// the whole acquire for blocks that share one XCD: the XCD's L2 is their common
+// point of coherence, and the coherent stores of the other blocks have reached it.
+__device__ __forceinline__ void invalidate_l1() { asm volatile("buffer_inv sc0" ::: "memory"); }
+
 struct Barrier {
   GridSync gs;
   unsigned rank = 0;         // this block's index among the gs.nblocks blocks (must be set)
   bool acquire = true;       // invalidate L1/L2 after the barrier (readers use plain loads); false:
                              // readers use coherent loads and the caches are left alone
+  bool one_xcd = false;      // every block of the barrier runs on one XCD: the acquire
+                             // invalidates L1 only and the XCD's L2 keeps its lines
+  bool plain_stores = false; // vectors are written with plain stores: unless one_xcd,
+                             // the L2 is written back before arriving (release)
   unsigned epoch = 0;
   unsigned nred = 0;         // reductions done (selects the partials buffer)
   double red[kMaxPartials];  // results of the last reduction (identical in every block)
@@ -135,6 +147,7 @@ struct Barrier {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
+      if (plain_stores && !one_xcd) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       const unsigned G = gs.nblocks < kSyncGroups ? gs.nblocks : kSyncGroups;
       const unsigned g = rank % G;
       const unsigned gsize = gs.nblocks / G + ((g < gs.nblocks % G) ? 1u : 0u);
@@ -168,7 +181,13 @@ struct Barrier {
       // Acquire: drop this CU's L1 and this XCD's L2 copies of lines written by
       // other blocks before the barrier (their coherent stores already reached
       // memory), so that plain cached loads in the next phase read fresh values.
-      if (acquire) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // Blocks on one XCD read through the L2 their stores went to: L1 only.
+      if (acquire) {
+        if (one_xcd)
+          invalidate_l1();
+        else
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
 #endif
     }
     __syncthreads();

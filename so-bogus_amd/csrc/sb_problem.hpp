@@ -67,16 +67,16 @@ struct PGGridArgs {
   GridSync gs;
 };
 
-// Batch: the grid is split into classes of teams; a team is k consecutive
-// workgroups (k = 1: one workgroup, synchronised by __syncthreads; k > 1: a
-// co-resident group with its own barrier and reduction state). Every team of a
-// class takes problems from the class's queue (a range of order[], most
-// expensive first) and solves them one at a time.
+// Batch: the grid is split into classes of teams; a team is k workgroups (k = 1:
+// one workgroup, synchronised by __syncthreads; k > 1: a co-resident group with
+// its own barrier and reduction state), placed by the block map so that a team's
+// workgroups share one XCD. Every team of a class takes problems from the class's
+// queue (a range of order[], most expensive first) and solves them one at a time.
 constexpr int kMaxBatchClasses = 8;
 constexpr int kMaxTeamBlocks = 16;
 constexpr unsigned kTeamSyncWords = 2 * kSyncGroups * kSyncStride + kSyncStride;  // per team
 struct BatchClass {
-  int b0;       // first block of the class
+  int b0;       // first block of the class (in block-map order)
   int k;        // blocks per team
   int nteams;
   int q0, q1;   // the class's problems: order[q0 .. q1)
@@ -87,6 +87,7 @@ struct PGBatchArgs {
   PGParams prm;
   int nprob;
   const int* order;           // problem indices grouped by class, most expensive first
+  const int* bmap;            // [grid] (class << 20 | team << 4 | rank) of each block, -1: idle
   int nclass;
   BatchClass cls[kMaxBatchClasses];
   unsigned* queue;            // [nclass] next position of each class's queue

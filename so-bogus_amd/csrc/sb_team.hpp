@@ -1,7 +1,7 @@
 // Teams: the set of threads that solves one problem together.
 //
-//  * GridTeam  - co-resident blocks (a whole persistent grid, or k consecutive
-//                blocks of the batched kernel); phases are separated by the
+//  * GridTeam  - co-resident blocks (a whole persistent grid, or k blocks of the
+//                batched kernel on one XCD); phases are separated by the
 //                fence-free grid barrier and reductions go through the
 //                deterministic grid reduction (Barrier) over the team's blocks.
 //  * BlockTeam - one workgroup; phases are separated by __syncthreads and
@@ -21,8 +21,10 @@ namespace sb {
 
 // Acquire = true: the barrier invalidates this XCD's caches and vectors written by
 // other blocks are read with plain loads. Acquire = false: no invalidation; such
-// vectors are read with coherent loads.
-template <bool Acquire>
+// vectors are read with coherent loads. Plain = true: vectors are written with
+// plain stores (they stay in this XCD's L2); the barrier then writes the L2 back
+// before arriving unless every block of the team runs on one XCD (bar.one_xcd).
+template <bool Acquire, bool Plain = false>
 struct GridTeamT {
   // vectors written by other blocks must be read with coherent loads
   static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
@@ -36,6 +38,7 @@ struct GridTeamT {
     bar.gs = g;
     bar.rank = rank;
     bar.acquire = Acquire;
+    bar.plain_stores = Plain;
     const int wpb = blockDim.x / SB_WAVE;
     gw = (int)rank * wpb + wave_in_block();
     nwaves = (int)g.nblocks * wpb;
@@ -54,7 +57,10 @@ struct GridTeamT {
   }
   template <typename T>
   __device__ static void st(T* p, T v) {
-    cst(p, v);
+    if (Plain)
+      gst(p, v);
+    else
+      cst(p, v);
   }
   template <typename T>
   __device__ static T ld(const T* p) {
