@@ -283,7 +283,11 @@ def main() -> int:
         cfg, _ = B.autotune(optsmx, probe_iters=40)
         for _ in range(3):
             res = solve(optsmx, cfg)
-            log(f"profile solve: {res.device_ms:.1f} ms, {res.n_converged} converged, {res.total_spmvs} W x")
+            st = res.stage_spmvs
+            algo = sum(int(st[k, 0]) * wx_matfree_bytes(ncs[k], nbs[k], 4) +
+                       int(st[k, 1]) * wx_matfree_bytes(ncs[k], nbs[k], 8) for k in range(len(probs)))
+            log(f"profile solve: {res.device_ms:.1f} ms, {res.n_converged} converged, {res.total_spmvs} W x, "
+                f"config {res.config}, algorithmic bytes {algo:.6e}")
         ctx.bench_copy(COPY_BYTES, reps=2)
         del r, B
         ctx.close()

@@ -33,7 +33,7 @@ struct GridTeamT {
   bool lead;
   double red[kMaxPartials];
   // The blocks rank = 0..g.nblocks-1 of a team (the whole grid, or a group of
-  // consecutive blocks of the batched kernel) that share the barrier state g.
+  // blocks of the batched kernel) that share the barrier state g.
   __device__ GridTeamT(const GridSync& g, unsigned rank) {
     bar.gs = g;
     bar.rank = rank;
