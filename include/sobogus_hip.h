@@ -75,7 +75,10 @@ enum sb_precision {
 /* Problem creation flags */
 enum sb_problem_flags {
   SB_PROBLEM_ASSEMBLE_W = 1,
-  SB_BATCH_CALLER_ORDER = 2  /* sb_batch_create: store each problem in the caller's contact and body order */
+  SB_BATCH_CALLER_ORDER = 2, /* sb_batch_create: store each problem in the caller's contact and body order */
+  SB_BATCH_SPREAD_TEAMS = 4  /* sb_batch_create: place a team's workgroups on consecutive blocks (spread
+                                over the XCDs) instead of on one XCD; the teams' barriers then write back
+                                and invalidate L2. Results are bit-identical; slower. For tests. */
 };
 /* BSR flags (bogus::flags) */
 enum sb_bsr_flags { SB_BSR_SYMMETRIC = 1 /* only blocks with col <= row are given */ };

@@ -1883,6 +1883,7 @@ struct sb_batch {
   // locality order: gperm[internal contact] = caller's contact (global indices); stage
   // holds r in the caller's order when it comes from host memory
   bool reordered = false;
+  bool spreadTeams = false;   // SB_BATCH_SPREAD_TEAMS: teams on consecutive blocks
   DBuf<int> gperm;
   DBuf<double> stage;
   int maxTeams = 0;
@@ -2074,12 +2075,13 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
 // to L2-invalidating barriers for a team that does not). Teams are taken in
 // decreasing size, each onto the XCD with the fewest blocks in use: the block
 // counts are powers of two and every XCD's share of the grid is a multiple of the
-// largest, so every team fits. bmap[b] = class << 20 | team << 4 | rank, -1 idle.
+// largest, so every team fits. spread: teams on consecutive blocks (each team over
+// several XCDs). bmap[b] = class << 20 | team << 4 | rank, -1 idle.
 constexpr int kXcds = 8;
-std::vector<int> batch_block_map(const BatchPlan& pl) {
+std::vector<int> batch_block_map(const BatchPlan& pl, bool spread) {
   int kmax = 1;
   for (int c = 0; c < pl.nclass; ++c) kmax = std::max(kmax, pl.cls[c].k);
-  const int nx = (pl.grid % kXcds == 0 && (pl.grid / kXcds) % kmax == 0) ? kXcds : 1;
+  const int nx = (!spread && pl.grid % kXcds == 0 && (pl.grid / kXcds) % kmax == 0) ? kXcds : 1;
   const int per = pl.grid / nx;
   std::vector<int> bmap(pl.grid, -1), used(nx, 0);
   for (int c = 0; c < pl.nclass; ++c) {
@@ -2181,7 +2183,7 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
   const sb::PGProblem<double>* p64 = B->d64.p;
   const int nwords = (int)(kQueueWords + (size_t)pl.nteams * sb::kTeamSyncWords);
   {
-    const std::vector<int> bmap = batch_block_map(pl);
+    const std::vector<int> bmap = batch_block_map(pl, B->spreadTeams);
     if (bmap.empty() || (int)bmap.size() > 8 * B->ctx->ncu) return SB_ERR_INVALID;
     SB_HIP(hipMemcpyAsync(B->bmap.p, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice, s));
     SB_HIP(hipStreamSynchronize(s));
@@ -2317,6 +2319,7 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
   B->cost.resize(n);
   B->iterHint.assign(n, 0);
   B->reordered = (flags & SB_BATCH_CALLER_ORDER) == 0;
+  B->spreadTeams = (flags & SB_BATCH_SPREAD_TEAMS) != 0;
   const int pflags = flags & SB_PROBLEM_ASSEMBLE_W;
   std::vector<int> gperm;
   for (int k = 0; k < n; ++k) {

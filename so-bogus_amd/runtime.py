@@ -31,6 +31,7 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 ADJOINT_UNROLL, ADJOINT_IMPLICIT = 0, 1
 COLORING_REFERENCE, COLORING_GREEDY = 0, 1
 BATCH_CALLER_ORDER = 2
+BATCH_SPREAD_TEAMS = 4
 BSR_SYMMETRIC = 1
 PROBLEM_ASSEMBLE_W = 1
 
@@ -575,9 +576,12 @@ class BatchResult:
 class Batch:
     """Many independent dual friction problems solved in one launch (sb_batch)."""
 
-    def __init__(self, ctx: Context, probs, assemble: bool = False, caller_order: bool = False):
+    def __init__(self, ctx: Context, probs, assemble: bool = False, caller_order: bool = False,
+                 spread_teams: bool = False):
         """caller_order: keep each problem in the given contact/body order instead of the
-        library's locality order (results are returned in the caller's order either way)."""
+        library's locality order (results are returned in the caller's order either way).
+        spread_teams: place each team's workgroups on consecutive blocks, over several XCDs
+        (barriers with L2 write-back and invalidation; same results, slower)."""
         self.ctx = ctx
         self.n = len(probs)
         self.ncs = np.array([p.nc for p in probs], dtype=np.int64)
@@ -599,7 +603,8 @@ class Batch:
             ctx.h, C.byref(h), self.n, arr(C.c_int, [p.nc for p in probs]), arr(C.c_int, [p.nb for p in probs]),
             arr(_ip, [_i(a) for a in rps]), arr(_ip, [_i(a) for a in cis]), arr(_dp, [_d(a) for a in hvs]),
             arr(_dp, [_d(a) for a in mis]), arr(_dp, [_d(a) for a in bs]), arr(_dp, [_d(a) for a in mus]),
-            (PROBLEM_ASSEMBLE_W if assemble else 0) | (BATCH_CALLER_ORDER if caller_order else 0)),
+            (PROBLEM_ASSEMBLE_W if assemble else 0) | (BATCH_CALLER_ORDER if caller_order else 0) |
+            (BATCH_SPREAD_TEAMS if spread_teams else 0)),
             "sb_batch_create")
         self.h = h
         self.total_contacts = int(self.ncs.sum())

@@ -129,6 +129,22 @@ def test_batch_multi_workgroup_teams(R, O, ctx, small_batch):
     assert np.array_equal(again.r, res.r)
 
 
+def test_batch_team_placement_is_transparent(R, ctx, small_batch):
+    """Teams of several workgroups run on one XCD by default (L2 kept, L1-only acquire);
+    spread over the XCDs they write back and invalidate L2 at every barrier. Either
+    way a problem's arithmetic is the same: results are bit-identical, both stages."""
+    print("[batch] team placement", flush=True)
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-6,
+              max_iters=5000, precision=R.PRECISION_MIXED, config=(256, 2, 500))
+    B = R.Batch(ctx, small_batch)
+    S = R.Batch(ctx, small_batch, spread_teams=True)
+    rb = B.solve(**kw)
+    rs = S.solve(**kw)
+    assert rb.n_converged >= len(small_batch) - 2
+    assert np.array_equal(rb.iterations, rs.iterations)
+    assert np.array_equal(rb.r, rs.r)
+
+
 def test_batch_locality_order_is_transparent(R, O, ctx, small_batch):
     """The batch stores each problem in its locality order by default; r goes in and
     comes out in the caller's order (host and device r), the same as a batch kept in
