@@ -181,6 +181,15 @@ struct sb_problem {
   DBuf<double> dmt;
   DBuf<int> hrow;
   int version = 0, pgsVersion = -1;
+  // matrix-free projected gradient: factored tiles F = H L and F^T (Minv = L L^T), FP64
+  // and FP32, current while fVersion == version; factored false: some Minv_k is not
+  // positive definite and the solvers apply Minv between the passes
+  DBuf<double> fH, fHT;
+  DBuf<float> fH32, fHT32;
+  DBuf<int> ferr;
+  int fVersion = -1;
+  bool factored = false;
+  bool useFactored() const { return fVersion == version && factored; }
   DBuf<double> out;  // [2][16] solver results (FP32 stage, FP64 stage)
   DevProblem<double> view(bool withScaling) const {
     DevProblem<double> v;
@@ -216,6 +225,11 @@ struct sb_problem {
   // set by the caller.
   void fill(sb::PGProblem<double>& pb, bool withScaling) const {
     pb.P = view(withScaling);
+    if (useFactored()) {
+      pb.P.H = H.viewT<double>(fH.p);
+      pb.P.HT = HT.viewT<double>(fHT.p);
+      pb.P.minv_folded = true;
+    }
     for (int k = 0; k < 4; ++k) {
       pb.X[k] = X[k].p;
       pb.MX[k] = MX[k].p;
@@ -227,6 +241,11 @@ struct sb_problem {
   }
   void fill(sb::PGProblem<float>& pb, bool withScaling) const {
     pb.P = view32(withScaling);
+    if (useFactored()) {
+      pb.P.H = H.viewT<float>(fH32.p);
+      pb.P.HT = HT.viewT<float>(fHT32.p);
+      pb.P.minv_folded = true;
+    }
     for (int k = 0; k < 4; ++k) {
       pb.X[k] = X32[k].p;
       pb.MX[k] = MX32[k].p;
@@ -1022,6 +1041,37 @@ int sb_fb_residual(sb_ctx* ctx, const sb_problem* Pc, int law, int op, int scali
   return std::isfinite(acc) ? SB_OK : SB_ERR_NAN;
 }
 
+// Factored operator tiles for the matrix-free projected-gradient solvers: with the
+// Cholesky factors Minv_k = L_k L_k^T, W = F F^T for F_ik = H_ik L_k, so an operator
+// application is the two tile passes F (F^T x), with no Minv blocks to read and no
+// body step between them, and stays exactly symmetric in FP32. Computed on the device
+// from the current tile values and Minv after the problem's values change; a Minv_k
+// that is not positive definite keeps the unfactored product.
+static int factor_prepare(sb_problem* P) {
+  if (P->fVersion == P->version) return SB_OK;
+  const size_t nh = 18 * (size_t)P->H.host.nblocks, nht = 18 * (size_t)P->HT.host.nblocks;
+  if (P->fH.n != nh) SB_HIP(P->fH.alloc(nh));
+  if (P->fH32.n != nh) SB_HIP(P->fH32.alloc(nh));
+  if (P->fHT.n != nht) SB_HIP(P->fHT.alloc(nht));
+  if (P->fHT32.n != nht) SB_HIP(P->fHT32.alloc(nht));
+  if (P->ferr.n != 1) SB_HIP(P->ferr.alloc(1));
+  DBuf<double> L;
+  SB_HIP(L.alloc(36 * (size_t)P->nb));
+  hipStream_t s = P->ctx->stream;
+  int err = 0;
+  SB_HIP(hipMemcpyAsync(P->ferr.p, &err, sizeof(int), hipMemcpyHostToDevice, s));
+  SB_HIP(sb::launch_minv_chol(P->nb, P->d_minv.p, L.p, P->ferr.p, s));
+  SB_HIP(sb::launch_factor_tiles(false, P->H.host.nblocks, P->nb, P->H.bcol.p, P->H.vals.p, L.p, P->fH.p, P->fH32.p,
+                                 s));
+  SB_HIP(sb::launch_factor_tiles(true, P->HT.host.nblocks, P->nb, P->HT.brow.p, P->HT.vals.p, L.p, P->fHT.p,
+                                 P->fHT32.p, s));
+  SB_HIP(hipMemcpyAsync(&err, P->ferr.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  P->factored = (err == 0);
+  P->fVersion = P->version;
+  return SB_OK;
+}
+
 int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, double* r, int on_device,
                 double* residual, sb_stats* st) {
   int rc = check_ctx(ctx);
@@ -1037,6 +1087,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
   const bool mf = (o->op == SB_OP_MATRIX_FREE);
   const bool mixed = (o->precision == SB_PRECISION_MIXED);
+  if (mf && (rc = factor_prepare(P))) return rc;
   const sb::PGParams prm = make_params(o, mf);
   const int workWaves = mf ? std::max(P->H.host.ntiles, P->HT.host.ntiles) : P->W.host.ntiles;
   const int threads = block_threads_for(o->block_threads, kMaxThreads);
@@ -2366,6 +2417,7 @@ int sb_batch_create(sb_ctx* ctx, sb_batch** out, int n, const int* ncs, const in
     B->totalNc += ncs[k];
     const sb_problem* P = B->probs[k].get();
     B->cost[k] = (int64_t)P->H.host.nblocks * 2 + P->HT.host.nblocks + (B->hasW ? P->W.host.nblocks : 0);
+    if ((rc = factor_prepare(B->probs[k].get()))) return rc;
   }
   hipStream_t s = ctx->stream;
   SB_HIP(B->d_order.alloc(n));

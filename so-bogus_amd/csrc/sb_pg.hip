@@ -67,11 +67,17 @@ __device__ __forceinline__ void widen3(const T v[3], double d[3]) {
 }
 
 // One operator application M x (assembled W, or H (Minv (H^T x)) with the
-// default body step: Minv applied and stored for the H pass).
+// default body step: Minv applied - or, with factored tiles, F (F^T x) and the
+// body vector F^T x stored as it is - for the H pass).
 template <bool MF, typename T, typename Team, typename Loader, typename Pre, typename Leader>
 __device__ __forceinline__ bool apply_op(const PGProblem<T>& pb, Team& tm, Loader& ld, Pre& pre, Leader& lead) {
   const DevProblem<T>& P = pb.P;
   auto body = [&](int k, T v[6]) {
+    if (P.minv_folded) {
+#pragma unroll
+      for (int r = 0; r < 6; ++r) Team::st(&pb.wbody[6 * k + r], v[r]);
+      return;
+    }
     T w[6];
     minv_apply<T>(P.minv, P.nb, k, v, w);
 #pragma unroll

@@ -11,6 +11,8 @@ namespace sb {
 // A dual friction problem u = W r + b resident in HBM.
 //   H  : contact-major tiles, 3x6 blocks (H_ik)
 //   HT : body-major tiles, 6x3 blocks (H_ik^T), for H^T r
+//   with minv_folded, H and HT hold F_ik = H_ik L_k and F_ik^T instead (Minv_k = L_k L_k^T):
+//        the matrix-free product is F (F^T x) with no body step
 //   W  : assembled Delassus operator, both triangles expanded, 3x3 blocks (optional)
 //   minv : inverse mass blocks, element-major [36][nb]
 //   diag : W_ii blocks [nc][9]; scal : max(1, tr(W_ii)/3)
@@ -24,6 +26,7 @@ struct DevProblem {
   const double* diag = nullptr;
   const double* scal = nullptr;
   bool hasW = false;
+  bool minv_folded = false;  // H / HT hold F = H L and F^T (Minv = L L^T): the body step is the identity
 };
 
 // Projected-gradient parameters shared by every problem of a launch.

@@ -177,6 +177,98 @@ __global__ void __launch_bounds__(128) k_mass_inverse(int nb, const double* __re
   }
 }
 
+// Factored operator tiles. With Minv_k = L_k L_k^T (Cholesky, L_k lower triangular),
+// W = H Minv H^T = F F^T for F_ik = H_ik L_k: the matrix-free product is then the
+// two tile passes F (F^T x) with no body step, exactly symmetric in any precision.
+// k_minv_chol: L_k from the element-major Minv [36][nb] into element-major [36][nb]
+// (upper part zero); a pivot that is not positive (Minv_k not positive definite)
+// sets err.
+__global__ void __launch_bounds__(256) k_minv_chol(int nb, const double* __restrict__ minv, double* __restrict__ L,
+                                                   int* __restrict__ err) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) {
+    double a[6][6], l[6][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        a[r][c] = minv[(size_t)(r * 6 + c) * nb + k];
+        l[r][c] = 0.0;
+      }
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double d = a[c][c];
+#pragma unroll
+      for (int m = 0; m < c; ++m) d -= l[c][m] * l[c][m];
+      ok = ok && d > 0.0;
+      const double ljj = sqrt(d > 0.0 ? d : 1.0);
+      l[c][c] = ljj;
+#pragma unroll
+      for (int r = c + 1; r < 6; ++r) {
+        double t = a[r][c];
+#pragma unroll
+        for (int m = 0; m < c; ++m) t -= l[r][m] * l[c][m];
+        l[r][c] = t / ljj;
+      }
+    }
+    if (!ok) atomicOr(err, 1);
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) L[(size_t)(r * 6 + c) * nb + k] = l[r][c];
+  }
+}
+
+// F tiles: Transposed = false: 3x6 blocks F_q = H_q L_k, k = bcol[q] (contact-major
+// H tiles); Transposed = true: 6x3 blocks F_q = L_k^T HT_q, k = brow[q] (body-major
+// H^T tiles). Element-major in and out, FP64 and FP32.
+template <bool Transposed>
+__global__ void __launch_bounds__(256) k_factor_tiles(int nq, int nb, const int* __restrict__ body,
+                                                      const double* __restrict__ v, const double* __restrict__ L,
+                                                      double* __restrict__ f, float* __restrict__ f32) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    const int k = body[q];
+    double h[18], l[36];
+#pragma unroll
+    for (int e = 0; e < 18; ++e) h[e] = v[(size_t)e * nq + q];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) l[e] = L[(size_t)e * nb + k];
+#pragma unroll
+    for (int r = 0; r < (Transposed ? 6 : 3); ++r)
+#pragma unroll
+      for (int c = 0; c < (Transposed ? 3 : 6); ++c) {
+        double acc = 0.0;
+        if (Transposed) {
+#pragma unroll
+          for (int m = r; m < 6; ++m) acc += l[m * 6 + r] * h[m * 3 + c];  // (L^T)_rm HT_mc
+        } else {
+#pragma unroll
+          for (int m = c; m < 6; ++m) acc += h[r * 6 + m] * l[m * 6 + c];  // H_rm L_mc
+        }
+        const int e = Transposed ? r * 3 + c : r * 6 + c;
+        f[(size_t)e * nq + q] = acc;
+        f32[(size_t)e * nq + q] = (float)acc;
+      }
+  }
+}
+
+hipError_t launch_minv_chol(int nb, const double* minv, double* L, int* err, hipStream_t s) {
+  (void)hipGetLastError();
+  if (nb > 0) k_minv_chol<<<grid_for_n(nb), 256, 0, s>>>(nb, minv, L, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_factor_tiles(bool transposed, int nq, int nb, const int* body, const double* v, const double* L,
+                               double* f, float* f32, hipStream_t s) {
+  (void)hipGetLastError();
+  if (nq <= 0) return hipSuccess;
+  if (transposed)
+    k_factor_tiles<true><<<grid_for_n(nq), 256, 0, s>>>(nq, nb, body, v, L, f, f32);
+  else
+    k_factor_tiles<false><<<grid_for_n(nq), 256, 0, s>>>(nq, nb, body, v, L, f, f32);
+  return hipGetLastError();
+}
+
 hipError_t launch_tile_values(int br, int bc, bool transposed, int nq, const int* src, const double* raw,
                               double* vals, float* vals32, hipStream_t s) {
   (void)hipGetLastError();

@@ -14,6 +14,9 @@ hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* 
 hipError_t launch_matfree(const DevProblem<double>& P, const double* x, double* y, double* wbody, double* carry,
                           hipStream_t s);
 // problem set-up on the device (sb_setup.hip)
+hipError_t launch_minv_chol(int nb, const double* minv, double* L, int* err, hipStream_t s);
+hipError_t launch_factor_tiles(bool transposed, int nq, int nb, const int* body, const double* v, const double* L,
+                               double* f, float* f32, hipStream_t s);
 hipError_t launch_tile_values(int br, int bc, bool transposed, int nq, const int* src, const double* raw,
                               double* vals, float* vals32, hipStream_t s);
 hipError_t launch_minv_layout(int nb, const double* raw, double* em, float* em32, hipStream_t s);
