@@ -14,6 +14,11 @@
 #ifndef SB_COHERENT_GATHER
 #define SB_COHERENT_GATHER 0
 #endif
+// 1: reductions whose blocks share one XCD exchange their partial sums through that
+//    XCD's L2 (plain stores, and plain loads after the barrier's L1 invalidation).
+#ifndef SB_XCD_LOCAL_PARTIALS
+#define SB_XCD_LOCAL_PARTIALS 1
+#endif
 
 namespace sb {
 
@@ -120,18 +125,14 @@ __device__ __forceinline__ uint64_t realtime() { return __builtin_amdgcn_s_memre
 // Index of the XCD (accelerator complex die) this wave runs on (HW_REG_XCC_ID).
 __device__ __forceinline__ unsigned xcd_id() { return __builtin_amdgcn_s_getreg((15 << 11) | 20) & 15u; }
 
-// Invalidates this CU's vector L1 only (buffer_inv sc0). After a barrier this is
-// the whole acquire for blocks that share one XCD: the XCD's L2 is their common
-// point of coherence, and the coherent stores of the other blocks have reached it.
-__device__ __forceinline__ void invalidate_l1() { asm volatile("buffer_inv sc0" ::: "memory"); }
-
 struct Barrier {
   GridSync gs;
   unsigned rank = 0;         // this block's index among the gs.nblocks blocks (must be set)
   bool acquire = true;       // invalidate L1/L2 after the barrier (readers use plain loads); false:
                              // readers use coherent loads and the caches are left alone
-  bool one_xcd = false;      // every block of the barrier runs on one XCD: the acquire
-                             // invalidates L1 only and the XCD's L2 keeps its lines
+  bool one_xcd = false;      // every block of the barrier runs on one XCD, whose L2 they
+                             // share: plain stores need no L2 write-back, and the
+                             // reduction's partial sums go through that L2 too
   bool plain_stores = false; // vectors are written with plain stores: unless one_xcd,
                              // the L2 is written back before arriving (release)
   unsigned epoch = 0;
@@ -178,16 +179,13 @@ struct Barrier {
       }
       s_ok = ok;
 #if !SB_COHERENT_GATHER
-      // Acquire: drop this CU's L1 and this XCD's L2 copies of lines written by
-      // other blocks before the barrier (their coherent stores already reached
-      // memory), so that plain cached loads in the next phase read fresh values.
-      // Blocks on one XCD read through the L2 their stores went to: L1 only.
-      if (acquire) {
-        if (one_xcd)
-          invalidate_l1();
-        else
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
+      // Acquire (agent scope): drop this CU's L1 (and stale L2 copies) of lines
+      // written by other blocks before the barrier, so that plain cached loads in
+      // the next phase read fresh values. Blocks on one XCD read through the L2
+      // their plain stores went to. (A workgroup-scope invalidation, buffer_inv sc0,
+      // leaves L1 alone when a workgroup's waves share one CU: a reduction through
+      // plain stores then read stale partial sums.)
+      if (acquire) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #endif
     }
     __syncthreads();
@@ -219,7 +217,12 @@ struct Barrier {
       const int nw = (blockDim.x + SB_WAVE - 1) / SB_WAVE;
       double acc = 0.0;
       for (int q = 0; q < nw; ++q) acc = isMax ? (s_p[k][q] > acc ? s_p[k][q] : acc) : acc + s_p[k][q];
-      cst(&part[(size_t)rank * kMaxPartials + k], acc);
+#if SB_XCD_LOCAL_PARTIALS
+      if (one_xcd)
+        gst(&part[(size_t)rank * kMaxPartials + k], acc);
+      else
+#endif
+        cst(&part[(size_t)rank * kMaxPartials + k], acc);
     }
     if (!sync()) return false;
     // every wave reduces one partial index over all blocks: lane l takes blocks l, l+64,
@@ -233,7 +236,9 @@ struct Barrier {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const unsigned b = b0 + j * SB_WAVE + lane_id();
-          x[j] = (b < gs.nblocks) ? cld(&part[(size_t)b * kMaxPartials + k]) : 0.0;
+          x[j] = (b < gs.nblocks) ? ((SB_XCD_LOCAL_PARTIALS && one_xcd) ? gld(&part[(size_t)b * kMaxPartials + k])
+                                             : cld(&part[(size_t)b * kMaxPartials + k]))
+                                  : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc = isMax ? (x[j] > acc ? x[j] : acc) : acc + x[j];

@@ -550,10 +550,10 @@ __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
   pg_body<MF, T, GridTeam, true>(a.prm, a.pb, tm, tsum);
 }
 
-// Teams of several workgroups are placed on one XCD. They write their vectors with
-// plain stores (kept in the XCD's L2), invalidate only L1 after each barrier, and
-// read what other workgroups wrote with plain loads. A team found spread over
-// several XCDs writes the L2 back before and invalidates it after each barrier.
+// Teams of several workgroups are placed on one XCD. They write their vectors (and
+// their reductions' partial sums) with plain stores, kept in the XCD's L2, and read
+// what other workgroups wrote with plain loads after the barrier's acquire. A team
+// found spread over several XCDs also writes the L2 back before each barrier.
 // Measured on the 512-problem batch: coherent loads without invalidation
 // (SB_BATCH_TEAM_ACQUIRE=0) 2x slower than invalidation; coherent (written-through)
 // stores (SB_BATCH_PLAIN_STORES=0) 16 % slower than plain ones.
@@ -609,9 +609,8 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
   g.timeout_ticks = a.timeout_ticks;
   BatchTeamSel tm(g, rank);
   // Which XCDs the team's blocks run on: one bit each, gathered through the first
-  // barrier. A team on one XCD shares that XCD's L2, and its barriers then leave
-  // the L2 alone (an L2 invalidation also drops the lines of every other team on
-  // the XCD).
+  // barrier. A team on one XCD shares that XCD's L2: its plain stores are visible
+  // to its other blocks without an L2 write-back.
   unsigned* xcds = g.top + 1;
   if (threadIdx.x == 0)
     __hip_atomic_fetch_or(xcds, 1u << xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
