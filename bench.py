@@ -109,10 +109,32 @@ def cpu_baseline_batch(probs, tol: float, max_iters: int, threads: int) -> dict:
             solved += p.nc
             nconv += 1
     dt = time.perf_counter() - t0
+    # one thread, on every 32nd problem
+    O.set_threads(1)
+    small = probs[::32]
+    t1 = time.perf_counter()
+    solved1 = 0
+    for p in small:
+        res, _, _ = O.OracleProblem(p, assemble=False).pg_solve(np.zeros(3 * p.nc), variant=O.PG_APGD, tol=tol,
+                                                               max_iters=max_iters, coulomb=True, assembled=False,
+                                                               diag_scaling=True)
+        solved1 += p.nc if res < tol else 0
+    dt1 = time.perf_counter() - t1
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": solved / dt, "unit": "contact-solves/s", "cores": threads, "kind": "port",
             "sample": f"every 4th problem of the batch ({len(sample)} problems, {sum(p.nc for p in sample)} "
                       f"contacts, {nconv} converged), oracle FP64 APGD matrix-free to FB {tol:g}, "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s",
+            "cpu_model": model,
+            "one_thread": {"value": solved1 / dt1, "unit": "contact-solves/s",
+                           "sample": f"every 32nd problem ({len(small)} problems, {sum(p.nc for p in small)} "
+                                     f"contacts), {dt1:.1f} s"}}
 
 
 def tight_50k(ctx, R, G, cpu_threads: int) -> dict:
