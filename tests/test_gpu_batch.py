@@ -9,6 +9,8 @@ bits; the autotuned geometry gives results of the same quality.
 """
 from __future__ import annotations
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -127,6 +129,21 @@ def test_batch_multi_workgroup_teams(R, O, ctx, small_batch):
     _check_each(O, small_batch, res, B.split(res.r), 1e-6, 5000)
     again = B.solve(**kw)
     assert np.array_equal(again.r, res.r)
+
+
+def test_batch_static_body_problem(R, O, ctx, small_batch):
+    """A problem with a static body (Minv_k = 0, not positive definite) keeps the
+    unfactored matrix-free product inside a batch whose other problems run the
+    factored one; every result is oracle-verified."""
+    print("[batch] static body", flush=True)
+    probs = list(small_batch[:8])
+    probs[3] = dataclasses.replace(probs[3], minv=probs[3].minv.copy())  # the fixture stays unchanged
+    probs[3].minv[0] = 0.0
+    B = R.Batch(ctx, probs)
+    res = B.solve(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-8,
+                  max_iters=5000, precision=R.PRECISION_FP64)
+    assert res.n_converged >= len(probs) - 2
+    _check_each(O, probs, res, B.split(res.r), 1e-8, 5000)
 
 
 def test_batch_team_placement_is_transparent(R, ctx, small_batch):

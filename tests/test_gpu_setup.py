@@ -37,6 +37,10 @@ def test_update_values_matches_fresh_problem(R, G, ctx, on_device):
     p1 = _perturbed(G, p0, 11)
     dp = R.DualFrictionProblem(ctx, p0, assemble=True)
     fresh = R.DualFrictionProblem(ctx, p1, assemble=True)
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-8,
+              max_iters=5000)
+    # a matrix-free solve before the update factors the operator tiles of the old values
+    assert dp.solve_pg(np.zeros(3 * p0.nc), **kw).converged
     if on_device:
         arrs = []
         for a in (p1.hvals, p1.minv, p1.b, p1.mu):
@@ -60,10 +64,10 @@ def test_update_values_matches_fresh_problem(R, G, ctx, on_device):
     assert np.array_equal(wa[0], wb[0]) and np.array_equal(wa[1], wb[1])
     assert np.allclose(wa[2], wb[2], rtol=1e-13, atol=1e-13 * np.abs(wb[2]).max())
     # solves: APGD (device data only) and GS (local matrices from the refreshed diagonal)
-    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-8,
-              max_iters=5000)
     s1, s2 = dp.solve_pg(np.zeros(3 * p0.nc), **kw), fresh.solve_pg(np.zeros(3 * p0.nc), **kw)
     assert s1.converged and s2.converged
+    # solved with the refactored tiles of the new values: the fresh problem agrees
+    assert fresh.eval_with(s1.r, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL) <= 1.05e-8
     assert abs(s1.iterations - s2.iterations) <= max(5, s2.iterations // 4)
     g1 = dp.solve_gs(np.zeros(3 * p0.nc), law=R.LAW_SOC, tol=1e-8, max_iters=2000)
     g2 = fresh.solve_gs(np.zeros(3 * p0.nc), law=R.LAW_SOC, tol=1e-8, max_iters=2000)
