@@ -2120,33 +2120,17 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
   pl.nteams = t0;  // the grid stays whole: blocks no team uses leave at once
 }
 
-// Places the teams on the grid. Workgroups are dispatched to the XCDs round-robin
-// (block b runs on XCD b % 8), so a team of k blocks b = 8 s + x, s = s0 .. s0 + k - 1,
-// shares one XCD's L2 (the batch kernel checks this on the device and falls back
-// to L2-invalidating barriers for a team that does not). Teams are taken in
-// decreasing size, each onto the XCD with the fewest blocks in use: the block
-// counts are powers of two and every XCD's share of the grid is a multiple of the
-// largest, so every team fits. spread: teams on consecutive blocks (each team over
-// several XCDs). bmap[b] = class << 20 | team << 4 | rank, -1 idle.
+// Places the teams on the grid (sbhost::batchBlockMap): the k blocks of a team on
+// one XCD; the batch kernel checks this on the device and falls back to
+// L2-writing-back barriers for a team that is not.
 constexpr int kXcds = 8;
 std::vector<int> batch_block_map(const BatchPlan& pl, bool spread) {
-  int kmax = 1;
-  for (int c = 0; c < pl.nclass; ++c) kmax = std::max(kmax, pl.cls[c].k);
-  const int nx = (!spread && pl.grid % kXcds == 0 && (pl.grid / kXcds) % kmax == 0) ? kXcds : 1;
-  const int per = pl.grid / nx;
-  std::vector<int> bmap(pl.grid, -1), used(nx, 0);
+  int k[sb::kMaxBatchClasses], nt[sb::kMaxBatchClasses];
   for (int c = 0; c < pl.nclass; ++c) {
-    const sb::BatchClass& cl = pl.cls[c];
-    for (int t = 0; t < cl.nteams; ++t) {
-      int x = 0;
-      for (int y = 1; y < nx; ++y)
-        if (used[y] < used[x]) x = y;
-      if (used[x] + cl.k > per) return {};  // the teams need more blocks than the grid has
-      for (int r = 0; r < cl.k; ++r) bmap[(size_t)(used[x] + r) * nx + x] = (c << 20) | (t << 4) | r;
-      used[x] += cl.k;
-    }
+    k[c] = pl.cls[c].k;
+    nt[c] = pl.cls[c].nteams;
   }
-  return bmap;
+  return sbhost::batchBlockMap(pl.grid, kXcds, pl.nclass, k, nt, spread);
 }
 
 int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, BatchPlan& pl,

@@ -387,4 +387,22 @@ void localityOrder(int nc, int nb, const int* hrowptr, const int* hcolidx, std::
   order(k, cperm);
 }
 
+
+std::vector<int> batchBlockMap(int grid, int nxcd, int nclass, const int* k, const int* nteams, bool spread) {
+  int kmax = 1;
+  for (int c = 0; c < nclass; ++c) kmax = std::max(kmax, k[c]);
+  const int nx = (!spread && nxcd > 1 && grid % nxcd == 0 && (grid / nxcd) % kmax == 0) ? nxcd : 1;
+  const int per = grid / nx;
+  std::vector<int> bmap(grid, -1), used(nx, 0);
+  for (int c = 0; c < nclass; ++c)
+    for (int t = 0; t < nteams[c]; ++t) {
+      int x = 0;
+      for (int y = 1; y < nx; ++y)
+        if (used[y] < used[x]) x = y;
+      if (used[x] + k[c] > per) return {};
+      for (int r = 0; r < k[c]; ++r) bmap[(size_t)(used[x] + r) * nx + x] = (c << 20) | (t << 4) | r;
+      used[x] += k[c];
+    }
+  return bmap;
+}
 }  // namespace sbhost

@@ -59,4 +59,15 @@ void coloringFromH(int nc, int nb, const int* hrowptr, const int* hcolidx, int m
 void localityOrder(int nc, int nb, const int* hrowptr, const int* hcolidx, std::vector<int>& cperm,
                    std::vector<int>& bmap);
 
+
+// Places the teams of a batch launch on its grid. Workgroups are dispatched to the
+// XCDs round-robin (block b runs on XCD b % nxcd), so a team of k blocks
+// b = nxcd * s + x, s = s0 .. s0 + k - 1, runs on one XCD. Classes c = 0..nclass-1
+// (team sizes k[c], powers of two in decreasing order, nteams[c] teams each) are
+// placed team by team, each onto the XCD with the fewest blocks in use: every
+// XCD's share of the grid is a multiple of the largest k, so every team fits.
+// Without that (or with spread) teams take consecutive blocks. Returns
+// bmap[b] = c << 20 | team << 4 | rank, -1 for an idle block; empty when the
+// teams need more blocks than the grid has.
+std::vector<int> batchBlockMap(int grid, int nxcd, int nclass, const int* k, const int* nteams, bool spread);
 }  // namespace sbhost

@@ -6,6 +6,7 @@
 // comparisons are bit-exact. Prints "ok <n checks>" and exits 0, or names the
 // first failed check and exits 1.
 #include <algorithm>
+#include <map>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -452,7 +453,66 @@ static void testLocalityOrder() {
     for (int k = 0; k < 3; ++k) CHECK(y2[3 * i + k] == y[3 * cperm[i] + k], "W x differs at contact %d", i);
 }
 
+// Batch block map: every block holds at most one (class, team, rank); every team has
+// all its ranks, on blocks of one XCD (b % 8) when placed per XCD; the XCDs' blocks
+// in use differ by at most the largest team; a spread map places teams on
+// consecutive blocks.
+static void testBatchBlockMap() {
+  struct Case {
+    int grid, nclass;
+    std::vector<int> k, nteams;
+  };
+  const std::vector<Case> cases = {
+      {256, 5, {16, 8, 4, 2, 1}, {3, 5, 7, 9, 100}},
+      {512, 4, {16, 4, 2, 1}, {16, 16, 32, 128}},
+      {256, 1, {1}, {256}},
+      {768, 3, {16, 8, 1}, {20, 20, 200}},
+      {1024, 2, {8, 1}, {64, 400}},
+  };
+  for (const Case& cs : cases) {
+    for (bool spread : {false, true}) {
+      const std::vector<int> bmap =
+          sbhost::batchBlockMap(cs.grid, 8, cs.nclass, cs.k.data(), cs.nteams.data(), spread);
+      CHECK((int)bmap.size() == cs.grid, "grid %d: map size %d", cs.grid, (int)bmap.size());
+      std::map<std::pair<int, int>, std::vector<int>> teams;  // (class, team) -> blocks by rank
+      for (int b = 0; b < (int)bmap.size(); ++b) {
+        if (bmap[b] < 0) continue;
+        const int c = bmap[b] >> 20, t = (bmap[b] >> 4) & 0xffff, r = bmap[b] & 15;
+        CHECK(c < cs.nclass && t < cs.nteams[c] && r < cs.k[c], "block %d: bad code", b);
+        auto& v = teams[{c, t}];
+        if ((int)v.size() < cs.k[c]) v.resize(cs.k[c], -1);
+        CHECK(v[r] < 0, "class %d team %d rank %d placed twice", c, t, r);
+        v[r] = b;
+      }
+      std::vector<int> perXcd(8, 0);
+      for (int c = 0; c < cs.nclass; ++c)
+        for (int t = 0; t < cs.nteams[c]; ++t) {
+          auto it = teams.find({c, t});
+          CHECK(it != teams.end(), "class %d team %d missing", c, t);
+          if (it == teams.end()) continue;
+          const std::vector<int>& v = it->second;
+          for (int r = 0; r < cs.k[c]; ++r) CHECK(v[r] >= 0, "class %d team %d rank %d missing", c, t, r);
+          if (spread) {
+            for (int r = 1; r < cs.k[c]; ++r) CHECK(v[r] == v[0] + r, "spread team not consecutive");
+          } else {
+            for (int r = 1; r < cs.k[c]; ++r) CHECK(v[r] % 8 == v[0] % 8, "team on two XCDs");
+            perXcd[v[0] % 8] += cs.k[c];
+          }
+        }
+      if (!spread) {
+        const int mx = *std::max_element(perXcd.begin(), perXcd.end());
+        const int mn = *std::min_element(perXcd.begin(), perXcd.end());
+        CHECK(mx - mn <= cs.k[0], "XCD blocks in use %d..%d", mn, mx);
+      }
+    }
+  }
+  // more blocks requested than the grid has: empty map
+  const int k1[1] = {16}, n1[1] = {17};
+  CHECK(sbhost::batchBlockMap(256, 8, 1, k1, n1, false).empty(), "overfull map not rejected");
+}
+
 int main() {
+  testBatchBlockMap();
   testLocalityOrder();
   testColoringFromH();
   testTiling();
