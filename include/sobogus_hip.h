@@ -63,7 +63,7 @@ enum sb_law { SB_LAW_SOC = 0, SB_LAW_COULOMB = 1 };
 /* Operator used by the projected-gradient solver. SB_OP_MATRIX_FREE: H (Minv (H^T x)),
    which the solvers apply as F (F^T x) with F = H L, Minv_k = L_k L_k^T (Cholesky per body,
    computed on the device at the first matrix-free solve after the values change); a problem
-   with an Minv_k that is not positive definite keeps the three-step product. */
+   with an Minv_k that is not symmetric positive definite keeps the three-step product. */
 enum sb_operator { SB_OP_ASSEMBLED = 0, SB_OP_MATRIX_FREE = 1 };
 /* Residual row scaling: reference semantics (max(1, tr(W_ii)/3) for an assembled
    W, 1 for the matrix-free product) or max(1, tr(W_ii)/3) in both modes */

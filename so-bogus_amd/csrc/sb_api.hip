@@ -1046,7 +1046,7 @@ int sb_fb_residual(sb_ctx* ctx, const sb_problem* Pc, int law, int op, int scali
 // application is the two tile passes F (F^T x), with no Minv blocks to read and no
 // body step between them, and stays exactly symmetric in FP32. Computed on the device
 // from the current tile values and Minv after the problem's values change; a Minv_k
-// that is not positive definite keeps the unfactored product.
+// that is not symmetric positive definite keeps the unfactored product.
 static int factor_prepare(sb_problem* P) {
   if (P->fVersion == P->version) return SB_OK;
   const size_t nh = 18 * (size_t)P->H.host.nblocks, nht = 18 * (size_t)P->HT.host.nblocks;

@@ -181,8 +181,9 @@ __global__ void __launch_bounds__(128) k_mass_inverse(int nb, const double* __re
 // W = H Minv H^T = F F^T for F_ik = H_ik L_k: the matrix-free product is then the
 // two tile passes F (F^T x) with no body step, exactly symmetric in any precision.
 // k_minv_chol: L_k from the element-major Minv [36][nb] into element-major [36][nb]
-// (upper part zero); a pivot that is not positive (Minv_k not positive definite)
-// sets err.
+// (upper part zero); Minv_k not symmetric (beyond 1e-12 of its largest diagonal
+// entry: the factors use its lower triangle only) or a pivot that is not positive
+// (Minv_k not positive definite) sets err.
 __global__ void __launch_bounds__(256) k_minv_chol(int nb, const double* __restrict__ minv, double* __restrict__ L,
                                                    int* __restrict__ err) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) {
@@ -194,7 +195,14 @@ __global__ void __launch_bounds__(256) k_minv_chol(int nb, const double* __restr
         a[r][c] = minv[(size_t)(r * 6 + c) * nb + k];
         l[r][c] = 0.0;
       }
-    bool ok = true;
+    double dmax = 0.0, asym = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      dmax = fmax(dmax, fabs(a[r][r]));
+#pragma unroll
+      for (int c = 0; c < r; ++c) asym = fmax(asym, fabs(a[r][c] - a[c][r]));
+    }
+    bool ok = asym <= 1e-12 * dmax;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       double d = a[c][c];

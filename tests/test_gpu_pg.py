@@ -90,22 +90,27 @@ def test_matrix_free_equals_assembled_operator(R, O, G, ctx):
     assert g.converged and op.eval(g.r, coulomb=True) <= 1.05e-8
 
 
-@pytest.mark.parametrize("static_body", [False, True])
-def test_matrix_free_factored_operator(R, O, G, ctx, static_body):
+@pytest.mark.parametrize("minv_case", ["spd", "static_body", "asymmetric"])
+def test_matrix_free_factored_operator(R, O, G, ctx, minv_case):
     """The matrix-free solvers apply W = F F^T with F = H L (Minv = L L^T, factored on the
     device); a problem with a body whose Minv is not positive definite (a static body,
-    Minv = 0) keeps the unfactored product H (Minv (H^T x)). Either way the GPU follows
-    the oracle's matrix-free APGD over the first iterations and meets the tolerance."""
-    print(f"case: 600-contact matrix-free APGD, static body {static_body}", flush=True)
+    Minv = 0) or not symmetric keeps the unfactored product H (Minv (H^T x)). Either way
+    the GPU follows the oracle's matrix-free APGD (full Minv blocks) over the first
+    iterations and meets the tolerance."""
+    print(f"case: 600-contact matrix-free APGD, Minv {minv_case}", flush=True)
     prob = G.generate(nc=600, nb=300, seed=31)
-    if static_body:
+    if minv_case == "static_body":
         prob.minv[0] = 0.0
+    elif minv_case == "asymmetric":
+        m = prob.minv[5].reshape(6, 6)
+        m[0, 1] += 0.05 * m[0, 0]  # still positive definite; the lower triangle alone is not Minv
+        prob.minv[5] = m.reshape(36)
     dp = R.DualFrictionProblem(ctx, prob, assemble=False)
     op = O.OracleProblem(prob)
     g = dp.solve_pg(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, tol=1e-8, max_iters=5000,
                     hist_cap=5000)
     assert g.converged
-    assert op.eval(g.r, coulomb=True, diag_scaling=False) <= 1.05e-8
+    assert op.eval(g.r, coulomb=True, diag_scaling=False, assembled=False) <= 1.05e-8
     ores, ox, oinfo = op.pg_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=5000, coulomb=True, assembled=False)
     k = min(20, len(g.history), len(oinfo["history"]))
     assert np.allclose(g.history[:k], oinfo["history"][:k], rtol=1e-6)
