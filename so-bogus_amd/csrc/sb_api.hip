@@ -2044,7 +2044,13 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
     }
     kOf[p] = k;
   }
-  auto costOf = [&](int p) { return (double)B->cost[p] * wIt[p]; };
+  // A team's time per iteration is a latency floor (barriers and dependent memory
+  // round trips, about as long as streaming kLatencyCost stored blocks per
+  // workgroup) plus its share of the problem's blocks: a problem's block-time is
+  // w (k floor + cost), its duration w (floor + cost / k).
+  constexpr double kLatencyCost = 4000.0;
+  auto costOf = [&](int p) { return wIt[p] * ((double)B->cost[p] + kOf[p] * kLatencyCost); };
+  auto durationOf = [&](int p) { return wIt[p] * (kLatencyCost + (double)B->cost[p] / kOf[p]); };
   // classes in decreasing team size
   std::vector<int> ks;
   for (int k = sb::kMaxTeamBlocks; k >= 1; k >>= 1)
@@ -2110,7 +2116,7 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
     std::vector<int> mine;
     for (int p = 0; p < B->n; ++p)
       if (kOf[p] == ks[c]) mine.push_back(p);
-    std::stable_sort(mine.begin(), mine.end(), [&](int a, int b) { return costOf(a) > costOf(b); });
+    std::stable_sort(mine.begin(), mine.end(), [&](int a, int b) { return durationOf(a) > durationOf(b); });
     order.insert(order.end(), mine.begin(), mine.end());
     cl.q1 = (int)order.size();
     cl.team0 = t0;
