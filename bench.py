@@ -280,6 +280,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: tuned mixed batch solved 3 times + calibration copies, no JSON")
+    ap.add_argument("--profile-spmv", action="store_true",
+                    help="short run for rocprofv3: the stand-alone products at 1M contacts (zipf-1m: H x, "
+                         "H^T r + Minv, matrix-free W x; 20 launches each after one warm-up), no JSON")
     ap.add_argument("--profile-config", type=str, default="",
                     help="with --profile-run: solve with this launch shape (threads,blocks_per_cu,contacts_per_block) "
                          "after the autotuner instead of the tuned one, so that separate profiler passes see the "
@@ -296,6 +299,20 @@ def main() -> int:
 
     ctx = R.Context(0)
     log(f"{ctx.arch} {ctx.n_cu} CUs")
+    if args.profile_spmv:
+        p1m = G.workload("zipf-1m")
+        dp1m = R.DualFrictionProblem(ctx, p1m, assemble=False)
+        s = 8
+        hx = 2 * p1m.nc * (18 * s + 4) + 4 * (p1m.nc + 1) + 6 * p1m.nb * s + 3 * p1m.nc * s
+        for kind, name, nbytes in ((1, "H x", hx), (2, "H^T r + Minv", hx + 48 * s * p1m.nb),
+                                   (3, "matrix-free W x", wx_matfree_bytes(p1m.nc, p1m.nb))):
+            ms = dp1m.bench_spmv(kind, reps=20)
+            log(f"profile spmv {name}: {1e3 * ms:.1f} us per launch, {nbytes} compulsory bytes, "
+                f"{nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
+        ctx.bench_copy(COPY_BYTES, reps=2)
+        del dp1m
+        ctx.close()
+        return 0
     t0 = time.perf_counter()
     probs = G.batch(512)
     ncs = np.array([p.nc for p in probs])
