@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <queue>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -2046,10 +2048,9 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
   }
   // A team's time per iteration is a latency floor (barriers and dependent memory
   // round trips, about as long as streaming kLatencyCost stored blocks per
-  // workgroup) plus its share of the problem's blocks: a problem's block-time is
-  // w (k floor + cost), its duration w (floor + cost / k).
+  // workgroup) plus its share of the problem's blocks: a problem's duration on its
+  // team is w (floor + cost / k).
   constexpr double kLatencyCost = 4000.0;
-  auto costOf = [&](int p) { return wIt[p] * ((double)B->cost[p] + kOf[p] * kLatencyCost); };
   auto durationOf = [&](int p) { return wIt[p] * (kLatencyCost + (double)B->cost[p] / kOf[p]); };
   // classes in decreasing team size
   std::vector<int> ks;
@@ -2060,47 +2061,55 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
         break;
       }
   const int C = (int)ks.size();
-  std::vector<double> work(C, 0.0);
   std::vector<int> count(C, 0);
+  std::vector<std::vector<double>> dur(C);
   for (int p = 0; p < B->n; ++p) {
     const int c = (int)(std::find(ks.begin(), ks.end(), kOf[p]) - ks.begin());
-    work[c] += costOf(p);
+    dur[c].push_back(durationOf(p));
     ++count[c];
   }
-  double total = 0;
-  for (double w : work) total += w;
-  // teams per class: proportional share of the blocks, at least one, at most one per problem
-  std::vector<int> teams(C);
+  for (auto& d : dur) std::sort(d.begin(), d.end(), std::greater<double>());
+  // time until class c's queue drains on n teams: longest first, each problem to the
+  // team that frees up first (what the queue does)
+  auto makespan = [&](int c, int n) {
+    std::priority_queue<double, std::vector<double>, std::greater<double>> free;
+    for (int i = 0; i < n; ++i) free.push(0.0);
+    double m = 0.0;
+    for (double d : dur[c]) {
+      const double t = free.top() + d;
+      free.pop();
+      free.push(t);
+      m = std::max(m, t);
+    }
+    return m;
+  };
+  // teams per class: one each (their sizes are distinct powers of two, at most half
+  // the grid, so they fit), then one more at a time for the class that would drain
+  // last while it has problems to spare and blocks remain; blocks left over then go
+  // to the other classes, longest drain first
+  std::vector<int> teams(C, 1);
+  std::vector<double> span(C);
   int used = 0;
   for (int c = 0; c < C; ++c) {
-    const int share = (int)std::floor(G * work[c] / total / ks[c]);
-    teams[c] = std::max(1, std::min(share, count[c]));
-    used += teams[c] * ks[c];
+    used += ks[c];
+    span[c] = makespan(c, 1);
   }
-  while (used > G) {  // shrink the class with the most teams per unit of work
-    int cmax = -1;
-    double best = -1;
-    for (int c = 0; c < C; ++c)
-      if (teams[c] > 1 && teams[c] * ks[c] / work[c] > best) {
-        best = teams[c] * ks[c] / work[c];
-        cmax = c;
-      }
-    if (cmax < 0) break;
-    --teams[cmax];
-    used -= ks[cmax];
+  for (;;) {
+    const int c = (int)(std::max_element(span.begin(), span.end()) - span.begin());
+    if (teams[c] >= count[c] || used + ks[c] > G) break;
+    ++teams[c];
+    used += ks[c];
+    span[c] = makespan(c, teams[c]);
   }
-  for (bool grew = true; grew;) {  // hand leftover blocks to the class with the most work per block
+  for (bool grew = true; grew;) {
     grew = false;
     int cbest = -1;
-    double best = -1;
     for (int c = 0; c < C; ++c)
-      if (teams[c] < count[c] && used + ks[c] <= G && work[c] / (teams[c] * ks[c]) > best) {
-        best = work[c] / (teams[c] * ks[c]);
-        cbest = c;
-      }
+      if (teams[c] < count[c] && used + ks[c] <= G && (cbest < 0 || span[c] > span[cbest])) cbest = c;
     if (cbest >= 0) {
       ++teams[cbest];
       used += ks[cbest];
+      span[cbest] = makespan(cbest, teams[cbest]);
       grew = true;
     }
   }
