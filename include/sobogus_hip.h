@@ -50,7 +50,8 @@ enum sb_status {
   SB_ERR_NO_DEVICE = -4,      /* no usable GPU */
   SB_ERR_UNSUPPORTED = -5,    /* option not available on this path */
   SB_ERR_BARRIER_TIMEOUT = -6,/* a persistent kernel's grid barrier timed out */
-  SB_ERR_NAN = -7             /* non-finite residual */
+  SB_ERR_NAN = -7,            /* non-finite residual */
+  SB_ERR_LIMIT = -8           /* a fixed limit of the entry point is exceeded (sb_status_string says which) */
 };
 
 /* Solver variants (Core/BlockSolvers.fwd.hpp:29-46, projected_gradient::Variant) */
@@ -148,6 +149,10 @@ typedef struct sb_stats {
   int refine_iterations;  /* mixed precision: iterations of the FP64 stage (0: verified at once) */
   int refine_spmvs;       /* mixed precision: FP64 operator applications (verification included) */
   double mixed_residual;  /* mixed precision: residual reached by the FP32 stage */
+  int plain_launch;       /* 1: a persistent kernel of this solve was launched without the cooperative
+                             launch's residency guarantee (the runtime refused it, or SOBOGUS_LAUNCH=regular);
+                             such a grid is sized to the occupancy limit, and a grid that is not resident
+                             ends in SB_ERR_BARRIER_TIMEOUT */
 } sb_stats;
 
 const char* sb_status_string(int status);
@@ -159,6 +164,11 @@ int sb_ctx_create(sb_ctx** ctx, int device);
 int sb_ctx_destroy(sb_ctx* ctx);
 /* Device name and compute-unit count (for logs). */
 int sb_ctx_info(sb_ctx* ctx, char* name, int name_len, int* n_cu);
+/* Persistent-kernel launches on this context since its creation: cooperative (every
+   workgroup resident, guaranteed by the runtime) and plain (a refused cooperative launch,
+   SOBOGUS_LAUNCH=regular, and the batch graphs' occupancy-sized grids). No reference
+   counterpart (the reference runs on the CPU). */
+int sb_ctx_launch_counts(sb_ctx* ctx, int64_t* cooperative, int64_t* plain);
 
 /* ---------------------------------------------------------------- BSR (SparseBlockMatrix) */
 /* Creates a device BSR matrix of nbrows x nbcols blocks of br x bc (br, bc <= 8). With

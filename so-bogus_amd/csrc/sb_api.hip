@@ -136,6 +136,11 @@ struct sb_ctx {
   std::string name;
   int coop = 0;          // device supports cooperative launches
   bool coopLaunch = true; // launch persistent kernels cooperatively (SOBOGUS_LAUNCH=regular: plain launch)
+  // persistent launches since creation: cooperative (residency guaranteed) and plain
+  // (refused cooperative launch, SOBOGUS_LAUNCH=regular, or a batch graph); lastPlain:
+  // the last solve launched at least one kernel plainly
+  int64_t coopLaunches = 0, plainLaunches = 0;
+  bool lastPlain = false;
   DBuf<unsigned> sync;  // group counters, top counter, release words
   DBuf<int> error;
   DBuf<double> partials, out;
@@ -189,6 +194,7 @@ struct sb_problem {
   DBuf<double> fH, fHT;
   DBuf<float> fH32, fHT32;
   DBuf<int> ferr;
+  DBuf<double> fL;  // the Cholesky factors L_k (element-major), kept so a refresh allocates nothing
   int fVersion = -1;
   bool factored = false;
   bool useFactored() const { return fVersion == version && factored; }
@@ -419,22 +425,48 @@ int read_status(sb_ctx* ctx, double* out, int nout) {
 hipError_t launch_persistent(sb_ctx* ctx, const void* fn, int grid, int threads, void** args, hipStream_t s) {
   if (ctx->coopLaunch) {
     const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
-    if (e != hipErrorIllegalState) return e;
+    if (e != hipErrorIllegalState) {
+      if (e == hipSuccess) ++ctx->coopLaunches;
+      return e;
+    }
     (void)hipGetLastError();
   }
-  return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+  const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+  if (e == hipSuccess) {
+    ++ctx->plainLaunches;
+    ctx->lastPlain = true;
+  }
+  return e;
 }
 
-sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
+// Device state of a persistent launch: barrier words (kSyncWords), the sticky error
+// word, reduction partials for up to maxBlocks blocks, and the solver results.
+struct GridState {
+  unsigned* sync = nullptr;
+  int* error = nullptr;
+  double* partials = nullptr;
+  int maxBlocks = 0;
+  double* out = nullptr;  // [32]: two stages' results (single-problem PG solves)
+};
+
+sb::GridSync make_sync(const GridState& st, int blocks) {
   sb::GridSync g;
-  g.group = ctx->sync.p;
-  g.release = ctx->sync.p + sb::kSyncGroups * sb::kSyncStride;
-  g.top = ctx->sync.p + 2 * sb::kSyncGroups * sb::kSyncStride;
-  g.error = ctx->error.p;
-  g.partials = ctx->partials.p;
+  g.group = st.sync;
+  g.release = st.sync + sb::kSyncGroups * sb::kSyncStride;
+  g.top = st.sync + 2 * sb::kSyncGroups * sb::kSyncStride;
+  g.error = st.error;
+  g.partials = st.partials;
   g.nblocks = (unsigned)blocks;
   g.timeout_ticks = 100ull * 1000 * 1000 * 20;  // 20 s at 100 MHz
   return g;
+}
+sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
+  GridState st;
+  st.sync = ctx->sync.p;
+  st.error = ctx->error.p;
+  st.partials = ctx->partials.p;
+  st.maxBlocks = ctx->maxGrid;
+  return make_sync(st, blocks);
 }
 
 int check_pg_options(const sb_pg_options* o, bool hasW) {
@@ -465,14 +497,31 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
   return p;
 }
 
-// Launches one persistent single-problem solver kernel (grid barrier state reset first).
+// Launches one persistent single-problem solver kernel (grid barrier state reset first)
+// with the context's grid state (grown on demand), or with a fixed one (fixed != null:
+// a grid larger than its capacity is refused, nothing is allocated).
 template <typename Args>
-int launch_pg_grid(sb_ctx* ctx, const void* fn, Args* a, int threads, int requested, int workWaves) {
+int launch_pg_grid(sb_ctx* ctx, const GridState* fixed, const void* fn, Args* a, int threads, int requested,
+                   int workWaves) {
   const int grid = grid_for(ctx, fn, threads, requested, workWaves);
-  int rc = ensure_grid_scratch(ctx, grid);
-  if (rc) return rc;
-  a->gs = make_sync(ctx, grid);
-  SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, ctx->stream));
+  GridState st;
+  if (fixed) {
+    if (grid > fixed->maxBlocks) {
+      last_error_string() = "persistent grid of " + std::to_string(grid) + " workgroups exceeds the fixed grid state (" +
+                            std::to_string(fixed->maxBlocks) + ")";
+      return SB_ERR_LIMIT;
+    }
+    st = *fixed;
+  } else {
+    int rc = ensure_grid_scratch(ctx, grid);
+    if (rc) return rc;
+    st.sync = ctx->sync.p;
+    st.error = ctx->error.p;
+    st.partials = ctx->partials.p;
+    st.maxBlocks = ctx->maxGrid;
+  }
+  a->gs = make_sync(st, grid);
+  SB_HIP(sb::launch_reset_sync(st.sync, (int)kSyncWords, st.error, ctx->stream));
   void* args[] = {a};
   SB_HIP(launch_persistent(ctx, fn, grid, threads, args, ctx->stream));
   return SB_OK;
@@ -605,6 +654,8 @@ const char* sb_status_string(int status) {
     case SB_ERR_UNSUPPORTED: return "unsupported option on this path";
     case SB_ERR_BARRIER_TIMEOUT: return "grid barrier timed out in a persistent kernel";
     case SB_ERR_NAN: return "non-finite residual";
+    case SB_ERR_LIMIT: return last_error_string().empty() ? "a fixed limit of the entry point is exceeded"
+                                                          : last_error_string().c_str();
     default: return "unknown status";
   }
 }
@@ -685,6 +736,13 @@ int sb_ctx_destroy(sb_ctx* ctx) {
   hipStream_t s = ctx->stream;
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
+  return SB_OK;
+}
+
+int sb_ctx_launch_counts(sb_ctx* ctx, int64_t* cooperative, int64_t* plain) {
+  if (!ctx) return SB_ERR_INVALID;
+  if (cooperative) *cooperative = ctx->coopLaunches;
+  if (plain) *plain = ctx->plainLaunches;
   return SB_OK;
 }
 
@@ -1049,23 +1107,30 @@ int sb_fb_residual(sb_ctx* ctx, const sb_problem* Pc, int law, int op, int scali
 // body step between them, and stays exactly symmetric in FP32. Computed on the device
 // from the current tile values and Minv after the problem's values change; a Minv_k
 // that is not symmetric positive definite keeps the unfactored product.
-static int factor_prepare(sb_problem* P) {
-  if (P->fVersion == P->version) return SB_OK;
+// Storage of the factored tiles (allocated once per pattern).
+static bool factor_allocated(const sb_problem* P) {
+  return P->fH.n == 18 * (size_t)P->H.host.nblocks && P->fHT.n == 18 * (size_t)P->HT.host.nblocks &&
+         P->fH32.n == P->fH.n && P->fHT32.n == P->fHT.n && P->ferr.n == 1 && P->fL.n == 36 * (size_t)P->nb;
+}
+static int factor_alloc(sb_problem* P) {
   const size_t nh = 18 * (size_t)P->H.host.nblocks, nht = 18 * (size_t)P->HT.host.nblocks;
   if (P->fH.n != nh) SB_HIP(P->fH.alloc(nh));
   if (P->fH32.n != nh) SB_HIP(P->fH32.alloc(nh));
   if (P->fHT.n != nht) SB_HIP(P->fHT.alloc(nht));
   if (P->fHT32.n != nht) SB_HIP(P->fHT32.alloc(nht));
   if (P->ferr.n != 1) SB_HIP(P->ferr.alloc(1));
-  DBuf<double> L;
-  SB_HIP(L.alloc(36 * (size_t)P->nb));
+  if (P->fL.n != 36 * (size_t)P->nb) SB_HIP(P->fL.alloc(36 * (size_t)P->nb));
+  return SB_OK;
+}
+// Recomputes the factored tiles into their storage (no allocation).
+static int factor_refresh(sb_problem* P) {
   hipStream_t s = P->ctx->stream;
   int err = 0;
   SB_HIP(hipMemcpyAsync(P->ferr.p, &err, sizeof(int), hipMemcpyHostToDevice, s));
-  SB_HIP(sb::launch_minv_chol(P->nb, P->d_minv.p, L.p, P->ferr.p, s));
-  SB_HIP(sb::launch_factor_tiles(false, P->H.host.nblocks, P->nb, P->H.bcol.p, P->H.vals.p, L.p, P->fH.p, P->fH32.p,
-                                 s));
-  SB_HIP(sb::launch_factor_tiles(true, P->HT.host.nblocks, P->nb, P->HT.brow.p, P->HT.vals.p, L.p, P->fHT.p,
+  SB_HIP(sb::launch_minv_chol(P->nb, P->d_minv.p, P->fL.p, P->ferr.p, s));
+  SB_HIP(sb::launch_factor_tiles(false, P->H.host.nblocks, P->nb, P->H.bcol.p, P->H.vals.p, P->fL.p, P->fH.p,
+                                 P->fH32.p, s));
+  SB_HIP(sb::launch_factor_tiles(true, P->HT.host.nblocks, P->nb, P->HT.brow.p, P->HT.vals.p, P->fL.p, P->fHT.p,
                                  P->fHT32.p, s));
   SB_HIP(hipMemcpyAsync(&err, P->ferr.p, sizeof(int), hipMemcpyDeviceToHost, s));
   SB_HIP(hipStreamSynchronize(s));
@@ -1073,28 +1138,30 @@ static int factor_prepare(sb_problem* P) {
   P->fVersion = P->version;
   return SB_OK;
 }
+static int factor_prepare(sb_problem* P) {
+  if (P->fVersion == P->version) return SB_OK;
+  int rc = factor_alloc(P);
+  return rc ? rc : factor_refresh(P);
+}
 
-int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, double* r, int on_device,
-                double* residual, sb_stats* st) {
-  int rc = check_ctx(ctx);
-  if (rc) return rc;
-  if (!Pc || !o || !r) return SB_ERR_INVALID;
-  sb_problem* P = const_cast<sb_problem*>(Pc);
-  if ((rc = check_pg_options(o, P->hasW))) return rc;
-  if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
+}  // extern "C"
+
+namespace {
+// The projected-gradient solve of a prepared problem on device vector x (warm start in,
+// result out): one persistent launch per stage (FP32 then FP64 in mixed precision), grid
+// state from the context or the fixed array (fixed != null), results into stats.
+int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int histCap, const GridState* fixed,
+           double* residual, sb_stats* st) {
   hipStream_t s = ctx->stream;
-  const size_t n3 = 3 * (size_t)P->nc;
-  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
-  const int histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
-  if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
   const bool mf = (o->op == SB_OP_MATRIX_FREE);
   const bool mixed = (o->precision == SB_PRECISION_MIXED);
-  if (mf && (rc = factor_prepare(P))) return rc;
   const sb::PGParams prm = make_params(o, mf);
   const int workWaves = mf ? std::max(P->H.host.ntiles, P->HT.host.ntiles) : P->W.host.ntiles;
   const int threads = block_threads_for(o->block_threads, kMaxThreads);
   if (threads < 0) return SB_ERR_INVALID;
-
+  double* dout = fixed ? fixed->out : P->out.p;
+  int rc;
+  ctx->lastPlain = false;
   SB_HIP(hipEventRecord(ctx->ev0, s));
   if (mixed) {
     // FP32 stage to tol * factor (no history: the callback stream is the FP64 one)
@@ -1104,10 +1171,10 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
     a.prm.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
     a.prm.stall_iters = o->mixed_stall_iters;
     P->fill(a.pb, true);
-    a.pb.x_io = P->rio.p;
-    a.pb.out = P->out.p;
+    a.pb.x_io = x;
+    a.pb.out = dout;
     const void* fn = mf ? (const void*)&sb::pg_kernel<true, float> : (const void*)&sb::pg_kernel<false, float>;
-    if ((rc = launch_pg_grid(ctx, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
+    if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
   }
   {
     // FP64 solve; in mixed precision it starts from the FP32 result, so it only
@@ -1116,17 +1183,16 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
     std::memset(&a, 0, sizeof(a));
     a.prm = prm;
     P->fill(a.pb, true);
-    a.pb.x_io = P->rio.p;
-    a.pb.hist = P->hist.p;
+    a.pb.x_io = x;
+    a.pb.hist = histCap > 0 ? P->hist.p : nullptr;
     a.pb.hist_cap = histCap;
-    a.pb.out = P->out.p + 16;
+    a.pb.out = dout + 16;
     const void* fn = mf ? (const void*)&sb::pg_kernel<true, double> : (const void*)&sb::pg_kernel<false, double>;
-    if ((rc = launch_pg_grid(ctx, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
+    if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
   }
   SB_HIP(hipEventRecord(ctx->ev1, s));
   double out[32];
-  SB_HIP(hipMemcpyAsync(out, P->out.p, sizeof(out), hipMemcpyDeviceToHost, s));
-  if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
+  SB_HIP(hipMemcpyAsync(out, dout, sizeof(out), hipMemcpyDeviceToHost, s));
   SB_HIP(hipStreamSynchronize(s));
   const double* o32 = out;
   const double* o64 = out + 16;
@@ -1146,6 +1212,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
     st->refine_iterations = mixed ? (int)o64[1] : 0;
     st->refine_spmvs = mixed ? (int)o64[2] : 0;
     st->mixed_residual = mixed ? o32[0] : 0.0;
+    st->plain_launch = ctx->lastPlain ? 1 : 0;
     if (histCap > 0) {
       const int hl = std::min(st->history_length, histCap);
       if (hl > 0) SB_HIP(hipMemcpy(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost));
@@ -1155,6 +1222,30 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   if (o64[6] != 0.0 || (mixed && o32[6] != 0.0)) return SB_ERR_BARRIER_TIMEOUT;
   if (!std::isfinite(o64[0])) return SB_ERR_NAN;
   return o64[5] != 0.0 ? SB_OK : SB_NOT_CONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, double* r, int on_device,
+                double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if ((rc = check_pg_options(o, P->hasW))) return rc;
+  if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
+  const size_t n3 = 3 * (size_t)P->nc;
+  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
+  const int histCap = (st && st->history && st->history_capacity > 0) ? st->history_capacity : 0;
+  if (histCap > 0 && P->hist.n < (size_t)histCap) SB_HIP(P->hist.alloc(histCap));
+  if (o->op == SB_OP_MATRIX_FREE && (rc = factor_prepare(P))) return rc;
+  const int prc = pg_run(ctx, P, o, P->rio.p, histCap, nullptr, residual, st);
+  if (prc < 0 && prc != SB_ERR_BARRIER_TIMEOUT && prc != SB_ERR_NAN) return prc;
+  if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
+  SB_HIP(hipStreamSynchronize(ctx->stream));
+  return prc;
 }
 
 // Set-up shared by the two Gauss-Seidel solvers: options, color offsets, local
@@ -1230,6 +1321,7 @@ static int gs_run(sb_ctx* ctx, sb_problem* P, const void* fn, int grid, int thre
                   int on_device, double* residual, sb_stats* st, int histCap) {
   hipStream_t s = ctx->stream;
   SB_HIP(sb::launch_reset_sync(ctx->sync.p, (int)kSyncWords, ctx->error.p, s));
+  ctx->lastPlain = false;
   SB_HIP(hipEventRecord(ctx->ev0, s));
   SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
@@ -1249,6 +1341,10 @@ static int gs_run(sb_ctx* ctx, sb_problem* P, const void* fn, int grid, int thre
     st->residual = out[0];
     st->device_ms = ms;
     st->history_length = (int)out[4];
+    st->refine_iterations = 0;
+    st->refine_spmvs = 0;
+    st->mixed_residual = 0.0;
+    st->plain_launch = ctx->lastPlain ? 1 : 0;
     if (histCap > 0) {
       const int hl = std::min(st->history_length, histCap);
       if (hl > 0) SB_HIP(hipMemcpyAsync(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost, s));
@@ -1429,10 +1525,14 @@ static int cadoux_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* po, const
                        (po->scaling == SB_SCALING_REFERENCE && op == SB_OP_ASSEMBLED);
   const int infNorm = gs ? go->inf_norm : po->inf_norm;
   const double tol = gs ? go->tol : po->tol;
+  // Host copies first: the inner solves run while d_b holds an inner problem's affine
+  // term, and a host refresh then (the GS set-up reads the host copies) would take
+  // that term for the problem's b.
+  int rc;
+  if ((rc = sync_host(P))) return rc;
   for (auto& b : P->cad) SB_HIP(b.alloc(n3));
   double *b0 = P->cad[0].p, *cx = P->cad[1].p, *cs = P->cad[2].p, *cw = P->cad[3].p, *cu = P->cad[4].p;
   SB_HIP(hipMemcpyAsync(b0, P->d_b.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  int rc;
   if ((rc = copy_in(ctx, cx, x, n3, on_device))) return rc;
   const DevProblem<double> V = P->view(true);
   int spmvs = 0;
@@ -1458,6 +1558,7 @@ static int cadoux_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* po, const
     res = acc;
     return std::isfinite(acc) ? SB_OK : SB_ERR_NAN;
   };
+  bool anyPlain = false;
   // the inner SOC solve of xi at tolerance t, with d_b already set; b restored after
   auto inner = [&](double* xi, double t) -> int {
     sb_stats is;
@@ -1476,6 +1577,7 @@ static int cadoux_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* po, const
       irc = sb_pg_solve(ctx, P, &g, xi, 1, &r, &is);
     }
     spmvs += is.spmvs + is.refine_spmvs;
+    anyPlain |= is.plain_launch != 0;
     const hipError_t e = hipMemcpyAsync(P->d_b.p, b0, n3 * sizeof(double), hipMemcpyDeviceToDevice, s);
     if (irc < 0) return irc;
     return e == hipSuccess ? SB_OK : SB_ERR_HIP;
@@ -1538,6 +1640,10 @@ static int cadoux_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* po, const
     st->residual = res;
     st->device_ms = ms;
     st->history_length = (int)hist.size();
+    st->refine_iterations = 0;
+    st->refine_spmvs = 0;
+    st->mixed_residual = 0.0;
+    st->plain_launch = anyPlain ? 1 : 0;
     if (st->history && st->history_capacity > 0)
       for (int k = 0; k < st->history_capacity && k < (int)hist.size(); ++k) st->history[k] = hist[k];
   }
@@ -2290,6 +2396,7 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
   }
   SB_HIP(hipEventRecord(ctx->ev0, s));
   SB_HIP(hipGraphLaunch(B->exec, s));
+  ctx->plainLaunches += pl.mixed ? 2 : 1;  // occupancy-sized grids in the graph, launched plainly
   SB_HIP(hipEventRecord(ctx->ev1, s));
   std::vector<double> outs((size_t)32 * B->n);
   int err = 0;

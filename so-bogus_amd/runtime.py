@@ -21,7 +21,8 @@ HEADER = PKG_DIR.parent / "include" / "sobogus_hip.h"
 
 SB_OK, SB_NOT_CONVERGED = 0, 1
 SB_ERR = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "no device", -5: "unsupported",
-          -6: "barrier timeout", -7: "non-finite residual"}
+          -6: "barrier timeout", -7: "non-finite residual", -8: "fixed limit exceeded"}
+SB_ERR_INVALID, SB_ERR_LIMIT = -1, -8
 PG_STANDARD, PG_DESCENT, PG_CONJUGATED, PG_APGD, PG_SPG = range(5)
 PURE_NEWTON, PURE_ENUMERATIVE, HYBRID, REV_HYBRID = range(4)
 LAW_SOC, LAW_COULOMB = 0, 1
@@ -38,7 +39,7 @@ PROBLEM_ASSEMBLE_W = 1
 # Every extern "C" symbol of include/sobogus_hip.h
 EXPORTED = [
     "sb_status_string", "sb_pg_options_default", "sb_gs_options_default", "sb_ctx_create", "sb_ctx_destroy",
-    "sb_ctx_info", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
+    "sb_ctx_info", "sb_ctx_launch_counts", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_problem_update_values",
@@ -69,7 +70,7 @@ class Stats(C.Structure):
                 ("converged", C.c_int), ("local_failures", C.c_int), ("residual", C.c_double),
                 ("device_ms", C.c_double), ("history", C.POINTER(C.c_double)), ("history_capacity", C.c_int),
                 ("history_length", C.c_int), ("refine_iterations", C.c_int), ("refine_spmvs", C.c_int),
-                ("mixed_residual", C.c_double)]
+                ("mixed_residual", C.c_double), ("plain_launch", C.c_int)]
 
 
 class AdjointOptions(C.Structure):
@@ -131,6 +132,7 @@ def lib():
         "sb_ctx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
         "sb_ctx_destroy": (C.c_int, [_vp]),
         "sb_ctx_info": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+        "sb_ctx_launch_counts": (C.c_int, [_vp, _lp, _lp]),
         "sb_bsr_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
         "sb_bsr_destroy": (C.c_int, [_vp]),
         "sb_bsr_multiply": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int]),
@@ -249,6 +251,12 @@ class Context:
         _check(lib().sb_bench_copy(self.h, int(nbytes), reps, C.byref(ms)), "sb_bench_copy")
         return ms.value
 
+    def launch_counts(self) -> dict:
+        """Persistent-kernel launches since the context was created, by launch mode."""
+        co, pl = C.c_int64(), C.c_int64()
+        _check(lib().sb_ctx_launch_counts(self.h, C.byref(co), C.byref(pl)), "sb_ctx_launch_counts")
+        return {"cooperative": co.value, "plain": pl.value}
+
     def last_phase_times(self) -> dict:
         """Phase-time breakdown (us) of the last projected-gradient solve."""
         out = np.zeros(4)
@@ -359,6 +367,7 @@ class SolveResult:
     refine_iterations: int = 0
     refine_spmvs: int = 0
     mixed_residual: float = 0.0
+    plain_launch: bool = False
 
 
 class DualFrictionProblem:
@@ -437,7 +446,7 @@ class DualFrictionProblem:
                            local_failures=st.local_failures, device_ms=st.device_ms,
                            history=hist[:min(st.history_length, hist_cap)].copy(),
                            refine_iterations=st.refine_iterations, refine_spmvs=st.refine_spmvs,
-                           mixed_residual=st.mixed_residual)
+                           mixed_residual=st.mixed_residual, plain_launch=bool(st.plain_launch))
 
     def solve_pg(self, r0, opts: PGOptions | None = None, on_device=False, hist_cap=0, **kw) -> SolveResult:
         """ProjectedGradient::solve through DualFrictionProblem::solveWith(pg, r, staticProblem)."""

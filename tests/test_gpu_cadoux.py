@@ -94,3 +94,26 @@ def test_cadoux_restores_b(R, G, ctx):
     before = dp.eval_with(r, law=R.LAW_COULOMB)
     dp.solve_cadoux(np.zeros(3 * prob.nc), solver="pg", iterations=3, tol=1e-6)
     assert dp.eval_with(r, law=R.LAW_COULOMB) == before
+
+
+def test_cadoux_gs_after_device_update_keeps_b(R, G, ctx):
+    """After a device-side value update the host copies lag the device; a GS Cadoux
+    solve (whose set-up reads the host copies) must not take an inner problem's
+    affine term for the problem's b: a later host-side update of mu alone uploads
+    the caller's b, and residuals match a freshly created problem's."""
+    print("case: Cadoux GS after sb_problem_update_values, then sb_problem_update(mu)", flush=True)
+    import dataclasses
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    newb = prob.b * 0.75 + 0.1
+    dp.update_values(b=newb)
+    dp.solve_cadoux(np.zeros(3 * prob.nc), solver="gs", iterations=3, tol=1e-6, max_iters=50)
+    newmu = prob.mu * 0.9
+    dp.update(mu=newmu)
+    fresh = R.DualFrictionProblem(ctx, dataclasses.replace(prob, b=newb, mu=newmu))
+    r = np.random.default_rng(4).standard_normal(3 * prob.nc) * 1e-2
+    for law in (R.LAW_COULOMB, R.LAW_SOC):
+        assert dp.eval_with(r, law=law) == fresh.eval_with(r, law=law)
+    # u = W r + b from the residual at r = 0 on the SOC law sees b alone
+    assert dp.eval_with(np.zeros(3 * prob.nc), law=R.LAW_SOC) == fresh.eval_with(np.zeros(3 * prob.nc),
+                                                                                  law=R.LAW_SOC)
