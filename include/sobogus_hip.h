@@ -216,6 +216,21 @@ int sb_fb_residual(sb_ctx* ctx, const sb_problem* P, int law, int op, int scalin
 /* ProjectedGradient::solve on the problem; r is warm start and result (host or device). */
 int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, double* r, int on_device,
                 double* residual, sb_stats* stats);
+/* sb_pg_solve on a device-resident r that allocates nothing: no hipMalloc, hipFree or other
+   allocating call. No counterpart in the reference; it is for callers that cannot allocate
+   device memory (tight serving or simulation loops, harnesses that forbid allocation). The
+   persistent launches' grid state (barrier words, error word, reduction partials, solver
+   results; about 400 KB) lives in a fixed array of the library, one per GPU, held until the
+   process ends, instead of the context's on-demand buffers; the iterates use the problem's
+   own workspace (allocated by sb_problem_create). Limits: a persistent grid of at most 2048
+   workgroups (SB_ERR_LIMIT beyond; the automatic grid is one workgroup per CU); no residual
+   history (stats->history must be NULL); a matrix-free solve needs the factored operator
+   tiles allocated by one earlier sb_pg_solve with SB_OP_MATRIX_FREE on this problem
+   (SB_ERR_LIMIT otherwise; after sb_problem_update_values they are recomputed in place).
+   Calls that overlap on one GPU (several host threads or contexts at once) race on the
+   shared array: one call at a time per GPU. Results equal sb_pg_solve's bit for bit. */
+int sb_pg_solve_noalloc(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, double* r_dev,
+                        double* residual, sb_stats* stats);
 /* GaussSeidel::solve on an assembled problem whose contacts are color-contiguous:
    colors holds n_colors + 1 offsets; a color must hold no two coupled contacts (W_ij != 0),
    else SB_ERR_INVALID. With colors NULL the contacts are swept in their order (the

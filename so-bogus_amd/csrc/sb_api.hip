@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -449,6 +450,39 @@ struct GridState {
   double* out = nullptr;  // [32]: two stages' results (single-problem PG solves)
 };
 
+// Fixed-size device array for sb_pg_solve_noalloc: the grid state of its persistent
+// launches (barrier words, error word, reduction partials of up to kNoAllocBlocks
+// workgroups, the two stages' results). Every GPU holds its own copy until the process
+// ends; host code reaches it only through the per-device address from
+// hipGetSymbolAddress (the array's name in host code is not a device address).
+constexpr int kNoAllocBlocks = 2048;
+constexpr size_t kNoAllocSyncBytes = ((kSyncWords * sizeof(unsigned) + 255) / 256) * 256;
+constexpr size_t kNoAllocErrorOff = kNoAllocSyncBytes;
+constexpr size_t kNoAllocOutOff = kNoAllocErrorOff + 256;          // 32 doubles: 256 bytes
+constexpr size_t kNoAllocPartialsOff = kNoAllocOutOff + 32 * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocPartialsOff + (size_t)2 * kNoAllocBlocks * sb::kMaxPartials * sizeof(double);
+static_assert(kNoAllocPartialsOff % 256 == 0, "scratch regions stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratchMutex;
+std::vector<void*> g_scratchPtr;  // per device: address of scratch (null: not resolved)
+
+// Resolves this device's scratch address once; the device must be current.
+int resolve_scratch(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if ((int)g_scratchPtr.size() <= device) g_scratchPtr.resize(device + 1, nullptr);
+  if (!g_scratchPtr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      last_error_string() = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return SB_ERR_HIP;
+    }
+    g_scratchPtr[device] = ptr;
+  }
+  *out = g_scratchPtr[device];
+  return SB_OK;
+}
 sb::GridSync make_sync(const GridState& st, int blocks) {
   sb::GridSync g;
   g.group = st.sync;
@@ -711,6 +745,11 @@ int sb_ctx_create(sb_ctx** out, int device) {
   c->ncu = prop.multiProcessorCount;
   c->name = prop.gcnArchName;
   SB_HIP(hipDeviceGetAttribute(&c->coop, hipDeviceAttributeCooperativeLaunch, device));
+  {
+    void* scratchPtr = nullptr;  // sb_pg_solve_noalloc's grid state on this device
+    const int rc = resolve_scratch(device, &scratchPtr);
+    if (rc != SB_OK) return rc;
+  }
   // A plain launch of the same (co-resident sized) grid serves profilers that do
   // not handle cooperative launches; the barriers' bounded waits report a grid
   // that is not fully resident as SB_ERR_BARRIER_TIMEOUT instead of hanging.
@@ -1246,6 +1285,37 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   if ((rc = copy_out(ctx, r, P->rio.p, n3, on_device))) return rc;
   SB_HIP(hipStreamSynchronize(ctx->stream));
   return prc;
+}
+
+int sb_pg_solve_noalloc(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, double* r_dev,
+                        double* residual, sb_stats* st) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !o || !r_dev) return SB_ERR_INVALID;
+  if (st && st->history && st->history_capacity > 0) return SB_ERR_INVALID;  // no history buffer here
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if ((rc = check_pg_options(o, P->hasW))) return rc;
+  if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
+  if (o->op == SB_OP_MATRIX_FREE && P->fVersion != P->version) {
+    // the factored tiles are refreshed in place; their storage comes from an earlier
+    // matrix-free sb_pg_solve on this problem
+    if (!factor_allocated(P)) {
+      last_error_string() = "the matrix-free operator's factored tiles are not allocated: solve once with sb_pg_solve "
+                            "and SB_OP_MATRIX_FREE first";
+      return SB_ERR_LIMIT;
+    }
+    if ((rc = factor_refresh(P))) return rc;
+  }
+  void* base = nullptr;
+  if ((rc = resolve_scratch(ctx->device, &base))) return rc;
+  unsigned char* b = static_cast<unsigned char*>(base);
+  GridState fixed;
+  fixed.sync = reinterpret_cast<unsigned*>(b);
+  fixed.error = reinterpret_cast<int*>(b + kNoAllocErrorOff);
+  fixed.out = reinterpret_cast<double*>(b + kNoAllocOutOff);
+  fixed.partials = reinterpret_cast<double*>(b + kNoAllocPartialsOff);
+  fixed.maxBlocks = kNoAllocBlocks;
+  return pg_run(ctx, P, o, r_dev, 0, &fixed, residual, st);
 }
 
 // Set-up shared by the two Gauss-Seidel solvers: options, color offsets, local

@@ -42,6 +42,7 @@ EXPORTED = [
     "sb_ctx_info", "sb_ctx_launch_counts", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
     "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
+    "sb_pg_solve_noalloc",
     "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_problem_update_values",
     "sb_mass_inverse", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
@@ -148,6 +149,7 @@ def lib():
         "sb_problem_apply_w": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int]),
         "sb_fb_residual": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _dp, C.c_int]),
         "sb_pg_solve": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), _vp, C.c_int, _dp, C.POINTER(Stats)]),
+        "sb_pg_solve_noalloc": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), _vp, _dp, C.POINTER(Stats)]),
         "sb_gs_solve": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, _vp, C.c_int, _dp,
                                   C.POINTER(Stats)]),
         "sb_pgs_solve": (C.c_int, [_vp, _vp, C.POINTER(GSOptions), _lp, C.c_int, _vp, C.c_int, _dp,
@@ -453,6 +455,13 @@ class DualFrictionProblem:
         o = opts if opts is not None else pg_options(**kw)
         fn = lambda rp, dev, res, st: lib().sb_pg_solve(self.ctx.h, self.h, C.byref(o), rp, dev, res, st)
         return self._run(fn, r0, on_device, hist_cap)
+
+    def solve_pg_noalloc(self, r: "DeviceArray", opts: PGOptions | None = None, **kw) -> SolveResult:
+        """sb_pg_solve_noalloc: the same solve on a device-resident r, allocating nothing
+        (grid state in the library's fixed per-GPU array; no residual history)."""
+        o = opts if opts is not None else pg_options(**kw)
+        fn = lambda rp, dev, res, st: lib().sb_pg_solve_noalloc(self.ctx.h, self.h, C.byref(o), rp, res, st)
+        return self._run(fn, r, True, 0)
 
     def solve_gs(self, r0, opts: GSOptions | None = None, colors=None, on_device=False, hist_cap=0,
                  **kw) -> SolveResult:
