@@ -153,6 +153,8 @@ typedef struct sb_stats {
                              launch's residency guarantee (the runtime refused it, or SOBOGUS_LAUNCH=regular);
                              such a grid is sized to the occupancy limit, and a grid that is not resident
                              ends in SB_ERR_BARRIER_TIMEOUT */
+  double mixed_device_ms; /* mixed precision: kernel time of the FP32 stage (HIP events); the FP64 stage
+                             took device_ms - mixed_device_ms */
 } sb_stats;
 
 const char* sb_status_string(int status);
@@ -388,7 +390,10 @@ int sb_memset_device(sb_ctx* ctx, void* dst, int value, size_t bytes);
 /* ---------------------------------------------------------------- benchmarking hooks
    Times `reps` back-to-back launches of one block-SpMV kernel of the problem on the
    context's stream with HIP events; kind: 0 = assembled W x, 1 = H w, 2 = H^T r (+ Minv),
-   3 = matrix-free W x (H^T+Minv, then H). Returns the mean milliseconds per launch. */
+   3 = matrix-free W x (H^T+Minv, then H). Returns the mean milliseconds per launch. With
+   SB_BENCH_FLUSH_CACHES or-ed into kind, every launch is timed alone after a 768 MB copy that
+   evicts its data from L2 and the Infinity Cache (the product then reads it from HBM). */
+#define SB_BENCH_FLUSH_CACHES 16
 int sb_bench_spmv(sb_ctx* ctx, const sb_problem* P, int kind, int reps, double* ms_per_launch);
 /* Times `reps` launches of a plain device-to-device copy of `bytes` (read + write
    traffic 2 * bytes per launch): achievable-bandwidth and counter calibration. */

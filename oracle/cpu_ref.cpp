@@ -1445,42 +1445,62 @@ double gsSolve(const Problem& P, const GSOptions& o, const std::vector<int>& col
   double ndxRef = 0;
   const double absSkipTol = std::min(o.skipTol, o.tol);
   const double absSkipIters = std::min(o.skipIters, (unsigned)std::sqrt((double)n));
+  // With the per-color ndxRef and colors that hold no coupled pair, the contacts of a
+  // color are independent of each other: they are swept in parallel, as the
+  // reference's OpenMP loop over a color (GaussSeidel.impl.hpp:77-86), with results
+  // identical to the sequential sweep.
+  bool parallelColors = o.ndxRefPerColor != 0;
+  if (parallelColors) {
+    std::vector<int> colorOf(n);
+    for (size_t c = 0; c + 1 < colors.size(); ++c)
+      for (int i = colors[c]; i < colors[c + 1]; ++i) colorOf[i] = (int)c;
+    for (int i = 0; i < n && parallelColors; ++i)
+      for (int p = P.W.rowptr[i]; p < P.W.rowptr[i + 1]; ++p)
+        if (P.W.colidx[p] != i && colorOf[P.W.colidx[p]] == colorOf[i]) parallelColors = false;
+  }
+  // one contact of the sweep; ndxMax: the per-color running maximum (ndxRefPerColor)
+  auto sweepContact = [&](int i, double ndxRefColor, double& ndxMax) {
+    if (skip[i]) {
+      --skip[i];
+      return;
+    }
+    double lx[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    double lb[3];
+    for (int k = 0; k < 3; ++k) lb[k] = w[3 * i + k] - reg[i] * lx[k];
+    splitRowSym(P, i, x.data(), lb);
+    double ldx[3] = {-lx[0], -lx[1], -lx[2]};
+    const double(*A)[3] = reinterpret_cast<const double(*)[3]>(&local[(size_t)i * 9]);
+    const double lres = localSolve(coulomb, strat, A, lb, lx, P.mu[i], localTol, scaling[i]);
+    const bool ok = localTol > lres;
+    for (int k = 0; k < 3; ++k) ldx[k] += lx[k];
+    if (!ok)
+      for (int k = 0; k < 3; ++k) ldx[k] *= .5;
+    for (int k = 0; k < 3; ++k) x[3 * i + k] += ldx[k];
+    const double s2 = scaling[i] * scaling[i];
+    const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
+    const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
+    double ref;
+    if (o.ndxRefPerColor) {
+      if (ndx2 > ndxMax) ndxMax = ndx2;
+      ref = ndxRefColor;
+    } else {
+      if (ndx2 > ndxRef) ndxRef = ndx2;
+      ref = ndxRef;
+    }
+    if (std::min(nx2, ndx2) < absSkipTol || ndx2 < o.skipTol * std::min(nx2, ref))
+      skip[i] = (unsigned char)absSkipIters;
+  };
   unsigned it;
   for (it = 1; it <= o.maxIters; ++it) {
     // innerLoop (GaussSeidel.impl.hpp:62-124), identity permutation within contiguous colors
     for (size_t c = 0; c + 1 < colors.size(); ++c) {
       const double ndxRefColor = ndxRef;
       double ndxRefNew = ndxRef;
-      for (int i = colors[c]; i < colors[c + 1]; ++i) {
-        if (skip[i]) {
-          --skip[i];
-          continue;
-        }
-        double lx[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-        double lb[3];
-        for (int k = 0; k < 3; ++k) lb[k] = w[3 * i + k] - reg[i] * lx[k];
-        splitRowSym(P, i, x.data(), lb);
-        double ldx[3] = {-lx[0], -lx[1], -lx[2]};
-        const double(*A)[3] = reinterpret_cast<const double(*)[3]>(&local[(size_t)i * 9]);
-        const double lres = localSolve(coulomb, strat, A, lb, lx, P.mu[i], localTol, scaling[i]);
-        const bool ok = localTol > lres;
-        for (int k = 0; k < 3; ++k) ldx[k] += lx[k];
-        if (!ok)
-          for (int k = 0; k < 3; ++k) ldx[k] *= .5;
-        for (int k = 0; k < 3; ++k) x[3 * i + k] += ldx[k];
-        const double s2 = scaling[i] * scaling[i];
-        const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
-        const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
-        double ref;
-        if (o.ndxRefPerColor) {
-          if (ndx2 > ndxRefNew) ndxRefNew = ndx2;
-          ref = ndxRefColor;
-        } else {
-          if (ndx2 > ndxRef) ndxRef = ndx2;
-          ref = ndxRef;
-        }
-        if (std::min(nx2, ndx2) < absSkipTol || ndx2 < o.skipTol * std::min(nx2, ref))
-          skip[i] = (unsigned char)absSkipIters;
+      if (parallelColors) {
+#pragma omp parallel for reduction(max : ndxRefNew) schedule(dynamic, 64)
+        for (int i = colors[c]; i < colors[c + 1]; ++i) sweepContact(i, ndxRefColor, ndxRefNew);
+      } else {
+        for (int i = colors[c]; i < colors[c + 1]; ++i) sweepContact(i, ndxRefColor, ndxRefNew);
       }
       if (o.ndxRefPerColor) ndxRef = ndxRefNew;
     }

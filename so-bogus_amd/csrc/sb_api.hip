@@ -147,7 +147,7 @@ struct sb_ctx {
   DBuf<double> partials, out;
   int maxGrid = 0;  // capacity of partials (blocks)
   double phase_us[4] = {0, 0, 0, 0};  // last PG solve: operator work/wait, update work/wait
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;  // evm: between the stages of a mixed solve
 };
 
 struct sb_bsr {
@@ -760,6 +760,7 @@ int sb_ctx_create(sb_ctx** out, int device) {
   SB_HIP(c->out.alloc(16));
   SB_HIP(hipEventCreate(&c->ev0));
   SB_HIP(hipEventCreate(&c->ev1));
+  SB_HIP(hipEventCreate(&c->evm));
   int rc = ensure_grid_scratch(c.get(), std::max(1, c->ncu * 8));
   if (rc != SB_OK) return rc;
   *out = c.release();
@@ -772,6 +773,7 @@ int sb_ctx_destroy(sb_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->evm) (void)hipEventDestroy(ctx->evm);
   hipStream_t s = ctx->stream;
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
@@ -1214,6 +1216,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     a.pb.out = dout;
     const void* fn = mf ? (const void*)&sb::pg_kernel<true, float> : (const void*)&sb::pg_kernel<false, float>;
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
+    SB_HIP(hipEventRecord(ctx->evm, s));
   }
   {
     // FP64 solve; in mixed precision it starts from the FP32 result, so it only
@@ -1252,6 +1255,12 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     st->refine_spmvs = mixed ? (int)o64[2] : 0;
     st->mixed_residual = mixed ? o32[0] : 0.0;
     st->plain_launch = ctx->lastPlain ? 1 : 0;
+    st->mixed_device_ms = 0.0;
+    if (mixed) {
+      float m32 = 0.f;
+      (void)hipEventElapsedTime(&m32, ctx->ev0, ctx->evm);
+      st->mixed_device_ms = m32;
+    }
     if (histCap > 0) {
       const int hl = std::min(st->history_length, histCap);
       if (hl > 0) SB_HIP(hipMemcpy(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost));
@@ -1415,6 +1424,7 @@ static int gs_run(sb_ctx* ctx, sb_problem* P, const void* fn, int grid, int thre
     st->refine_spmvs = 0;
     st->mixed_residual = 0.0;
     st->plain_launch = ctx->lastPlain ? 1 : 0;
+    st->mixed_device_ms = 0.0;
     if (histCap > 0) {
       const int hl = std::min(st->history_length, histCap);
       if (hl > 0) SB_HIP(hipMemcpyAsync(st->history, P->hist.p, sizeof(double) * hl, hipMemcpyDeviceToHost, s));
@@ -1714,6 +1724,7 @@ static int cadoux_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* po, const
     st->refine_spmvs = 0;
     st->mixed_residual = 0.0;
     st->plain_launch = anyPlain ? 1 : 0;
+    st->mixed_device_ms = 0.0;
     if (st->history && st->history_capacity > 0)
       for (int k = 0; k < st->history_capacity && k < (int)hist.size(); ++k) st->history[k] = hist[k];
   }
@@ -2054,6 +2065,8 @@ int sb_last_phase_times(sb_ctx* ctx, double* us4) {
 int sb_bench_spmv(sb_ctx* ctx, const sb_problem* Pc, int kind, int reps, double* ms) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
+  const bool flush = (kind & SB_BENCH_FLUSH_CACHES) != 0;
+  kind &= ~SB_BENCH_FLUSH_CACHES;
   if (!Pc || !ms || reps < 1 || kind < 0 || kind > 3) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (kind == 0 && !P->hasW) return SB_ERR_INVALID;
@@ -2076,6 +2089,28 @@ int sb_bench_spmv(sb_ctx* ctx, const sb_problem* Pc, int kind, int reps, double*
     }
   };
   SB_HIP(launch());  // warm-up
+  if (flush) {
+    // every launch timed alone after a copy of 2 x 384 MB, more than the 256 MB
+    // Infinity Cache holds: the launch finds none of its data in the caches
+    const size_t fbytes = (size_t)384 << 20;
+    DBuf<char> fa, fb;
+    SB_HIP(fa.alloc(fbytes));
+    SB_HIP(fb.alloc(fbytes));
+    SB_HIP(hipMemsetAsync(fa.p, 0, fbytes, s));
+    double total = 0.0;
+    for (int k = 0; k < reps; ++k) {
+      SB_HIP(sb::launch_copy(fa.p, fb.p, fbytes, s));
+      SB_HIP(hipEventRecord(ctx->ev0, s));
+      SB_HIP(launch());
+      SB_HIP(hipEventRecord(ctx->ev1, s));
+      SB_HIP(hipEventSynchronize(ctx->ev1));
+      float t = 0.f;
+      SB_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+      total += t;
+    }
+    *ms = total / reps;
+    return SB_OK;
+  }
   SB_HIP(hipEventRecord(ctx->ev0, s));
   for (int k = 0; k < reps; ++k) SB_HIP(launch());
   SB_HIP(hipEventRecord(ctx->ev1, s));

@@ -35,6 +35,7 @@ BATCH_CALLER_ORDER = 2
 BATCH_SPREAD_TEAMS = 4
 BSR_SYMMETRIC = 1
 PROBLEM_ASSEMBLE_W = 1
+BENCH_FLUSH_CACHES = 16
 
 # Every extern "C" symbol of include/sobogus_hip.h
 EXPORTED = [
@@ -71,7 +72,7 @@ class Stats(C.Structure):
                 ("converged", C.c_int), ("local_failures", C.c_int), ("residual", C.c_double),
                 ("device_ms", C.c_double), ("history", C.POINTER(C.c_double)), ("history_capacity", C.c_int),
                 ("history_length", C.c_int), ("refine_iterations", C.c_int), ("refine_spmvs", C.c_int),
-                ("mixed_residual", C.c_double), ("plain_launch", C.c_int)]
+                ("mixed_residual", C.c_double), ("plain_launch", C.c_int), ("mixed_device_ms", C.c_double)]
 
 
 class AdjointOptions(C.Structure):
@@ -370,6 +371,7 @@ class SolveResult:
     refine_spmvs: int = 0
     mixed_residual: float = 0.0
     plain_launch: bool = False
+    mixed_device_ms: float = 0.0
 
 
 class DualFrictionProblem:
@@ -448,7 +450,8 @@ class DualFrictionProblem:
                            local_failures=st.local_failures, device_ms=st.device_ms,
                            history=hist[:min(st.history_length, hist_cap)].copy(),
                            refine_iterations=st.refine_iterations, refine_spmvs=st.refine_spmvs,
-                           mixed_residual=st.mixed_residual, plain_launch=bool(st.plain_launch))
+                           mixed_residual=st.mixed_residual, plain_launch=bool(st.plain_launch),
+                           mixed_device_ms=st.mixed_device_ms)
 
     def solve_pg(self, r0, opts: PGOptions | None = None, on_device=False, hist_cap=0, **kw) -> SolveResult:
         """ProjectedGradient::solve through DualFrictionProblem::solveWith(pg, r, staticProblem)."""

@@ -330,3 +330,31 @@ def test_gs_per_color_ndx_matches_sequential_on_one_color(G, O):
     res2, r2, _ = op.gs_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=2000, ndx_ref_per_color=True,
                               colors=colors)
     assert res1 < 1e-8 and res2 < 1e-8
+
+
+def test_gs_parallel_colors_equal_one_thread(G, O):
+    """The oracle sweeps a color's contacts on several threads when the colors hold no
+    coupled pair (the reference's OpenMP loop over a color): the iterates and sweep
+    count equal a one-thread run bit for bit. A single color holding
+    coupled contacts keeps the sequential sweep, whatever the thread count (the residual
+    evaluations are OpenMP sums whose order follows the thread count)."""
+    prob = G.generate(nc=3000, nb=1500, seed=21)
+    op = O.OracleProblem(prob)
+    perm, colors = op.coloring()
+    op.permute(perm)
+    saved = O.threads()
+    try:
+        out = []
+        for t in (1, 8):
+            O.set_threads(t)
+            out.append(op.gs_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=400, ndx_ref_per_color=True,
+                                   colors=colors, coulomb=False))
+            out.append(op.gs_solve(np.zeros(3 * prob.nc), tol=1e-8, max_iters=50, ndx_ref_per_color=True,
+                                   colors=np.array([0, prob.nc]), coulomb=False))
+        (res1, r1, d1), (ress1, rs1, _), (res8, r8, d8), (ress8, rs8, _) = out
+        # the iterates are identical; the residual's own OpenMP sum follows the thread count
+        assert d1["iterations"] == d8["iterations"] and np.array_equal(r1, r8)
+        assert np.allclose(d1["history"], d8["history"], rtol=1e-12, atol=0)
+        assert np.array_equal(rs1, rs8) and ress1 == pytest.approx(ress8, rel=1e-12)
+    finally:
+        O.set_threads(saved)
