@@ -251,22 +251,28 @@ def tight_50k(ctx, R, G, cpu_threads: int) -> dict:
     return out
 
 
-def adjoint_100k(ctx, R, G, n_dirs: int = 8) -> dict:
-    """The 100,000-contact derivative workload: mixed-precision solve, then the
-    gradient of L = g . T^K(r*) (K = 200, SOC law, checkpointed unroll) in b, mu and
-    Minv, its cost against the forward K steps, and central differences
-    (h = 1e-6, FP64, projection branches of the unperturbed run replayed) on
-    n_dirs random directions."""
+def adjoint_100k(ctx, R, G, n_dirs: int = 64) -> dict:
+    """The 100,000-contact derivative workload (Coulomb law, L = g . r, g ~ N(0,1) seed 55):
+    mixed-precision solve (FP32 bulk, FP64 residual and verification), then
+      * the gradient of a 200-step checkpointed unroll from the solution in b, mu and
+        Minv, its cost against the forward steps, and central differences (h = 1e-6,
+        FP64) on n_dirs random directions of the map replayed on the recorded piece
+        (projection branches and De Saxce directions of the unperturbed run);
+      * the implicit gradient at the solution (GMRES on the adjoint system);
+      * at an FP64 solution to FB 1e-10, the implicit gradient against a
+        40,000-step unroll from it (the unroll's limit)."""
     prob = G.workload("adjoint-100k")
     dp = R.DualFrictionProblem(ctx, prob, assemble=False)
-    sol = dp.solve_pg(np.zeros(3 * prob.nc), variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE,
-                      scaling=R.SCALING_DIAGONAL, tol=1e-6, max_iters=20000, precision=R.PRECISION_MIXED)
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL,
+              max_iters=100000)
+    sol = dp.solve_pg(np.zeros(3 * prob.nc), tol=1e-6, precision=R.PRECISION_MIXED, **kw)
     g = np.random.default_rng(55).standard_normal(3 * prob.nc)
     K = 200
-    ad = dp.adjoint(sol.r, g, iterations=K, law=R.LAW_SOC, record_branches=True)
-    ad = dp.adjoint(sol.r, g, iterations=K, law=R.LAW_SOC, record_branches=True)
-    fwd = dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=R.LAW_SOC)
-    fwd = dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=R.LAW_SOC)
+    law = R.LAW_COULOMB
+    ad = dp.adjoint(sol.r, g, iterations=K, law=law, record_branches=True)
+    ad = dp.adjoint(sol.r, g, iterations=K, law=law, record_branches=True)
+    fwd = dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=law)
+    fwd = dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=law)
     rng = np.random.default_rng(56)
     h, errs = 1e-6, []
     for _ in range(n_dirs):
@@ -278,16 +284,36 @@ def adjoint_100k(ctx, R, G, n_dirs: int = 8) -> dict:
         lv = []
         for t in (h, -h):
             dp.update(b=prob.b + t * db, mu=prob.mu + t * dmu, minv=prob.minv.reshape(prob.nb, 36) + t * dM)
-            lv.append(float(g @ dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=R.LAW_SOC,
+            lv.append(float(g @ dp.adjoint(sol.r, None, iterations=K, rho=ad["rho"], law=law,
                                            branches=ad["branches"])["r"]))
         errs.append(abs((lv[0] - lv[1]) / (2 * h) - an) / max(abs(an), 1e-12))
     dp.update(b=prob.b, mu=prob.mu, minv=prob.minv.reshape(prob.nb, 36))
-    out = {"contacts": prob.nc, "bodies": prob.nb, "forward_solve_ms": sol.device_ms,
-           "forward_solve_iterations": sol.iterations, "unroll_steps": K, "gradient_ms": ad["device_ms"],
-           "forward_steps_ms": fwd["device_ms"], "gradient_vs_forward": ad["device_ms"] / fwd["device_ms"],
-           "fd_directions": n_dirs, "fd_max_rel_error": float(max(errs)), "law": "soc"}
-    log(f"adjoint-100k: gradient of {K} steps {ad['device_ms']:.1f} ms ({out['gradient_vs_forward']:.2f}x the "
-        f"forward steps), FD max rel error {out['fd_max_rel_error']:.1e} on {n_dirs} directions")
+    imp = dp.adjoint(sol.r, g, mode=R.ADJOINT_IMPLICIT, rho=ad["rho"], law=law, tol=1e-10, max_iters=5000)
+    # the implicit gradient as the limit of the unroll, at a tight FP64 fixed point
+    s64 = dp.solve_pg(np.zeros(3 * prob.nc), tol=1e-10, precision=R.PRECISION_FP64, **kw)
+    imp64 = dp.adjoint(s64.r, g, mode=R.ADJOINT_IMPLICIT, rho=ad["rho"], law=law, tol=1e-10, max_iters=5000)
+    K_long = 40000
+    unr = dp.adjoint(s64.r, g, iterations=K_long, rho=ad["rho"], law=law)
+    agree = {k: float(np.linalg.norm(imp64[k] - unr[k]) / np.linalg.norm(imp64[k]))
+             for k in ("grad_b", "grad_mu", "grad_minv")}
+    out = {"contacts": prob.nc, "bodies": prob.nb, "law": "coulomb",
+           "forward_solve": {"precision": "mixed", "ms": sol.device_ms, "iterations": sol.iterations,
+                             "residual": sol.residual},
+           "unroll": {"steps": K, "gradient_ms": ad["device_ms"], "forward_steps_ms": fwd["device_ms"],
+                      "gradient_vs_forward": ad["device_ms"] / fwd["device_ms"], "fd_directions": n_dirs,
+                      "fd_h": h, "fd_max_rel_error": float(max(errs)), "fd_median_rel_error": float(np.median(errs)),
+                      "fd_map": "the map replayed on the recorded piece (projection branches and De Saxce "
+                                "directions of the unperturbed run)"},
+           "implicit": {"solver": "restarted GMRES(40) on (I - dT/dr^T) lambda = g", "ms": imp["device_ms"],
+                        "arnoldi_steps": imp["iterations"], "relative_residual": imp["change"],
+                        "converged": bool(imp["converged"])},
+           "implicit_vs_unroll_limit": {"fixed_point": f"FP64 APGD to FB 1e-10 ({s64.iterations} iterations)",
+                                        "unroll_steps": K_long, "relative_difference": agree,
+                                        "implicit_ms": imp64["device_ms"], "unroll_ms": unr["device_ms"]}}
+    log(f"adjoint-100k (Coulomb): gradient of {K} steps {ad['device_ms']:.1f} ms ({out['unroll']['gradient_vs_forward']:.2f}x "
+        f"the forward steps), FD max rel error {max(errs):.1e} on {n_dirs} directions; implicit (GMRES) "
+        f"{imp['device_ms']:.0f} ms, {imp['iterations']} steps; implicit vs {K_long}-step unroll: "
+        + ", ".join(f"{k} {v:.1e}" for k, v in agree.items()))
     return out
 
 

@@ -282,12 +282,13 @@ int sb_gs_solve_cadoux(sb_ctx* ctx, const sb_problem* P, const sb_gs_options* op
 /* ---------------------------------------------------------------- derivatives
    Reverse-mode derivatives of L = g . r for the projected fixed-point map of the
    dual problem, T(r) = P_mu(r - rho (W r + b + s_mu(W r + b))) (s_mu: the De Saxce
-   term of the Coulomb law, 0 for SOC), whose fixed points are the solutions of
-   the projected-gradient solvers. No counterpart in the reference (it has no
-   differentiation). Matrix-free (needs H and Minv). Host arrays. */
+   term mu |u_T| e_N of the Coulomb law, 0 for SOC), whose fixed points are the
+   solutions of the projected-gradient solvers. No counterpart in the reference (it
+   has no differentiation). Matrix-free (needs H and Minv). Host arrays. */
 enum sb_adjoint_mode {
   SB_ADJOINT_UNROLL = 0,   /* L = g . T^K(r0): exact gradient of K steps, checkpointed */
-  SB_ADJOINT_IMPLICIT = 1  /* L = g . r* at a fixed point r0 = r*: implicit function theorem */
+  SB_ADJOINT_IMPLICIT = 1  /* L = g . r* at a fixed point r0 = r*: implicit function theorem, the adjoint
+                              system (I - dT/dr^T) lambda = g solved by restarted GMRES */
 };
 typedef struct sb_adjoint_options {
   int mode;               /* sb_adjoint_mode */
@@ -295,19 +296,24 @@ typedef struct sb_adjoint_options {
   int iterations;         /* unroll: K (200) */
   int checkpoint_every;   /* unroll: C (0: ceil(sqrt(K))) */
   double rho;             /* step of the map; 0: 1 / lambda_max(W) by power iteration */
-  double tol;             /* implicit: relative change of the adjoint vector (1e-10) */
-  int max_iters;          /* implicit: iterations of the adjoint fixed point (20000) */
+  double tol;             /* implicit: relative residual |g - (I - A^T) lambda| / |g| (1e-10) */
+  int max_iters;          /* implicit: GMRES (Arnoldi) steps in all, over the restarts (3000) */
   int grid_blocks;        /* 0 = automatic */
-  int branch_mode;        /* unroll: 0 projection branches from the cone tests; 1 also record them
-                             into branches; 2 replay them from branches (the map is then one smooth
-                             piece: finite differences see no kinks) */
+  int branch_mode;        /* unroll: 0 branches from the cone tests; 1 also record them into branches
+                             (and, Coulomb law, the De Saxce directions u_T / |u_T| of every step, kept
+                             on the device with the problem); 2 replay them (the projection formula of
+                             the recorded branch, and mu (d . u_T) with the recorded direction d for
+                             mu |u_T|: the map is then one smooth piece, with the recorded run's values
+                             and gradient, and finite differences see no kinks; a Coulomb replay needs
+                             a record of the same K on this problem) */
+  int krylov_dim;         /* implicit: GMRES restart length, 1..64 (40) */
 } sb_adjoint_options;
 typedef struct sb_adjoint_stats {
   double rho;             /* step used */
   double loss;            /* g . r_K (unroll) or g . r* (implicit) */
-  int iterations;         /* unroll: K; implicit: adjoint iterations */
-  int converged;          /* implicit: relative change below tol */
-  double change;          /* implicit: last relative change */
+  int iterations;         /* unroll: K; implicit: Arnoldi steps */
+  int converged;          /* implicit: relative residual below tol */
+  double change;          /* implicit: final relative residual of the adjoint system (recomputed) */
   double device_ms;
 } sb_adjoint_stats;
 void sb_adjoint_options_default(sb_adjoint_options* o);

@@ -200,6 +200,10 @@ struct sb_problem {
   bool factored = false;
   bool useFactored() const { return fVersion == version && factored; }
   DBuf<double> out;  // [2][16] solver results (FP32 stage, FP64 stage)
+  // derivatives, Coulomb law: De Saxce directions recorded by an unroll (branch_mode 1) for
+  // its replay (branch_mode 2): [adjDirK][nc][2]
+  DBuf<double> adjDir;
+  int adjDirK = 0;
   DevProblem<double> view(bool withScaling) const {
     DevProblem<double> v;
     v.nc = nc;
@@ -1781,9 +1785,10 @@ void sb_adjoint_options_default(sb_adjoint_options* o) {
   o->checkpoint_every = 0;
   o->rho = 0.0;
   o->tol = 1e-10;
-  o->max_iters = 20000;
+  o->max_iters = 3000;
   o->grid_blocks = 0;
   o->branch_mode = 0;
+  o->krylov_dim = 40;
 }
 
 int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, const double* r0, const double* g,
@@ -1796,6 +1801,7 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   if (o->mode == SB_ADJOINT_IMPLICIT && !g) return SB_ERR_INVALID;
   if (o->mode == SB_ADJOINT_UNROLL && o->iterations < 1) return SB_ERR_INVALID;
   if (o->mode == SB_ADJOINT_IMPLICIT && (o->max_iters < 1 || !(o->tol > 0))) return SB_ERR_INVALID;
+  if (o->mode == SB_ADJOINT_IMPLICIT && (o->krylov_dim < 1 || o->krylov_dim > sb::kMaxKrylov)) return SB_ERR_INVALID;
   if (!(o->rho >= 0) || o->checkpoint_every < 0) return SB_ERR_INVALID;
   if (o->branch_mode < 0 || o->branch_mode > 2) return SB_ERR_INVALID;
   if (o->branch_mode != 0 && (!branches || o->mode != SB_ADJOINT_UNROLL)) return SB_ERR_INVALID;
@@ -1832,7 +1838,7 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   int C = o->checkpoint_every;
   if (C == 0) C = std::max(1, (int)std::ceil(std::sqrt((double)K)));
   const int nseg = (K + C - 1) / C;
-  DBuf<double> r0d, gd, rK, ckpt, seg, lam, mz, muu, hr, jac, gb, gmu, gminv, out;
+  DBuf<double> r0d, gd, rK, ckpt, seg, lam, kry, mz, muu, hr, jac, gb, gmu, gminv, out;
   DBuf<unsigned char> br;
   const size_t nbr = (size_t)K * nc;
   if (o->branch_mode == 2) SB_HIP(br.upload(branches, nbr, s));
@@ -1853,6 +1859,21 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
     SB_HIP(seg.alloc((size_t)std::max(C, 2) * n3));
   } else {
     SB_HIP(jac.alloc(16 * (size_t)nc));
+    SB_HIP(kry.alloc((size_t)(o->krylov_dim + 1) * n3));
+  }
+  // Coulomb law: the De Saxce directions of every step, recorded with the branches and
+  // replayed with them (kept with the problem between the two calls)
+  double* sdir = nullptr;
+  if (o->law == SB_LAW_COULOMB && o->mode == SB_ADJOINT_UNROLL && o->branch_mode != 0) {
+    const size_t nd = 2 * (size_t)K * nc;
+    if (o->branch_mode == 1) {
+      if (P->adjDir.n != nd) SB_HIP(P->adjDir.alloc(nd));
+      P->adjDirK = K;
+    } else if (P->adjDir.n != nd || P->adjDirK != K) {
+      last_error_string() = "replay of a Coulomb unroll needs the directions recorded by an unroll of the same length";
+      return SB_ERR_INVALID;
+    }
+    sdir = P->adjDir.p;
   }
   sb::AdjArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -1864,6 +1885,9 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   a.rho = rho;
   a.tol = o->tol;
   a.max_iters = o->max_iters;
+  a.kdim = o->krylov_dim;
+  a.kry = kry.p;
+  a.sdir = sdir;
   a.r0 = r0d.p;
   a.g = g ? gd.p : nullptr;
   a.rK = rK.p;

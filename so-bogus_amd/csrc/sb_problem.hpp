@@ -102,20 +102,23 @@ struct PGBatchArgs {
 };
 
 // Reverse-mode derivatives of the projected fixed-point map (matrix-free, FP64).
+constexpr int kMaxKrylov = 64;  // implicit mode: largest GMRES restart length
 struct AdjArgs {
   DevProblem<double> P;
   int mode;             // 0: unrolled K steps from r0; 1: implicit at the fixed point r0
   int K, C;             // unroll length, checkpoint interval
   int coulomb;          // De Saxce term on (Coulomb law) or off (SOC law)
   double rho;           // step of the map
-  double tol;           // implicit: relative change of lambda
-  int max_iters;        // implicit: Neumann iterations
+  double tol;           // implicit: relative residual |g - (I - A^T) lam| / |g| of the adjoint system
+  int max_iters;        // implicit: GMRES (Arnoldi) steps in all
+  int kdim;             // implicit: GMRES restart length (<= kMaxKrylov)
   const double* r0;     // [3 nc] start (unroll) or fixed point (implicit)
   const double* g;      // [3 nc] dL/dr (null: forward only)
   double* rK;           // [3 nc] out: r_K (unroll)
   double* ckpt;         // [ceil(K/C) + 1][3 nc]
   double* seg;          // [max(C, 2)][3 nc]
   double* lam;          // [2][3 nc]
+  double* kry;          // implicit: [kdim + 1][3 nc] Krylov basis
   double* mz;           // [3 nc]
   double* mu_u;         // [3 nc]
   double* hr;           // [6 nb] H^T r of the current step
@@ -127,8 +130,10 @@ struct AdjArgs {
   double* grad_minv;    // [nb][36] out (row-major 6x6 per body)
   unsigned char* branch;  // unroll: [K][nc] projection branch per step (null: not kept)
   int branch_mode;      // 0: branches from the projection tests, 1: record them, 2: replay them
+  double* sdir;         // unroll, Coulomb law: [K][nc][2] tangential direction u_T / |u_T| of the De Saxce
+                        // term per step (recorded with branch_mode 1, replayed with 2; null otherwise)
   GridSync gs;
-  double* out;          // [0] loss [1] iterations [2] last relative change [3] converged [4] failed
+  double* out;          // [0] loss [1] iterations [2] last relative residual [3] converged [4] failed
 };
 
 struct GSArgs {

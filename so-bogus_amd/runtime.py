@@ -78,7 +78,7 @@ class Stats(C.Structure):
 class AdjointOptions(C.Structure):
     _fields_ = [("mode", C.c_int), ("law", C.c_int), ("iterations", C.c_int), ("checkpoint_every", C.c_int),
                 ("rho", C.c_double), ("tol", C.c_double), ("max_iters", C.c_int), ("grid_blocks", C.c_int),
-                ("branch_mode", C.c_int)]
+                ("branch_mode", C.c_int), ("krylov_dim", C.c_int)]
 
 
 class AdjointStats(C.Structure):
@@ -533,14 +533,14 @@ class DualFrictionProblem:
         _check(lib().sb_problem_update(self.ctx.h, self.h, *ptr), "sb_problem_update")
 
     def adjoint(self, r0, g=None, mode=ADJOINT_UNROLL, law=LAW_COULOMB, iterations=200, checkpoint_every=0, rho=0.0,
-                tol=1e-10, max_iters=20000, record_branches=False, branches=None) -> dict:
+                tol=1e-10, max_iters=3000, record_branches=False, branches=None, krylov_dim=40) -> dict:
         """Forward map T^K(r0) (g None) or the gradients of L = g . r (sb_adjoint).
         record_branches: also return the projection branch of every contact and step;
         branches: replay them (the map then stays on one smooth piece)."""
         o = AdjointOptions()
         lib().sb_adjoint_options_default(C.byref(o))
         o.mode, o.law, o.iterations, o.checkpoint_every = mode, law, iterations, checkpoint_every
-        o.rho, o.tol, o.max_iters = rho, tol, max_iters
+        o.rho, o.tol, o.max_iters, o.krylov_dim = rho, tol, max_iters, krylov_dim
         br = None
         if branches is not None:
             br = np.ascontiguousarray(branches, dtype=np.uint8)
