@@ -98,7 +98,8 @@ typedef struct sb_problem sb_problem;
 
 /* ProjectedGradient parameters (ProjectedGradient.hpp:57-95, BlockSolverBase.hpp:37-43) */
 typedef struct sb_pg_options {
-  int variant;            /* sb_pg_variant (APGD, DESCENT, SPG on the GPU) */
+  int variant;            /* sb_pg_variant; every variant runs on the GPU (Standard and Conjugated in
+                             single-problem solves only: sb_batch_* return SB_ERR_UNSUPPORTED) */
   int law;                /* sb_law; DualFrictionProblem::solveWith(pg) defaults to SOC */
   int op;                 /* sb_operator */
   int scaling;            /* sb_scaling */

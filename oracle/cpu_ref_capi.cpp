@@ -96,9 +96,23 @@ static void copyHistory(const SolveStats& st, double* hist, int cap, int* histLe
     for (int k = 0; k < cap && k < (int)st.history.size(); ++k) hist[k] = st.history[k];
 }
 
+// Line-search parameters of the following PG solves (ProjectedGradient.hpp:87-95 defaults:
+// 8 trials, x1.25 optimistic, x0.5 pessimistic, Armijo 0.5).
+static PGOptions g_pgLineSearch;
+void orc_set_pg_line_search(int lsIters, double optimistic, double pessimistic, double armijo) {
+  g_pgLineSearch.lsIters = (unsigned)lsIters;
+  g_pgLineSearch.lsOptimistic = optimistic;
+  g_pgLineSearch.lsPessimistic = pessimistic;
+  g_pgLineSearch.lsArmijo = armijo;
+}
+
 double orc_pg_solve(void* p, int variant, double tol, int maxIters, int coulomb, int useAssembled, int diagScaling,
                     int infNorm, double* r, int* iters, int* spmvs, double* hist, int histCap, int* histLen) {
   PGOptions o;
+  o.lsIters = g_pgLineSearch.lsIters;
+  o.lsOptimistic = g_pgLineSearch.lsOptimistic;
+  o.lsPessimistic = g_pgLineSearch.lsPessimistic;
+  o.lsArmijo = g_pgLineSearch.lsArmijo;
   o.variant = variant;
   o.tol = tol;
   o.maxIters = (unsigned)maxIters;

@@ -41,6 +41,7 @@ def lib():
         L = C.CDLL(str(LIB_PATH))
         sig = {
             "orc_set_threads": (None, [C.c_int]),
+            "orc_set_pg_line_search": (None, [C.c_int, C.c_double, C.c_double, C.c_double]),
             "orc_get_threads": (C.c_int, []),
             "orc_problem_create": (C.c_void_p, [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]),
             "orc_problem_destroy": (None, [C.c_void_p]),
@@ -109,6 +110,12 @@ def i32(a) -> np.ndarray:
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+def set_pg_line_search(ls_iters: int = 8, optimistic: float = 1.25, pessimistic: float = 0.5,
+                       armijo: float = 0.5) -> None:
+    """Line-search parameters of later pg_solve calls (defaults: the reference's)."""
+    lib().orc_set_pg_line_search(int(ls_iters), float(optimistic), float(pessimistic), float(armijo))
 
 
 class OracleProblem:

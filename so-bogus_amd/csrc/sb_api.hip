@@ -507,8 +507,10 @@ sb::GridSync make_sync(sb_ctx* ctx, int blocks) {
   return make_sync(st, blocks);
 }
 
-int check_pg_options(const sb_pg_options* o, bool hasW) {
-  if (o->variant != SB_PG_APGD && o->variant != SB_PG_DESCENT && o->variant != SB_PG_SPG) return SB_ERR_UNSUPPORTED;
+int check_pg_options(const sb_pg_options* o, bool hasW, bool batch = false) {
+  if (o->variant < SB_PG_STANDARD || o->variant > SB_PG_SPG) return SB_ERR_INVALID;
+  // the batch kernel runs the APGD / Descent / SPG body only
+  if (batch && (o->variant == SB_PG_STANDARD || o->variant == SB_PG_CONJUGATED)) return SB_ERR_UNSUPPORTED;
   if (o->op != SB_OP_ASSEMBLED && o->op != SB_OP_MATRIX_FREE) return SB_ERR_INVALID;
   if (o->op == SB_OP_ASSEMBLED && !hasW) return SB_ERR_INVALID;
   if (o->max_iters < 1 || o->ls_iters < 1 || !(o->tol >= 0)) return SB_ERR_INVALID;
@@ -531,6 +533,7 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
   p.ls_iters = o->ls_iters;
   p.ls_opt = o->ls_optimistic;
   p.ls_pess = o->ls_pessimistic;
+  p.ls_armijo = o->ls_armijo;
   p.stall_iters = 0;
   return p;
 }
@@ -2498,7 +2501,7 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
 
 int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, double* r, int on_device,
               sb_batch_stats* st, bool updateHints) {
-  int rc = check_pg_options(o, B->hasW);
+  int rc = check_pg_options(o, B->hasW, true);
   if (rc) return rc;
   BatchPlan pl;
   std::memset(&pl, 0, sizeof(pl));

@@ -540,12 +540,290 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
   }
 }
 
+// PgMethod<Standard> and PgMethod<Conjugated> (ProjectedGradient.impl.hpp:157-209,
+// 211-286) with armijo_ls (:54-72), for one team. x, M x, the gradient y = M x + b + s
+// and J are kept as in pg_body; the line-search candidate P(x + alpha dir) is formed
+// inside the gather, the direction computed on the fly from stored vectors:
+//  * Standard: dir = beta (x - P(x - y)), beta = -y.pg / (pg . M pg), which takes one
+//    operator application on pg before the line search;
+//  * Conjugated: dir = -y or beta dir_prev - y (Polak-Ribiere or Hestenes-Stiefel beta
+//    from the projected gradients x - P(x - y) of this and the previous iteration),
+//    with the test dir . y >= 0 evaluated from reduced dot products.
+// Slots: x and its candidate alternate in X[0..1], MX[0..1], G[0..1]; Conjugated keeps
+// the previous direction in X[2] and the previous projected gradient in X[3].
+template <bool MF, typename T, typename Team>
+__device__ __forceinline__ void pg_body_sc(const PGParams& prm, const PGProblem<T>& pb, Team& tm) {
+  const DevProblem<T>& P = pb.P;
+  const int nc = P.nc;
+  const int n3 = 3 * nc;
+  const bool coulomb = prm.coulomb != 0;
+  const unsigned resMax = prm.inf_norm ? 1u : 0u;
+  const bool conj = (prm.variant == 2);
+  int it = 0, spmvs = 0, lsTrials = 0;
+  bool ok = true;
+  double minRes = -1.0, res = 0.0, J = 0.0, alpha = 1.0;
+  int rx = 0, rg = 0;
+  auto rowScale = [&](int i) { return prm.use_scaling ? gld(P.scal + i) : 1.0; };
+  auto initPre = [&](int i) {
+    InitRow r;
+    T xr[3];
+    start_point<T>(pb, i, xr);
+    widen3<T>(xr, r.x);
+    r.mu = gld(P.mu + i);
+    dld3<T>(P.b, i, r.b);
+    r.s = rowScale(i);
+    return r;
+  };
+  // ---- x <- P(x0): M x, gradient and the first residual
+  {
+    double part[2] = {0.0, 0.0};
+    auto ld = [&](int j, T xc[3]) { start_point<T>(pb, j, xc); };
+    auto lead = [&](int i, T v[3], const InitRow& r) {
+      T xr[3] = {(T)r.x[0], (T)r.x[1], (T)r.x[2]};
+      tst3<Team, T>(pb.X[0], i, xr);
+      tst3<Team, T>(pb.MX[0], i, v);
+      if (conj) {
+        const T z3[3] = {T(0), T(0), T(0)};
+        tst3<Team, T>(pb.X[3], i, xr);  // prev_proj_grad = x
+        tst3<Team, T>(pb.X[2], i, z3);  // no previous direction yet
+      }
+      double vd[3], rr, g[3], Ji;
+      widen3<T>(v, vd);
+      eval_contact(coulomb, r.mu, r.s, r.b, r.x, vd, rr, g, Ji);
+      const T gt[3] = {(T)g[0], (T)g[1], (T)g[2]};
+      tst3<Team, T>(pb.G[0], i, gt);
+      part[0] = resMax ? fmax(part[0], rr) : part[0] + rr;
+      part[1] += Ji;
+    };
+    ok = apply_op<MF, T>(pb, tm, ld, initPre, lead) && tm.template reduce<2>(part, resMax);
+    ++spmvs;
+    if (ok) {
+      res = tm.red[0];
+      if (!prm.inf_norm) res /= (1.0 + nc);
+      J = tm.red[1];
+    }
+  }
+  // x - P(x - y) of contact j (the projected gradient)
+  auto projGrad = [&](T mu, const T x[3], const T y[3], T pgv[3]) {
+    T z[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+    project<T>(mu, z[0], z[1], z[2]);
+    pgv[0] = x[0] - z[0];
+    pgv[1] = x[1] - z[1];
+    pgv[2] = x[2] - z[2];
+  };
+  bool converged = false;
+  double stallRef = 1.7976931348623157e308;
+  int lastImprove = 0;
+  double prevN2 = 0.0;
+  while (ok) {
+    // ---- test_residual at the current x
+    if (tm.leader() && it < pb.hist_cap) gst(pb.hist + it, res);
+    if (it == 0 || res < minRes) {
+      minRes = res;
+      team_copy(tm, pb.xbest, pb.X[rx], n3);
+    }
+    if (res < prm.tol) {
+      converged = true;
+      break;
+    }
+    if (minRes < 0.99 * stallRef) {
+      stallRef = minRes;
+      lastImprove = it;
+    }
+    if (prm.stall_iters > 0) {
+      const int since = it - lastImprove;
+      if ((since >= prm.stall_iters && minRes < 10.0 * prm.tol) || since >= 4 * prm.stall_iters) break;
+    }
+    const int rxs = 1 - rx, rgs = 1 - rg;
+    const T* Xx = pb.X[rx];
+    const T* Gx = pb.G[rg];
+    T* dirPrev = pb.X[2];
+    T* pgPrev = pb.X[3];
+    // direction: dir_j = cx * pg_j (Standard), or cd * dirPrev_j - y_j (Conjugated)
+    double cx = 0.0, cd = 0.0;
+    if (!conj) {
+      // beta = -y . pg / (pg . M pg), one operator application on pg
+      double part[2] = {0.0, 0.0};
+      auto ld = [&](int j, T xc[3]) {
+        T xv[3], gv[3];
+        tld3<Team, T>(Xx, j, xv);
+        tld3<Team, T>(Gx, j, gv);
+        projGrad((T)gld(P.mu + j), xv, gv, xc);
+      };
+      auto pre = [&](int i) {
+        SpgRow<T> r;
+        tld3<Team, T>(Xx, i, r.x);
+        tld3<Team, T>(Gx, i, r.g);
+        r.mu = gld(P.mu + i);
+        return r;
+      };
+      auto lead = [&](int i, T v[3], const SpgRow<T>& r) {
+        T pgv[3];
+        projGrad((T)r.mu, r.x, r.g, pgv);
+        double pd[3], gd[3], vd[3];
+        widen3<T>(pgv, pd);
+        widen3<T>(r.g, gd);
+        widen3<T>(v, vd);
+        part[0] += gd[0] * pd[0] + gd[1] * pd[1] + gd[2] * pd[2];
+        part[1] += pd[0] * vd[0] + pd[1] * vd[1] + pd[2] * vd[2];
+      };
+      ok = apply_op<MF, T>(pb, tm, ld, pre, lead) && tm.template reduce<2>(part, 0u);
+      ++spmvs;
+      if (!ok) break;
+      cx = -tm.red[0] / tm.red[1];
+      alpha = fmin(1.0, alpha * prm.ls_opt);
+    } else {
+      // projected gradient, its norm and the conjugation dot products (one pass)
+      double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int i = tm.tid; i < nc; i += tm.nthreads) {
+        T xv[3], gv[3], pgv[3], pp[3], dp[3];
+        tld3<Team, T>(Xx, i, xv);
+        tld3<Team, T>(Gx, i, gv);
+        tld3<Team, T>(pgPrev, i, pp);
+        tld3<Team, T>(dirPrev, i, dp);
+        projGrad((T)gld(P.mu + i), xv, gv, pgv);
+        tst3<Team, T>(pgPrev, i, pgv);
+        double pd[3], ppd[3], dpd[3], gd[3];
+        widen3<T>(pgv, pd);
+        widen3<T>(pp, ppd);
+        widen3<T>(dp, dpd);
+        widen3<T>(gv, gd);
+        for (int c = 0; c < 3; ++c) {
+          part[0] += pd[c] * pd[c];
+          part[1] += dpd[c] * (pd[c] - ppd[c]);
+          part[2] += pd[c] * ppd[c];
+          part[3] += dpd[c] * gd[c];
+          part[4] += gd[c] * gd[c];
+        }
+      }
+      ok = tm.template reduce<5>(part, 0u);
+      if (!ok) break;
+      const double ng2 = tm.red[0];
+      if (prevN2 == 0.0) {
+        cd = 0.0;
+      } else {
+        const double den = tm.red[1], xp = tm.red[2];
+        const double beta = (den < 1.e-12) ? fmax(0.0, ng2 - xp) / prevN2 : fmax(0.0, ng2 - xp) / den;
+        // dir . y = beta dirPrev . y - y . y; a non-descent direction falls back to -y
+        cd = (beta * tm.red[3] - tm.red[4] >= 0.0) ? 0.0 : beta;
+      }
+      prevN2 = ng2;
+      alpha *= prm.ls_opt;
+    }
+    // ---- armijo_ls: xs = P(x + alpha dir)
+    double Js = J, decr = 0.0, resN = 0.0, JN = 0.0;
+    auto dirOf = [&](int j, const T xv[3], const T gv[3], T mu, T d[3]) {
+      if (!conj) {
+        T pgv[3];
+        projGrad(mu, xv, gv, pgv);
+        for (int c = 0; c < 3; ++c) d[c] = (T)cx * pgv[c];
+      } else {
+        T dp[3] = {T(0), T(0), T(0)};
+        if (cd != 0.0) tld3<Team, T>(dirPrev, j, dp);
+        for (int c = 0; c < 3; ++c) d[c] = (T)cd * dp[c] - gv[c];
+      }
+    };
+    auto pre = [&](int i) {
+      TrialRow<T> r;
+      tld3<Team, T>(Xx, i, r.x);
+      tld3<Team, T>(Gx, i, r.g);
+      tld3<Team, T>(pb.MX[rx], i, r.m);
+      dld3<T>(P.b, i, r.b);
+      r.mu = gld(P.mu + i);
+      r.s = rowScale(i);
+      return r;
+    };
+    for (int ls = 0; ls < prm.ls_iters && ok; ++ls) {
+      double part[4] = {0, 0, 0, 0};
+      const T al = (T)alpha;
+      auto ld = [&](int j, T xc[3]) {
+        T xv[3], gv[3], d[3];
+        tld3<Team, T>(Xx, j, xv);
+        tld3<Team, T>(Gx, j, gv);
+        const T mu = (T)gld(P.mu + j);
+        dirOf(j, xv, gv, mu, d);
+        for (int c = 0; c < 3; ++c) xc[c] = xv[c] + al * d[c];
+        project<T>(mu, xc[0], xc[1], xc[2]);
+      };
+      auto lead = [&](int i, T v[3], const TrialRow<T>& r) {
+        T d[3], xst[3];
+        dirOf(i, r.x, r.g, (T)r.mu, d);
+        for (int c = 0; c < 3; ++c) xst[c] = r.x[c] + al * d[c];
+        project<T>((T)r.mu, xst[0], xst[1], xst[2]);
+        tst3<Team, T>(pb.X[rxs], i, xst);
+        tst3<Team, T>(pb.MX[rxs], i, v);
+        double xs[3], vd[3], x[3], g[3], m[3];
+        widen3<T>(xst, xs);
+        widen3<T>(v, vd);
+        widen3<T>(r.x, x);
+        widen3<T>(r.g, g);
+        widen3<T>(r.m, m);
+        const double s0 = cov_normal<double>(coulomb, r.mu, m[1] + r.b[1], m[2] + r.b[2]);
+        part[0] += xs[0] * (0.5 * vd[0] + r.b[0] + s0) + xs[1] * (0.5 * vd[1] + r.b[1]) +
+                   xs[2] * (0.5 * vd[2] + r.b[2]);
+        part[1] += g[0] * (xs[0] - x[0]) + g[1] * (xs[1] - x[1]) + g[2] * (xs[2] - x[2]);
+        double rr, gn[3], Jn;
+        eval_contact(coulomb, r.mu, r.s, r.b, xs, vd, rr, gn, Jn);
+        const T gt[3] = {(T)gn[0], (T)gn[1], (T)gn[2]};
+        tst3<Team, T>(pb.G[rgs], i, gt);
+        part[2] = resMax ? fmax(part[2], rr) : part[2] + rr;
+        part[3] += Jn;
+      };
+      ok = apply_op<MF, T>(pb, tm, ld, pre, lead) && tm.template reduce<4>(part, resMax << 2);
+      ++spmvs;
+      ++lsTrials;
+      if (!ok) break;
+      Js = tm.red[0];
+      decr = tm.red[1];
+      resN = tm.red[2];
+      JN = tm.red[3];
+      if (Js < J + prm.ls_armijo * decr) break;
+      alpha *= prm.ls_pess;
+    }
+    if (!ok) break;
+    if (conj) {
+      // prev_dir = (xs - x) / alpha (the line search's last alpha)
+      for (int i = tm.tid; i < nc; i += tm.nthreads) {
+        T xv[3], xsv[3], d[3];
+        tld3<Team, T>(Xx, i, xv);
+        tld3<Team, T>(pb.X[rxs], i, xsv);
+        for (int c = 0; c < 3; ++c) d[c] = (T)(((double)xsv[c] - (double)xv[c]) / alpha);
+        tst3<Team, T>(dirPrev, i, d);
+      }
+      ok = tm.sync();
+      if (!ok) break;
+    }
+    rx = rxs;
+    rg = rgs;
+    res = resN;
+    J = JN;
+    if (!prm.inf_norm) res /= (1.0 + nc);
+    ++it;
+    if (it >= prm.max_iters) break;
+  }
+  if (ok) team_copy(tm, pb.x_io, pb.xbest, n3);
+  if (tm.leader()) {
+    gst(pb.out + 0, minRes);
+    gst(pb.out + 1, (double)it);
+    gst(pb.out + 2, (double)spmvs);
+    gst(pb.out + 3, (double)lsTrials);
+    gst(pb.out + 4, 0.0);
+    gst(pb.out + 5, converged ? 1.0 : 0.0);
+    gst(pb.out + 6, ok ? 0.0 : 1.0);
+    for (int k = 0; k < 4; ++k) gst(pb.out + 7 + k, 0.0);
+  }
+}
+
 // ------------------------------------------------------------------ kernels
 
 // One problem on the whole co-resident grid.
 template <bool MF, typename T>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
   GridTeam tm(a.gs, blockIdx.x);
+  if (a.prm.variant == 0 || a.prm.variant == 2) {  // Standard, Conjugated
+    pg_body_sc<MF, T, GridTeam>(a.prm, a.pb, tm);
+    return;
+  }
   uint64_t tsum[4] = {0, 0, 0, 0};
   pg_body<MF, T, GridTeam, true>(a.prm, a.pb, tm, tsum);
 }

@@ -31,7 +31,7 @@ struct DevProblem {
 
 // Projected-gradient parameters shared by every problem of a launch.
 struct PGParams {
-  int variant;      // 1 Descent, 3 APGD, 4 SPG
+  int variant;      // 0 Standard, 1 Descent, 2 Conjugated, 3 APGD, 4 SPG
   int coulomb;
   int inf_norm;
   int use_scaling;  // 1: residual row scaling from scal[], 0: all ones
@@ -39,6 +39,7 @@ struct PGParams {
   int max_iters;
   int ls_iters;
   double ls_opt, ls_pess;
+  double ls_armijo;  // Standard / Conjugated: Armijo coefficient of armijo_ls
   int stall_iters;  // > 0: stop (not converged) after this many iterations without a 1% decrease of
                     // the best residual within 10 tol (4x as many above): the FP32 stage of a mixed
                     // solve hands over to FP64

@@ -435,7 +435,9 @@ struct DualFrictionProblem<3> {
     st.history_capacity = (int)hist.size();
     double res = 0;
     const int s = sb_pg_solve(m_ctx.get(), m_P.get(), &o, rp.data(), 0, &res, &st);
-    if (s == SB_ERR_UNSUPPORTED) return -1;  // unknown / host-only variant, as the reference's -1
+    // an unknown variant returns -1, as the reference's ProjectedGradient::solve
+    if (s == SB_ERR_UNSUPPORTED || (s == SB_ERR_INVALID && (o.variant < SB_PG_STANDARD || o.variant > SB_PG_SPG)))
+      return -1;
     sb_check(s, "sb_pg_solve");
     for (int k = 0; k < st.history_length && k < (int)hist.size(); ++k) pg.callback().trigger(k, hist[k]);
     st.history = nullptr;

@@ -102,9 +102,10 @@ static void testProjectedGradient() {
   pg.setTol(1.e-8);
   unsigned ncb = 0;
   pg.callback().connect([&](unsigned, double) { ++ncb; });
-  const bogus::projected_gradient::Variant vs[3] = {bogus::projected_gradient::APGD,
-                                                    bogus::projected_gradient::Descent,
-                                                    bogus::projected_gradient::SPG};
+  // every variant of tests/ProjectedGradient.cpp:31-60, in the reference test's order
+  const bogus::projected_gradient::Variant vs[5] = {
+      bogus::projected_gradient::APGD, bogus::projected_gradient::Descent, bogus::projected_gradient::Standard,
+      bogus::projected_gradient::Conjugated, bogus::projected_gradient::SPG};
   for (auto v : vs) {
     std::vector<double> x(6, 1.0);
     pg.setDefaultVariant(v);
