@@ -2721,7 +2721,7 @@ int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, int prob
   po.max_iters = std::min(probe_iters, o->max_iters);
   std::vector<std::pair<double, sb_batch_config>> screened;
   const int threadsList[3] = {256, 512, 1024};
-  const int cpbList[5] = {-1, 8000, 4000, 2000, 1000};
+  const int cpbList[7] = {-1, 8000, 4000, 2000, 1000, 750, 500};
   for (int th : threadsList) {
     int occ = batch_occupancy(batch_fn<double>(mf, th), th);
     if (mixed) occ = std::min(occ, batch_occupancy(batch_fn<float>(mf, th), th));

@@ -29,6 +29,18 @@ __device__ __forceinline__ int shfl_down_i(int v, int d) { return __shfl_down(v,
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, SB_WAVE); }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (SB_WAVE - 1); }
+
+// A value that every lane of the wave holds (the result of a team reduction, a
+// decision taken from one): read from the first active lane into scalar registers,
+// so that the compiler keeps it out of the vector register file.
+__device__ __forceinline__ double uni(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int wave_in_block() { return threadIdx.x / SB_WAVE; }
 
 template <typename T>

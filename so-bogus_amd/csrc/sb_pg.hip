@@ -164,6 +164,8 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
   double minRes = -1.0, res = 0.0, J = 0.0, alpha = 1.0, theta = 1.0;
   int rx = 0, rprev = 0, rg = 0;
   uint64_t tp0 = 0, tp1 = 0;
+  // uniform solver state in scalar registers where the team asks for it
+  auto U = [](double v) { return Team::kScalarState ? uni(v) : v; };
   auto rowScale = [&](int i) { return prm.use_scaling ? gld(P.scal + i) : 1.0; };
 
   auto initPre = [&](int i) {
@@ -196,7 +198,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
     ++spmvs;
     if (ok) {
       res = tm.red[0];
-      if (!prm.inf_norm) res /= (1.0 + nc);
+      res = U(prm.inf_norm ? res : res / (1.0 + nc));
       J = tm.red[1];
     }
   }
@@ -219,7 +221,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
     ++spmvs;
     if (ok) {
       const double nMx = tm.red[0], nd = tm.red[1];
-      if (nMx > 1.e-16) alpha = fmin(1.0, nd / nMx);
+      if (nMx > 1.e-16) alpha = U(fmin(1.0, nd / nMx));
     }
   }
 
@@ -264,7 +266,8 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
     // M xn = M xs + beta (M xs - M prev), g(xn), its residual and energy; the
     // iteration then adopts xn (decr2 < 0) or restarts at xs, without another pass.
     double thetaN = theta;
-    const double beta = useNesterov ? nesterov(thetaN, 0.0) : 0.0;
+    const double beta = U(useNesterov ? nesterov(thetaN, 0.0) : 0.0);
+    thetaN = U(thetaN);
     // contact i's Nesterov point from its candidate (T arithmetic as the candidate)
     auto nesterov_point = [&](int i, const T xst[3], const T v[3], const T pv[3], const T mp[3], double mu,
                               double sc, const double b[3], double& rr, double& Jn) {
@@ -285,7 +288,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
     };
     if (!spg) {
       // ---- APGD / Descent: backtracking line search (ProjectedGradient.impl.hpp:117-133)
-      alpha *= prm.ls_opt;
+      alpha = U(alpha * prm.ls_opt);
       double Js = 0, decr = 0, nd = 0, decr2 = 0, resN = 0, JN = 0, resX = 0, JX = 0;
       auto pre = [&](int i) {
         TrialRow<T> r;
@@ -358,7 +361,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         resX = tm.red[6];
         JX = tm.red[7];
         if (Js < J + decr + 0.5 / alpha * nd) break;
-        alpha *= prm.ls_pess;
+        alpha = U(alpha * prm.ls_pess);
       }
       if (!ok) break;
       if (useNesterov && decr2 < 0) {
@@ -401,7 +404,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         J = JN;
         theta = 1.0;
       }
-      if (!prm.inf_norm) res /= (1.0 + nc);
+      res = U(prm.inf_norm ? res : res / (1.0 + nc));
     } else {
       // ---- SPG (ProjectedGradient.impl.hpp:316-346): one product per iteration.
       double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -517,9 +520,9 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         res = tm.red[7];
         theta = 1.0;
       }
-      if (!prm.inf_norm) res /= (1.0 + nc);
+      res = U(prm.inf_norm ? res : res / (1.0 + nc));
       alpha = wz / zz;
-      alpha = fmin(1.e6, fmax(1.e-6, alpha));
+      alpha = U(fmin(1.e6, fmax(1.e-6, alpha)));
     }
     ++it;
     if (it >= prm.max_iters) break;
@@ -841,7 +844,8 @@ __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
 #ifndef SB_BATCH_PLAIN_STORES
 #define SB_BATCH_PLAIN_STORES 1
 #endif
-using BatchTeamSel = GridTeamT<SB_BATCH_TEAM_ACQUIRE != 0, SB_BATCH_PLAIN_STORES != 0 && SB_BATCH_TEAM_ACQUIRE != 0>;
+using BatchTeamSel =
+    GridTeamT<SB_BATCH_TEAM_ACQUIRE != 0, SB_BATCH_PLAIN_STORES != 0 && SB_BATCH_TEAM_ACQUIRE != 0, true>;
 
 // Many problems. The block map names each block's class, team and rank; its
 // team takes the next problem of the class's queue and solves it alone. Teams

@@ -24,10 +24,15 @@ namespace sb {
 // vectors are read with coherent loads. Plain = true: vectors are written with
 // plain stores (they stay in this XCD's L2); the barrier then writes the L2 back
 // before arriving unless every block of the team runs on one XCD (bar.one_xcd).
-template <bool Acquire, bool Plain = false>
+// Scalar = true: reduction results and the solver's uniform state are read into scalar
+// registers (uni): fewer vector registers, one more wave per SIMD for the batch kernel,
+// at the cost of scalar spills (the single-problem grid kernel, one block per CU, keeps
+// them in vector registers).
+template <bool Acquire, bool Plain = false, bool Scalar = false>
 struct GridTeamT {
   // vectors written by other blocks must be read with coherent loads
   static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
+  static constexpr bool kScalarState = Scalar;
   Barrier bar;
   int gw, nwaves, tid, nthreads;
   bool lead;
@@ -40,7 +45,7 @@ struct GridTeamT {
     bar.acquire = Acquire;
     bar.plain_stores = Plain;
     const int wpb = blockDim.x / SB_WAVE;
-    gw = (int)rank * wpb + wave_in_block();
+    gw = Scalar ? uni((int)rank * wpb + wave_in_block()) : (int)rank * wpb + wave_in_block();
     nwaves = (int)g.nblocks * wpb;
     tid = (int)rank * blockDim.x + threadIdx.x;
     nthreads = (int)g.nblocks * blockDim.x;
@@ -52,7 +57,7 @@ struct GridTeamT {
   __device__ __forceinline__ bool reduce(const double (&v)[NV], unsigned max_mask) {
     if (!bar.reduce<NV>(v, max_mask)) return false;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) red[k] = bar.red[k];
+    for (int k = 0; k < NV; ++k) red[k] = Scalar ? uni(bar.red[k]) : bar.red[k];
     return true;
   }
   template <typename T>
@@ -95,10 +100,11 @@ __device__ __forceinline__ void team_copy(const Team& tm, TD* dst, const TS* src
 
 struct BlockTeam {
   static constexpr bool kCoherentReads = false;
+  static constexpr bool kScalarState = true;  // used by the batch kernel only
   int gw, nwaves, tid, nthreads;
   double red[kMaxPartials];
   __device__ BlockTeam() {
-    gw = wave_in_block();
+    gw = uni(wave_in_block());
     nwaves = blockDim.x / SB_WAVE;
     tid = threadIdx.x;
     nthreads = blockDim.x;
@@ -126,7 +132,7 @@ struct BlockTeam {
       const bool isMax = (max_mask >> k) & 1u;
       double acc = 0.0;
       for (int q = 0; q < nwaves; ++q) acc = isMax ? (s_p[k][q] > acc ? s_p[k][q] : acc) : acc + s_p[k][q];
-      red[k] = acc;
+      red[k] = uni(acc);
     }
     __syncthreads();
     return true;

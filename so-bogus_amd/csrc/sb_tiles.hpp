@@ -49,9 +49,9 @@ struct TileIdx {
 template <typename T>
 __device__ __forceinline__ TileIdx tile_indices(const TileMat<T>& A, int t) {
   TileIdx ix;
-  ix.q0 = gld(A.tstart + t);
-  ix.q1 = gld(A.tstart + t + 1);
-  ix.tl = gld(A.tlong + t);
+  ix.q0 = uni(gld(A.tstart + t));
+  ix.q1 = uni(gld(A.tstart + t + 1));
+  ix.tl = uni(gld(A.tlong + t));
   const int q = ix.q0 + lane_id();
   if (q < ix.q1) {
     ix.row = gld(A.brow + q);
