@@ -113,7 +113,9 @@ typedef struct sb_pg_options {
   int grid_blocks;        /* 0 = automatic; else persistent grid size (clamped) */
   int block_threads;      /* 0 = automatic (512); else 256 or 512 threads per block */
   int precision;          /* sb_precision, FP64 by default */
-  double mixed_tol_factor;/* mixed: the FP32 stage stops at tol * factor (0.8) */
+  double mixed_tol_factor;/* mixed: the FP32 stage stops at tol * factor (0.95; its residual is evaluated in FP64
+                             from the FP32 iterate and product, within 1e-4 relative of the FP64 stage's
+                             verification, which refines if it is not below tol) */
   int mixed_stall_iters;  /* mixed: the FP32 stage also stops after this many iterations without a 1%
                              decrease of its best residual once that is below 10 tol (its rounding
                              floor), or after 4x as many anywhere (100) */

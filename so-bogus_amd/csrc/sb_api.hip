@@ -538,6 +538,12 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
   return p;
 }
 
+template <typename T>
+const void* pg_grid_fn(bool mf, bool scalar) {
+  if (scalar) return mf ? (const void*)&sb::pg_kernel<true, T, true> : (const void*)&sb::pg_kernel<false, T, true>;
+  return mf ? (const void*)&sb::pg_kernel<true, T, false> : (const void*)&sb::pg_kernel<false, T, false>;
+}
+
 // Launches one persistent single-problem solver kernel (grid barrier state reset first)
 // with the context's grid state (grown on demand), or with a fixed one (fixed != null:
 // a grid larger than its capacity is refused, nothing is allocated).
@@ -717,7 +723,7 @@ void sb_pg_options_default(sb_pg_options* o) {
   o->grid_blocks = 0;
   o->block_threads = 0;
   o->precision = SB_PRECISION_FP64;
-  o->mixed_tol_factor = 0.8;
+  o->mixed_tol_factor = 0.95;
   o->mixed_stall_iters = 100;
 }
 
@@ -1208,6 +1214,10 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
   const int threads = block_threads_for(o->block_threads, kMaxThreads);
   if (threads < 0) return SB_ERR_INVALID;
   double* dout = fixed ? fixed->out : P->out.p;
+  // several workgroups per CU: the kernel variant with the uniform state in scalar
+  // registers, which leaves room for them
+  static const int scalarEnv = std::getenv("SOBOGUS_GRID_SCALAR") ? std::atoi(std::getenv("SOBOGUS_GRID_SCALAR")) : -1;
+  const bool scalarState = scalarEnv >= 0 ? scalarEnv != 0 : false;
   int rc;
   ctx->lastPlain = false;
   SB_HIP(hipEventRecord(ctx->ev0, s));
@@ -1216,12 +1226,12 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     sb::PGGridArgs<float> a;
     std::memset(&a, 0, sizeof(a));
     a.prm = prm;
-    a.prm.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
+    a.prm.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.95);
     a.prm.stall_iters = o->mixed_stall_iters;
     P->fill(a.pb, true);
     a.pb.x_io = x;
     a.pb.out = dout;
-    const void* fn = mf ? (const void*)&sb::pg_kernel<true, float> : (const void*)&sb::pg_kernel<false, float>;
+    const void* fn = pg_grid_fn<float>(mf, scalarState);
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
     SB_HIP(hipEventRecord(ctx->evm, s));
   }
@@ -1236,7 +1246,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     a.pb.hist = histCap > 0 ? P->hist.p : nullptr;
     a.pb.hist_cap = histCap;
     a.pb.out = dout + 16;
-    const void* fn = mf ? (const void*)&sb::pg_kernel<true, double> : (const void*)&sb::pg_kernel<false, double>;
+    const void* fn = pg_grid_fn<double>(mf, scalarState);
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
   }
   SB_HIP(hipEventRecord(ctx->ev1, s));
@@ -2415,7 +2425,7 @@ int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, 
   batch_classes(B, pl, order);
   pl.prm64 = make_params(o, pl.mf);
   pl.prm32 = pl.prm64;
-  pl.prm32.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.8);
+  pl.prm32.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.95);
   pl.prm32.stall_iters = o->mixed_stall_iters;
   return SB_OK;
 }

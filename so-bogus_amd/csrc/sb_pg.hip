@@ -819,16 +819,18 @@ __device__ __forceinline__ void pg_body_sc(const PGParams& prm, const PGProblem<
 
 // ------------------------------------------------------------------ kernels
 
-// One problem on the whole co-resident grid.
-template <bool MF, typename T>
+// One problem on the whole co-resident grid. Scalar: the solver's uniform state in
+// scalar registers (fewer vector registers: more waves per SIMD for grids of several
+// workgroups per CU).
+template <bool MF, typename T, bool Scalar>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
-  GridTeam tm(a.gs, blockIdx.x);
+  GridTeamT<true, false, Scalar> tm(a.gs, blockIdx.x);
   if (a.prm.variant == 0 || a.prm.variant == 2) {  // Standard, Conjugated
-    pg_body_sc<MF, T, GridTeam>(a.prm, a.pb, tm);
+    pg_body_sc<MF, T, GridTeamT<true, false, Scalar>>(a.prm, a.pb, tm);
     return;
   }
   uint64_t tsum[4] = {0, 0, 0, 0};
-  pg_body<MF, T, GridTeam, true>(a.prm, a.pb, tm, tsum);
+  pg_body<MF, T, GridTeamT<true, false, Scalar>, true>(a.prm, a.pb, tm, tsum);
 }
 
 // Teams of several workgroups are placed on one XCD. They write their vectors (and
@@ -854,9 +856,10 @@ using BatchTeamSel =
 // bounded and set the sticky error flag on timeout.
 template <bool MF, typename T, int NT>
 __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
-  // FP64: fused Nesterov step (its long problems run on multi-workgroup teams);
-  // FP32: separate update phase (fewer registers for its single-workgroup problems)
-  constexpr bool kFuse = sizeof(T) == sizeof(double);
+  // the Nesterov point is formed inside the line-search trials in both stages (with the
+  // uniform state in scalar registers the FP32 stage keeps 3 waves per SIMD with it:
+  // 153 VGPRs; mixed batch 103.5 -> 99.6 ms against a separate update phase)
+  constexpr bool kFuse = true;
   __shared__ int s_next;
   const int code = a.bmap[blockIdx.x];
   if (code < 0) return;  // a block no team uses
@@ -918,10 +921,14 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
   }
 }
 
-template __global__ void pg_kernel<false, double>(PGGridArgs<double>);
-template __global__ void pg_kernel<true, double>(PGGridArgs<double>);
-template __global__ void pg_kernel<false, float>(PGGridArgs<float>);
-template __global__ void pg_kernel<true, float>(PGGridArgs<float>);
+template __global__ void pg_kernel<false, double, false>(PGGridArgs<double>);
+template __global__ void pg_kernel<true, double, false>(PGGridArgs<double>);
+template __global__ void pg_kernel<false, float, false>(PGGridArgs<float>);
+template __global__ void pg_kernel<true, float, false>(PGGridArgs<float>);
+template __global__ void pg_kernel<false, double, true>(PGGridArgs<double>);
+template __global__ void pg_kernel<true, double, true>(PGGridArgs<double>);
+template __global__ void pg_kernel<false, float, true>(PGGridArgs<float>);
+template __global__ void pg_kernel<true, float, true>(PGGridArgs<float>);
 
 #define SB_BATCH_INST(MF, T, NT) \
   template __global__ void pg_batch_kernel<MF, T, NT>(PGBatchArgs<T>, const PGProblem<T>* __restrict__);

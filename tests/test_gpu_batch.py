@@ -93,7 +93,7 @@ def test_batch_warm_start_and_autotune(R, O, ctx, small_batch):
     assert res.config == cfg
     warm = B.solve(r0=res.r, **kw)
     # converged results stay converged; the FP32 stage may take a few iterations to
-    # get from the FP64-verified residual (< tol) below its own target (tol * 0.8)
+    # get from the FP64-verified residual (< tol) below its own target (tol * 0.95)
     assert warm.n_converged >= res.n_converged
     assert np.all(warm.converged[res.converged])
     assert warm.iterations[res.converged].sum() < 0.5 * res.iterations[res.converged].sum()

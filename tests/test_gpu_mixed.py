@@ -38,8 +38,8 @@ def test_mixed_matches_fp64_1k(G, R, O, ctx, variant, op):
     chk = op_.eval(m.r, coulomb=True)
     assert chk <= 1.05e-8
     assert m.residual == pytest.approx(chk, rel=1e-6)
-    # the FP32 stage reached tol * 0.8 on its own residual, or the FP64 stage refined
-    assert m.mixed_residual < 0.8e-8 or m.refine_iterations > 0
+    # the FP32 stage reached tol * 0.95 (the default factor) on its own residual, or the FP64 stage refined
+    assert m.mixed_residual < 0.95e-8 or m.refine_iterations > 0
     assert m.refine_spmvs >= 1  # the FP64 verification is at least one operator application
     u_m, u_d = _u(prob, m.r), _u(prob, d.r)
     assert np.linalg.norm(u_m - u_d) / np.linalg.norm(u_d) < 1e-2
