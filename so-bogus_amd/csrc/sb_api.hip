@@ -539,9 +539,8 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
 }
 
 template <typename T>
-const void* pg_grid_fn(bool mf, bool scalar) {
-  if (scalar) return mf ? (const void*)&sb::pg_kernel<true, T, true> : (const void*)&sb::pg_kernel<false, T, true>;
-  return mf ? (const void*)&sb::pg_kernel<true, T, false> : (const void*)&sb::pg_kernel<false, T, false>;
+const void* pg_grid_fn(bool mf) {
+  return mf ? (const void*)&sb::pg_kernel<true, T> : (const void*)&sb::pg_kernel<false, T>;
 }
 
 // Launches one persistent single-problem solver kernel (grid barrier state reset first)
@@ -1214,10 +1213,6 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
   const int threads = block_threads_for(o->block_threads, kMaxThreads);
   if (threads < 0) return SB_ERR_INVALID;
   double* dout = fixed ? fixed->out : P->out.p;
-  // several workgroups per CU: the kernel variant with the uniform state in scalar
-  // registers, which leaves room for them
-  static const int scalarEnv = std::getenv("SOBOGUS_GRID_SCALAR") ? std::atoi(std::getenv("SOBOGUS_GRID_SCALAR")) : -1;
-  const bool scalarState = scalarEnv >= 0 ? scalarEnv != 0 : false;
   int rc;
   ctx->lastPlain = false;
   SB_HIP(hipEventRecord(ctx->ev0, s));
@@ -1231,7 +1226,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     P->fill(a.pb, true);
     a.pb.x_io = x;
     a.pb.out = dout;
-    const void* fn = pg_grid_fn<float>(mf, scalarState);
+    const void* fn = pg_grid_fn<float>(mf);
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
     SB_HIP(hipEventRecord(ctx->evm, s));
   }
@@ -1246,7 +1241,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     a.pb.hist = histCap > 0 ? P->hist.p : nullptr;
     a.pb.hist_cap = histCap;
     a.pb.out = dout + 16;
-    const void* fn = pg_grid_fn<double>(mf, scalarState);
+    const void* fn = pg_grid_fn<double>(mf);
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
   }
   SB_HIP(hipEventRecord(ctx->ev1, s));

@@ -819,18 +819,18 @@ __device__ __forceinline__ void pg_body_sc(const PGParams& prm, const PGProblem<
 
 // ------------------------------------------------------------------ kernels
 
-// One problem on the whole co-resident grid. Scalar: the solver's uniform state in
-// scalar registers (fewer vector registers: more waves per SIMD for grids of several
-// workgroups per CU).
-template <bool MF, typename T, bool Scalar>
+// One problem on the whole co-resident grid (the solver state stays in vector
+// registers: at one workgroup per CU the scalar variant's spills cost 3-4 %, and more
+// workgroups per CU did not shorten an operator application).
+template <bool MF, typename T>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
-  GridTeamT<true, false, Scalar> tm(a.gs, blockIdx.x);
+  GridTeam tm(a.gs, blockIdx.x);
   if (a.prm.variant == 0 || a.prm.variant == 2) {  // Standard, Conjugated
-    pg_body_sc<MF, T, GridTeamT<true, false, Scalar>>(a.prm, a.pb, tm);
+    pg_body_sc<MF, T, GridTeam>(a.prm, a.pb, tm);
     return;
   }
   uint64_t tsum[4] = {0, 0, 0, 0};
-  pg_body<MF, T, GridTeamT<true, false, Scalar>, true>(a.prm, a.pb, tm, tsum);
+  pg_body<MF, T, GridTeam, true>(a.prm, a.pb, tm, tsum);
 }
 
 // Teams of several workgroups are placed on one XCD. They write their vectors (and
@@ -921,14 +921,10 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
   }
 }
 
-template __global__ void pg_kernel<false, double, false>(PGGridArgs<double>);
-template __global__ void pg_kernel<true, double, false>(PGGridArgs<double>);
-template __global__ void pg_kernel<false, float, false>(PGGridArgs<float>);
-template __global__ void pg_kernel<true, float, false>(PGGridArgs<float>);
-template __global__ void pg_kernel<false, double, true>(PGGridArgs<double>);
-template __global__ void pg_kernel<true, double, true>(PGGridArgs<double>);
-template __global__ void pg_kernel<false, float, true>(PGGridArgs<float>);
-template __global__ void pg_kernel<true, float, true>(PGGridArgs<float>);
+template __global__ void pg_kernel<false, double>(PGGridArgs<double>);
+template __global__ void pg_kernel<true, double>(PGGridArgs<double>);
+template __global__ void pg_kernel<false, float>(PGGridArgs<float>);
+template __global__ void pg_kernel<true, float>(PGGridArgs<float>);
 
 #define SB_BATCH_INST(MF, T, NT) \
   template __global__ void pg_batch_kernel<MF, T, NT>(PGBatchArgs<T>, const PGProblem<T>* __restrict__);

@@ -37,7 +37,7 @@ hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_reset_sync(unsigned* words, int n, int* error, hipStream_t s);
 
-template <bool MF, typename T, bool Scalar>
+template <bool MF, typename T>
 __global__ void pg_kernel(PGGridArgs<T> a);
 template <bool MF, typename T, int NT>
 __global__ void pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs);
