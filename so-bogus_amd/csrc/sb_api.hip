@@ -83,8 +83,11 @@ struct DevTiles {
   DBuf<double> vals;
   DBuf<float> vals32;  // FP32 copy for the mixed-precision solvers
   sbhost::Tiles host;  // index arrays kept (values dropped after upload)
+  bool pairs = false;  // every row exactly two blocks, in order (the contact rows of H)
   hipError_t upload(sbhost::Tiles&& t, hipStream_t s) {
     host = std::move(t);
+    pairs = host.nblocks == 2 * host.nrows && host.long_row.empty() && host.nblocks % 2 == 0;
+    for (int i = 0; pairs && i <= host.nrows; ++i) pairs = host.rowptr[i] == 2 * i;
     hipError_t e;
     if ((e = tstart.upload(host.tstart, s)) != hipSuccess) return e;
     if ((e = brow.upload(host.brow, s)) != hipSuccess) return e;
@@ -117,6 +120,7 @@ struct DevTiles {
     m.bcol = bcol.p;
     m.tlong = tlong.p;
     m.vals = v;
+    m.pairs = pairs ? 1 : 0;
     m.nlong = (int)host.long_row.size();
     m.long_row = long_row.p;
     m.long_t0 = long_t0.p;

@@ -191,9 +191,48 @@ hipError_t launch_w(const DevProblem<double>& P, const double* x, double* y, dou
   run_tiles<3, 3>(P.W, VecLoader3{x}, Store3{y}, carry, s);
   return hipGetLastError();
 }
+// y = H w for an H whose rows all hold two blocks (a contact between two bodies),
+// blocks 2i and 2i + 1 for row i: one lane per row, no scan. Element e of the two
+// blocks is adjacent in the element-major layout, so a lane reads both with one
+// 16-byte non-temporal load (a wave: 1 KB contiguous per element). The row sum is
+// formed as in the tiled kernel (each block's products in order, then the second
+// block's sum plus the first's): the same bits.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) k_h_pairs(TileMat<double> A, const double* __restrict__ w,
+                                                 double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.nrows) return;
+  const size_t nb2 = (size_t)A.nblocks / 2;
+  const dbl2* v2 = reinterpret_cast<const dbl2*>(A.vals) + i;
+  const int ca = A.bcol[2 * i], cb = A.bcol[2 * i + 1];
+  dbl2 e[18];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) e[k] = __builtin_nontemporal_load(v2 + (size_t)k * nb2);
+  double xa[6], xb[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    xa[c] = w[6 * (size_t)ca + c];
+    xb[c] = w[6 * (size_t)cb + c];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double aa = 0.0, ab = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      aa += e[r * 6 + c].x * xa[c];
+      ab += e[r * 6 + c].y * xb[c];
+    }
+    y[3 * (size_t)i + r] = ab + aa;
+  }
+}
+
 hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, double* carry, hipStream_t s) {
   (void)hipGetLastError();  // clear a stale error: report this launch only
-  run_tiles<3, 6>(P.H, VecLoader6{w}, Store3{y}, carry, s);
+  if (P.H.pairs) {
+    if (P.H.nrows > 0) k_h_pairs<<<(P.H.nrows + 255) / 256, 256, 0, s>>>(P.H, w, y);
+  } else {
+    run_tiles<3, 6>(P.H, VecLoader6{w}, Store3{y}, carry, s);
+  }
   return hipGetLastError();
 }
 hipError_t launch_ht_minv(const DevProblem<double>& P, const double* r, double* w, double* carry, hipStream_t s) {
@@ -303,21 +342,23 @@ hipError_t launch_lincomb(size_t n, double a, const double* x, double c, const d
 }
 
 // Plain streaming copy (bandwidth and counter calibration).
-__global__ void __launch_bounds__(256) k_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst, size_t n) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-    dst[k] = src[k];
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + k), dst + k);
 }
 
-// One 32-byte element per thread over a grid that covers the buffer (grid-stride
-// only beyond 2^22 workgroups): measured on MI355X with 1 GiB each way, 5.78 TB/s
-// against 5.0-5.4 TB/s for 1k-16k workgroups that loop (with 1, 2, 4 or 8 loads in
-// flight per thread, or a contiguous chunk per workgroup) and 4.85 TB/s for
-// hipMemcpyAsync.
+// One 16-byte element per thread, non-temporal both ways, over a grid that covers
+// the buffer (grid-stride only beyond 2^22 workgroups): measured on MI355X with
+// 1 GiB each way, 6.5 TB/s, against 6.2 TB/s for plain 16-byte accesses, 5.8 TB/s
+// for 32-byte ones, 5.0-5.4 TB/s for 1k-16k workgroups that loop and 4.85 TB/s for
+// hipMemcpyAsync. bytes: a multiple of 16.
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
   (void)hipGetLastError();  // clear a stale error: report this launch only
-  const size_t n = bytes / sizeof(double4);
+  if (bytes % 16 != 0) return hipErrorInvalidValue;
+  const size_t n = bytes / 16;
   const size_t grid = std::min<size_t>((n + 255) / 256, (size_t)1 << 22);
-  k_copy<<<dim3((unsigned)std::max<size_t>(grid, 1)), 256, 0, s>>>((const double4*)src, (double4*)dst, n);
+  k_copy<<<dim3((unsigned)std::max<size_t>(grid, 1)), 256, 0, s>>>((const f32x4*)src, (f32x4*)dst, n);
   return hipGetLastError();
 }
 

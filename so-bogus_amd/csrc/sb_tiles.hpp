@@ -30,6 +30,7 @@ struct TileMat {
   const int* tlong = nullptr;  // [ntiles] < 0: whole rows, -(1 + scan steps of the tile);
                                // >= 0: the tile holds a piece of that long row
   const T* vals = nullptr;     // [BR*BC][nblocks]
+  int pairs = 0;               // 1: every row holds exactly two blocks, row i's at 2i and 2i + 1
   int nlong = 0;               // long rows
   const int* long_row = nullptr;  // [nlong]
   const int* long_t0 = nullptr;   // [nlong] first tile

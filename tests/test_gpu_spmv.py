@@ -93,6 +93,25 @@ def test_friction_operators_bit_exact_on_integers(R, O, G, ctx):
     assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(v, ov)
 
 
+def test_friction_operators_ragged_contact_rows(R, O, G, ctx):
+    """Contacts with one body (every fifth row keeps only its first block) take the
+    tiled H kernel; rows of two blocks take the one-lane-per-contact kernel. Both
+    bit-exact on integer data."""
+    print("case: W x / matrix-free on integer data with one-body contacts (nc=3000)", flush=True)
+    p = _int_problem(G, 3000, 1200, 6)
+    keep = np.ones(2 * p.nc, bool)
+    keep[2 * np.arange(0, p.nc, 5) + 1] = False
+    p.colidx = np.ascontiguousarray(p.colidx[keep])
+    p.hvals = np.ascontiguousarray(p.hvals[keep])
+    p.rowptr = np.concatenate([[0], np.cumsum(keep.reshape(-1, 2).sum(axis=1))]).astype(np.int32)
+    dp = R.DualFrictionProblem(ctx, p)
+    op = O.OracleProblem(p)
+    x = rng.integers(-4, 5, size=3 * p.nc).astype(float)
+    ref = op.apply_w(x, assembled=True)
+    assert np.array_equal(dp.apply_w(x, R.OP_ASSEMBLED), ref)
+    assert np.array_equal(dp.apply_w(x, R.OP_MATRIX_FREE), ref)
+
+
 def test_friction_operators_random(R, O, G, ctx):
     print("case: W x / matrix-free random (nc=20000)", flush=True)
     p = G.generate(nc=20000, nb=8000, seed=12)
