@@ -318,14 +318,19 @@ def adjoint_100k(ctx, R, G, n_dirs: int = 64) -> dict:
 
 
 def setup_times(ctx, R, G) -> dict:
-    """Per-step set-up cost of a contact problem whose pattern is unchanged: creating
-    the problem on the host (tiling, transposition, W assembly, upload) against
-    refreshing its values on the device (sb_problem_update_values: H and H^T tiles,
-    Minv, diagonal blocks, scalings, W), with the new values in host memory (PCIe
-    included) or already in HBM."""
+    """Set-up cost of a contact problem. A new contact set: creating the problem with
+    the set-up on the device (default: tiles, transpose, W pattern and values built by
+    kernels; the arrays in host memory, so the PCIe upload is included, or already in
+    HBM) against the set-up on the host. An unchanged pattern: refreshing the values on
+    the device (sb_problem_update_values), from host memory or from HBM."""
     out = {}
     for name, assemble in (("contacts-50k", True), ("zipf-1m", False)):
         p = G.workload(name)
+        t0 = time.perf_counter()
+        dp = R.DualFrictionProblem(ctx, p, assemble=assemble, host_setup=True)
+        ctx.synchronize()
+        create_host = time.perf_counter() - t0
+        del dp
         t0 = time.perf_counter()
         dp = R.DualFrictionProblem(ctx, p, assemble=assemble)
         ctx.synchronize()
@@ -335,6 +340,14 @@ def setup_times(ctx, R, G) -> dict:
             d = R.DeviceArray(ctx, np.asarray(a).size)
             d.upload(np.asarray(a, dtype=np.float64).reshape(-1))
             arrs.append(d)
+        irp = R.DeviceArray(ctx, p.nc + 1, np.int32).upload(p.rowptr)
+        ici = R.DeviceArray(ctx, np.asarray(p.colidx).size, np.int32).upload(p.colidx)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        dd = R.DualFrictionProblem.from_device(ctx, p.nc, p.nb, irp, ici, *arrs, assemble=assemble)
+        ctx.synchronize()
+        create_dev = time.perf_counter() - t0
+        del dd
         host, dev = [], []
         for _ in range(3):
             t0 = time.perf_counter()
@@ -343,12 +356,14 @@ def setup_times(ctx, R, G) -> dict:
             t0 = time.perf_counter()
             dp.update_values(*arrs)
             dev.append(time.perf_counter() - t0)
-        out[name] = {"assembled_w": assemble, "create_on_host_ms": 1e3 * create,
+        out[name] = {"assembled_w": assemble, "create_host_setup_ms": 1e3 * create_host,
+                     "create_device_setup_ms": 1e3 * create, "create_from_device_arrays_ms": 1e3 * create_dev,
                      "update_from_host_ms": 1e3 * float(np.median(host)),
                      "update_on_device_ms": 1e3 * float(np.median(dev))}
-        log(f"set-up {name}: create {1e3 * create:.1f} ms, device update {out[name]['update_on_device_ms']:.2f} ms "
-            f"({out[name]['update_from_host_ms']:.2f} ms with the values in host memory)")
-        del dp, arrs
+        log(f"set-up {name}: create with the host set-up {1e3 * create_host:.1f} ms, device set-up "
+            f"{1e3 * create:.1f} ms ({1e3 * create_dev:.1f} ms from arrays in HBM); value refresh "
+            f"{out[name]['update_on_device_ms']:.2f} ms ({out[name]['update_from_host_ms']:.2f} ms from host memory)")
+        del dp, arrs, irp, ici
     return out
 
 

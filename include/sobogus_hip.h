@@ -80,9 +80,11 @@ enum sb_precision {
 enum sb_problem_flags {
   SB_PROBLEM_ASSEMBLE_W = 1,
   SB_BATCH_CALLER_ORDER = 2, /* sb_batch_create: store each problem in the caller's contact and body order */
-  SB_BATCH_SPREAD_TEAMS = 4  /* sb_batch_create: place a team's workgroups on consecutive blocks (spread
+  SB_BATCH_SPREAD_TEAMS = 4, /* sb_batch_create: place a team's workgroups on consecutive blocks (spread
                                 over the XCDs) instead of on one XCD; the teams' barriers then write back
                                 and invalidate L2. Results are bit-identical; slower. For tests. */
+  SB_PROBLEM_HOST_SETUP = 8   /* sb_problem_create: build the tiles, the transpose of H and W's pattern on the
+                                 host (the default builds them on the device; both give the same arrays) */
 };
 /* BSR flags (bogus::flags) */
 enum sb_bsr_flags { SB_BSR_SYMMETRIC = 1 /* only blocks with col <= row are given */ };
@@ -192,10 +194,32 @@ int sb_bsr_split_row_multiply(sb_ctx* ctx, const sb_bsr* A, int row, const doubl
 /* A dual problem u = W r + b with W = H Minv H^T, built from the primal data:
    H: n_contacts x n_bodies BSR of 3x6 blocks (row-major), Minv: n_bodies 6x6 blocks
    (explicit inverse mass, row-major), b: 3 n_contacts, mu: n_contacts.
-   With SB_PROBLEM_ASSEMBLE_W the library also assembles W (DualFrictionProblem::computeFrom). */
+   With SB_PROBLEM_ASSEMBLE_W the library also assembles W (DualFrictionProblem::computeFrom).
+   The arrays are copied to the device and the set-up (tiles of H and H^T, the stable transpose
+   of H, W's pattern, product terms and values, diagonal blocks) runs there, unless
+   SB_PROBLEM_HOST_SETUP is given; host copies of the pattern and values are made on demand by
+   the calls that need them (coloring, W export, permutation, host updates). */
 int sb_problem_create(sb_ctx* ctx, sb_problem** P, int n_contacts, int n_bodies, const int32_t* h_rowptr,
                       const int32_t* h_colidx, const double* h_vals, const double* minv, const double* b,
                       const double* mu, int flags);
+/* The same from device arrays (a contact set produced on the GPU): checked on the device
+   (row pointer non-decreasing from 0, columns in range: SB_ERR_INVALID otherwise), copied,
+   and set up there. Replaces DualFrictionProblem::computeFrom (FrictionProblem.cpp:30-61) with
+   its transpose (SparseBlockMatrixBase.impl.hpp:169-202) and product pattern
+   (SparseMatrixMatrixProduct.impl.hpp:124-212) for a changed contact set. */
+int sb_problem_create_device(sb_ctx* ctx, sb_problem** P, int n_contacts, int n_bodies, const int32_t* d_rowptr,
+                             const int32_t* d_colidx, const double* d_vals, const double* d_minv, const double* d_b,
+                             const double* d_mu, int flags);
+/* The tile layout the solvers use for H (which 0), H^T (1) or W with both triangles (2; only
+   with an assembled W): sizes[6] = rows, blocks, tiles, long rows, scan steps, 1 if every row
+   holds two blocks; then, each optional (NULL: skipped), tstart (tiles + 1), brow, bcol (blocks),
+   tlong (tiles), long_rows (3 x long rows: row, first tile, end tile), rowptr (rows + 1), src
+   (blocks: H and H^T the H block, W the block of the two-triangle W; -1 an empty row's zero
+   block) and vals ([element][block], 18 or 9 elements). No counterpart in the reference (it
+   has no tiled layout): for tests and tools that compare set-ups. */
+int sb_problem_layout(sb_ctx* ctx, const sb_problem* P, int which, int64_t* sizes, int32_t* tstart, int32_t* brow,
+                      int32_t* bcol, int32_t* tlong, int32_t* long_rows, int32_t* rowptr, int32_t* src,
+                      double* vals);
 /* Supplies an assembled W (lower triangle, 3x3 row-major blocks) instead of assembling it. */
 int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* w_rowptr, const int32_t* w_colidx,
                      const double* w_vals);

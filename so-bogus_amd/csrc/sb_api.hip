@@ -1,6 +1,8 @@
 // extern "C" layer of the MI355X friction solve path (include/sobogus_hip.h).
-// Host setup (tiling, W assembly, coloring) happens here; every solve, product
-// and residual runs as HIP kernels on the context's stream.
+// Problem set-up (tiling, transposition, W) is driven from here, by default on the
+// device (sb_symbolic.hip, sb_setup.hip), else on the host (sb_host.cpp); the
+// coloring runs on the host; every solve, product and residual runs as HIP
+// kernels on the context's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -20,6 +23,7 @@
 #include "sb_host.hpp"
 #include "sb_problem.hpp"
 #include "sb_spmv.hpp"
+#include "sb_symbolic.hpp"
 
 using sb::DevProblem;
 using sb::TileMat;
@@ -82,10 +86,14 @@ struct DevTiles {
   DBuf<int> tstart, brow, bcol, tlong, long_row, long_t0, long_t1, rowptr;
   DBuf<double> vals;
   DBuf<float> vals32;  // FP32 copy for the mixed-precision solvers
-  sbhost::Tiles host;  // index arrays kept (values dropped after upload)
+  sbhost::Tiles host;  // sizes, and the index arrays while hostArrays (values dropped after upload)
   bool pairs = false;  // every row exactly two blocks, in order (the contact rows of H)
+  int nlong = 0;       // long rows
+  bool hostArrays = true;  // false after a build on the device, until download()
   hipError_t upload(sbhost::Tiles&& t, hipStream_t s) {
     host = std::move(t);
+    nlong = (int)host.long_row.size();
+    hostArrays = true;
     pairs = host.nblocks == 2 * host.nrows && host.long_row.empty() && host.nblocks % 2 == 0;
     for (int i = 0; pairs && i <= host.nrows; ++i) pairs = host.rowptr[i] == 2 * i;
     hipError_t e;
@@ -121,7 +129,7 @@ struct DevTiles {
     m.tlong = tlong.p;
     m.vals = v;
     m.pairs = pairs ? 1 : 0;
-    m.nlong = (int)host.long_row.size();
+    m.nlong = nlong;
     m.long_row = long_row.p;
     m.long_t0 = long_t0.p;
     m.long_t1 = long_t1.p;
@@ -130,7 +138,115 @@ struct DevTiles {
   TileMat<double> view() const { return viewT<double>(vals.p); }
   TileMat<float> view32() const { return viewT<float>(vals32.p); }
   size_t carryNeed(int br) const { return (size_t)host.ntiles * br; }
+  // host copies of the index arrays of a layout built on the device
+  hipError_t download(hipStream_t s) {
+    if (hostArrays) return hipSuccess;
+    auto get = [&](std::vector<int>& v, const DBuf<int>& d, size_t n) -> hipError_t {
+      v.resize(n);
+      return n ? hipMemcpyAsync(v.data(), d.p, sizeof(int) * n, hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    hipError_t e;
+    if ((e = get(host.tstart, tstart, (size_t)host.ntiles + 1)) != hipSuccess) return e;
+    if ((e = get(host.brow, brow, (size_t)host.nblocks)) != hipSuccess) return e;
+    if ((e = get(host.bcol, bcol, (size_t)host.nblocks)) != hipSuccess) return e;
+    if ((e = get(host.tlong, tlong, (size_t)host.ntiles)) != hipSuccess) return e;
+    if ((e = get(host.long_row, long_row, (size_t)nlong)) != hipSuccess) return e;
+    if ((e = get(host.long_t0, long_t0, (size_t)nlong)) != hipSuccess) return e;
+    if ((e = get(host.long_t1, long_t1, (size_t)nlong)) != hipSuccess) return e;
+    if ((e = get(host.rowptr, rowptr, (size_t)host.nrows + 1)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    hostArrays = true;
+    return hipSuccess;
+  }
 };
+
+int cub_tmp(DBuf<char>& tmp, size_t bytes) {
+  if (tmp.n < bytes) SB_HIP(tmp.alloc(bytes));
+  return SB_OK;
+}
+int read_int(const int* d, int* h, hipStream_t s) {
+  SB_HIP(hipMemcpyAsync(h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  return SB_OK;
+}
+// out[0] = 0, out[1..n] = inclusive prefix sums of in (n + 1 entries)
+int scan_exclusive(const int* in, DBuf<int>& out, int n, DBuf<char>& tmp, hipStream_t s) {
+  SB_HIP(out.alloc((size_t)n + 1));
+  SB_HIP(hipMemsetAsync(out.p, 0, sizeof(int), s));
+  if (n == 0) return SB_OK;
+  size_t bytes = 0;
+  SB_HIP(sb::scan_inclusive_int(in, out.p + 1, n, nullptr, bytes, s));
+  const int rc = cub_tmp(tmp, bytes);
+  if (rc) return rc;
+  SB_HIP(sb::scan_inclusive_int(in, out.p + 1, n, tmp.p, bytes, s));
+  return SB_OK;
+}
+
+// The tile layout of a BSR pattern (rowptr rp, columns ci: device arrays) built on
+// the device, element for element the layout sbhost::buildTiles builds on the host;
+// src receives each tile block's source block (-1: an empty row's zero block). The
+// values are allocated, not filled.
+int build_tiles_device(DevTiles& T, int nrows, int ncols, int br, int bc, const int* rp, const int* ci,
+                       DBuf<int>& src, DBuf<char>& tmp, hipStream_t s) {
+  int rc;
+  DBuf<int> tlen, next, ntl, nlg, tbase, lbase, entry, mx;
+  DBuf<unsigned char> isStart, exits;
+  SB_HIP(tlen.alloc(nrows));
+  SB_HIP(sb::launch_row_len(nrows, rp, tlen.p, s));
+  if ((rc = scan_exclusive(tlen.p, T.rowptr, nrows, tmp, s))) return rc;
+  SB_HIP(mx.alloc(1));
+  {
+    size_t bytes = 0;
+    SB_HIP(sb::reduce_max_int(tlen.p, mx.p, nrows, nullptr, bytes, s));
+    if ((rc = cub_tmp(tmp, bytes))) return rc;
+    SB_HIP(sb::reduce_max_int(tlen.p, mx.p, nrows, tmp.p, bytes, s));
+  }
+  int nblocks = 0, maxLen = 0;
+  SB_HIP(hipMemcpyAsync(&maxLen, mx.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  if ((rc = read_int(T.rowptr.p + nrows, &nblocks, s))) return rc;
+  maxLen = std::min(maxLen, 64);
+  SB_HIP(T.brow.alloc(nblocks));
+  SB_HIP(T.bcol.alloc(nblocks));
+  SB_HIP(src.alloc(nblocks));
+  SB_HIP(sb::launch_tile_expand(nrows, nblocks, T.rowptr.p, rp, ci, T.brow.p, T.bcol.p, src.p, s));
+  SB_HIP(next.alloc(nrows));
+  SB_HIP(sb::launch_tile_next(nrows, T.rowptr.p, tlen.p, next.p, s));
+  const int C = sb::chain_chunk_rows(nrows), nchunks = (nrows + C - 1) / C;
+  SB_HIP(exits.alloc((size_t)nchunks * 64));
+  SB_HIP(entry.alloc(nchunks));
+  SB_HIP(isStart.alloc(nrows));
+  SB_HIP(sb::launch_tile_chain(nrows, next.p, exits.p, entry.p, isStart.p, s));
+  SB_HIP(ntl.alloc(nrows));
+  SB_HIP(nlg.alloc(nrows));
+  SB_HIP(sb::launch_tile_counts(nrows, tlen.p, isStart.p, ntl.p, nlg.p, s));
+  if ((rc = scan_exclusive(ntl.p, tbase, nrows, tmp, s))) return rc;
+  if ((rc = scan_exclusive(nlg.p, lbase, nrows, tmp, s))) return rc;
+  int ntiles = 0, nlong = 0;
+  SB_HIP(hipMemcpyAsync(&ntiles, tbase.p + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
+  if ((rc = read_int(lbase.p + nrows, &nlong, s))) return rc;
+  SB_HIP(T.tstart.alloc((size_t)ntiles + 1));
+  SB_HIP(T.tlong.alloc(ntiles));
+  SB_HIP(T.long_row.alloc(nlong));
+  SB_HIP(T.long_t0.alloc(nlong));
+  SB_HIP(T.long_t1.alloc(nlong));
+  SB_HIP(sb::launch_tile_fill(nrows, nblocks, ntiles, T.rowptr.p, tlen.p, next.p, isStart.p, tbase.p, lbase.p,
+                              T.tstart.p, T.tlong.p, T.long_row.p, T.long_t0.p, T.long_t1.p, s));
+  SB_HIP(T.vals.alloc((size_t)nblocks * br * bc));
+  SB_HIP(T.vals32.alloc((size_t)nblocks * br * bc));
+  SB_HIP(hipStreamSynchronize(s));  // the local buffers are freed on return
+  T.host = sbhost::Tiles();
+  T.host.nrows = nrows;
+  T.host.ncols = ncols;
+  T.host.br = br;
+  T.host.bc = bc;
+  T.host.nblocks = nblocks;
+  T.host.ntiles = ntiles;
+  T.host.scan_steps = sb::ceil_log2_host(std::max(1, maxLen));
+  T.nlong = nlong;
+  T.pairs = nblocks == 2 * nrows && maxLen == 2;
+  T.hostArrays = false;
+  return SB_OK;
+}
 
 }  // namespace
 
@@ -183,6 +299,14 @@ struct sb_problem {
   std::vector<int> htsrc;
   bool devSetup = false, hostStale = false, devRawValid = false, wFromH = true;
   DBuf<int> d_hsrc, d_htsrc, d_wtptr, d_wterm, d_hrp, d_hci;
+  // pattern built on the device (sb_problem_create_device): the host copies of the
+  // pattern (hrp, hci, wrp, wci, the tiles' index arrays, htsrc) are downloaded by
+  // the host paths that need them (hostPattern false until then); the lower
+  // triangle of W and its tiles' source blocks stay on the device
+  bool hostPattern = true;
+  int nnz = 0;        // blocks of H
+  int64_t nwl = 0;    // blocks of W's lower triangle (0 without W)
+  DBuf<int> d_wrp, d_wci, d_wsrc;
   DBuf<double> d_hraw, d_mraw;
   // workspace of the FP32 (mixed-precision) projected-gradient solver
   DBuf<float> X32[4], MX32[4], G32[3], xbest32, wbody32, carry32;
@@ -344,25 +468,9 @@ int upload_values(sb_problem* P) {
   return SB_OK;
 }
 
-int upload_problem(sb_problem* P) {
-  hipStream_t s = P->ctx->stream;
+// Solver workspace of a problem (FP64 and FP32 iterates, products, carries).
+int alloc_workspace(sb_problem* P) {
   const int nc = P->nc, nb = P->nb;
-  ++P->version;
-  P->devSetup = false;  // index maps follow the (possibly new) pattern
-  // H contact-major, H^T body-major
-  SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
-  {
-    std::vector<int> trp, tci, tsrc;
-    std::vector<double> tv;
-    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv, &tsrc);
-    sbhost::Tiles ht = sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data());
-    P->htsrc.assign(ht.src.size(), -1);
-    for (size_t q = 0; q < ht.src.size(); ++q)
-      if (ht.src[q] >= 0) P->htsrc[q] = tsrc[ht.src[q]];
-    SB_HIP(P->HT.upload(std::move(ht), s));
-  }
-  int rc = upload_values(P);
-  if (rc) return rc;
   const size_t n3 = 3 * (size_t)nc;
   for (int k = 0; k < 4; ++k) {
     SB_HIP(P->X[k].alloc(n3));
@@ -385,6 +493,32 @@ int upload_problem(sb_problem* P) {
   SB_HIP(P->wbody32.alloc(6 * (size_t)nb));
   SB_HIP(P->carry32.alloc(carry));
   SB_HIP(P->out.alloc(32));
+  return SB_OK;
+}
+
+int upload_problem(sb_problem* P) {
+  hipStream_t s = P->ctx->stream;
+  const int nc = P->nc, nb = P->nb;
+  ++P->version;
+  P->devSetup = false;  // index maps follow the (possibly new) pattern
+  P->hostPattern = true;
+  P->nnz = P->hrp[nc];
+  P->nwl = P->hasW ? (int64_t)P->wci.size() : 0;
+  // H contact-major, H^T body-major
+  SB_HIP(P->H.upload(sbhost::buildTiles(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data()), s));
+  {
+    std::vector<int> trp, tci, tsrc;
+    std::vector<double> tv;
+    sbhost::transposeBSR(nc, nb, 3, 6, P->hrp.data(), P->hci.data(), P->hv.data(), trp, tci, tv, &tsrc);
+    sbhost::Tiles ht = sbhost::buildTiles(nb, nc, 6, 3, trp.data(), tci.data(), tv.data());
+    P->htsrc.assign(ht.src.size(), -1);
+    for (size_t q = 0; q < ht.src.size(); ++q)
+      if (ht.src[q] >= 0) P->htsrc[q] = tsrc[ht.src[q]];
+    SB_HIP(P->HT.upload(std::move(ht), s));
+  }
+  int rc = upload_values(P);
+  if (rc) return rc;
+  if ((rc = alloc_workspace(P))) return rc;
   SB_HIP(hipStreamSynchronize(s));
   return SB_OK;
 }
@@ -628,13 +762,53 @@ __global__ void k_split_row(const int* lrp, const int* lci, const double* lv, in
 }  // namespace
 
 namespace {
+// Host copies of a pattern built on the device: H's row pointer and columns, the
+// tiles' index arrays and source blocks, and W's lower-triangle pattern.
+int sync_pattern(sb_problem* P) {
+  if (P->hostPattern) return SB_OK;
+  hipStream_t s = P->ctx->stream;
+  const int nc = P->nc;
+  P->hrp.resize((size_t)nc + 1);
+  P->hci.resize((size_t)P->nnz);
+  SB_HIP(hipMemcpyAsync(P->hrp.data(), P->d_hrp.p, sizeof(int) * P->hrp.size(), hipMemcpyDeviceToHost, s));
+  if (P->nnz)
+    SB_HIP(hipMemcpyAsync(P->hci.data(), P->d_hci.p, sizeof(int) * P->hci.size(), hipMemcpyDeviceToHost, s));
+  SB_HIP(P->H.download(s));
+  SB_HIP(P->HT.download(s));
+  P->H.host.src.resize((size_t)P->H.host.nblocks);
+  P->htsrc.resize((size_t)P->HT.host.nblocks);
+  SB_HIP(hipMemcpyAsync(P->H.host.src.data(), P->d_hsrc.p, sizeof(int) * P->H.host.src.size(),
+                        hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(P->htsrc.data(), P->d_htsrc.p, sizeof(int) * P->htsrc.size(), hipMemcpyDeviceToHost, s));
+  if (P->hasW) {
+    SB_HIP(P->W.download(s));
+    P->W.host.src.resize((size_t)P->W.host.nblocks);
+    SB_HIP(hipMemcpyAsync(P->W.host.src.data(), P->d_wsrc.p, sizeof(int) * P->W.host.src.size(),
+                          hipMemcpyDeviceToHost, s));
+    P->wrp.resize((size_t)nc + 1);
+    P->wci.resize((size_t)P->nwl);
+    SB_HIP(hipMemcpyAsync(P->wrp.data(), P->d_wrp.p, sizeof(int) * P->wrp.size(), hipMemcpyDeviceToHost, s));
+    if (P->nwl)
+      SB_HIP(hipMemcpyAsync(P->wci.data(), P->d_wci.p, sizeof(int) * P->wci.size(), hipMemcpyDeviceToHost, s));
+  }
+  SB_HIP(hipStreamSynchronize(s));
+  P->hostPattern = true;
+  return SB_OK;
+}
+
 // Host copies of the values after device-side updates (paths that rebuild or read
 // them on the host: permutation, W export, host updates, GS local matrices, the
 // matrix-free GS's per-block products).
 int sync_host(sb_problem* P) {
+  int rc = sync_pattern(P);
+  if (rc) return rc;
   if (!P->hostStale) return SB_OK;
   hipStream_t s = P->ctx->stream;
   const int nc = P->nc, nb = P->nb;
+  P->hv.resize((size_t)P->nnz * 18);
+  P->minv.resize((size_t)nb * 36);
+  P->b.resize((size_t)nc * 3);
+  P->mu.resize((size_t)nc);
   SB_HIP(hipMemcpyAsync(P->hv.data(), P->d_hraw.p, sizeof(double) * P->hv.size(), hipMemcpyDeviceToHost, s));
   SB_HIP(hipMemcpyAsync(P->minv.data(), P->d_mraw.p, sizeof(double) * 36 * (size_t)nb, hipMemcpyDeviceToHost, s));
   SB_HIP(hipMemcpyAsync(P->b.data(), P->d_b.p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToHost, s));
@@ -684,11 +858,152 @@ int setup_maps(sb_problem* P) {
   P->devSetup = true;
   return SB_OK;
 }
+
+// The pattern-dependent set-up on the device, from the raw arrays in d_hrp, d_hci,
+// d_hraw, d_mraw, d_b, d_mu (nc, nb, nnz and hasW set): tiles of H, the stable
+// transpose and tiles of H^T, with W the pattern of its lower triangle, both
+// triangles' tiles and the product terms of its blocks; then the numeric phase
+// (tile values, Minv layout, diagonal blocks, scalings, W) and the workspace.
+// Every array equals what the host set-up (upload_problem) builds.
+int build_device(sb_problem* P) {
+  hipStream_t s = P->ctx->stream;
+  const int nc = P->nc, nb = P->nb, nnz = P->nnz;
+  int rc;
+  DBuf<char> tmp;
+  ++P->version;
+  if ((rc = build_tiles_device(P->H, nc, nb, 3, 6, P->d_hrp.p, P->d_hci.p, P->d_hsrc, tmp, s))) return rc;
+  // H^T: H's blocks sorted by body, stably (within a body, in contact order)
+  DBuf<unsigned> tkeys;
+  DBuf<int> iota, tsrc, trp, tci, htsrcT;
+  SB_HIP(tkeys.alloc(nnz));
+  SB_HIP(iota.alloc(nnz));
+  SB_HIP(tsrc.alloc(nnz));
+  SB_HIP(trp.alloc((size_t)nb + 1));
+  SB_HIP(tci.alloc(nnz));
+  SB_HIP(sb::launch_iota(nnz, iota.p, s));
+  if (nnz > 0) {
+    const int bits = std::max(1, sb::ceil_log2_host(nb));
+    size_t bytes = 0;
+    const unsigned* keys = reinterpret_cast<const unsigned*>(P->d_hci.p);  // columns are non-negative
+    SB_HIP(sb::sort_pairs_u32(keys, tkeys.p, iota.p, tsrc.p, nnz, bits, nullptr, bytes, s));
+    if ((rc = cub_tmp(tmp, bytes))) return rc;
+    SB_HIP(sb::sort_pairs_u32(keys, tkeys.p, iota.p, tsrc.p, nnz, bits, tmp.p, bytes, s));
+  }
+  SB_HIP(sb::launch_lower_bounds_u32(nb + 1, tkeys.p, nnz, trp.p, s));
+  SB_HIP(sb::launch_rows_of(nnz, tsrc.p, nc, P->d_hrp.p, tci.p, s));
+  if ((rc = build_tiles_device(P->HT, nb, nc, 6, 3, trp.p, tci.p, htsrcT, tmp, s))) return rc;
+  SB_HIP(P->d_htsrc.alloc((size_t)P->HT.host.nblocks));
+  SB_HIP(sb::launch_compose_src(P->HT.host.nblocks, htsrcT.p, tsrc.p, P->d_htsrc.p, s));
+  P->nwl = 0;
+  if (P->hasW) {
+    // lower triangle: for contact i, the contacts j <= i sharing a body (sorted, unique)
+    DBuf<int> cnt, cptr, flags, pos, uptr, flen, frp, fci, tcnt;
+    DBuf<uint64_t> keys, skeys, ukeys, lk, slk;
+    DBuf<long long> total;
+    SB_HIP(cnt.alloc(nc));
+    SB_HIP(sb::launch_w_cand_count(nc, P->d_hrp.p, P->d_hci.p, trp.p, tci.p, cnt.p, s));
+    SB_HIP(total.alloc(1));
+    {
+      size_t bytes = 0;
+      SB_HIP(sb::reduce_sum_int64(cnt.p, total.p, nc, nullptr, bytes, s));
+      if ((rc = cub_tmp(tmp, bytes))) return rc;
+      SB_HIP(sb::reduce_sum_int64(cnt.p, total.p, nc, tmp.p, bytes, s));
+    }
+    long long ncand = 0;
+    SB_HIP(hipMemcpyAsync(&ncand, total.p, sizeof(long long), hipMemcpyDeviceToHost, s));
+    SB_HIP(hipStreamSynchronize(s));
+    if (ncand > (long long)std::numeric_limits<int>::max() / 2) {
+      last_error_string() = "W = H Minv H^T: more than 2^30 contact pairs share a body";
+      return SB_ERR_LIMIT;
+    }
+    const int nk = (int)ncand;
+    if ((rc = scan_exclusive(cnt.p, cptr, nc, tmp, s))) return rc;
+    SB_HIP(keys.alloc(nk));
+    SB_HIP(skeys.alloc(nk));
+    SB_HIP(sb::launch_w_cand_fill(nc, P->d_hrp.p, P->d_hci.p, trp.p, tci.p, cptr.p, keys.p, s));
+    const uint64_t nn = (uint64_t)nc * (uint64_t)nc;
+    if (nk > 0) {
+      const int bits = std::max(1, sb::ceil_log2_host((int64_t)nn));
+      size_t bytes = 0;
+      SB_HIP(sb::sort_keys_u64(keys.p, skeys.p, nk, bits, nullptr, bytes, s));
+      if ((rc = cub_tmp(tmp, bytes))) return rc;
+      SB_HIP(sb::sort_keys_u64(keys.p, skeys.p, nk, bits, tmp.p, bytes, s));
+    }
+    SB_HIP(flags.alloc(nk));
+    SB_HIP(sb::launch_unique_flags(nk, skeys.p, flags.p, s));
+    if ((rc = scan_exclusive(flags.p, pos, nk, tmp, s))) return rc;
+    int nw = 0;
+    if ((rc = read_int(pos.p + nk, &nw, s))) return rc;
+    SB_HIP(ukeys.alloc(nw));
+    SB_HIP(sb::launch_unique_scatter(nk, skeys.p, pos.p + 1, ukeys.p, s));
+    SB_HIP(P->d_wrp.alloc((size_t)nc + 1));
+    SB_HIP(P->d_wci.alloc(nw));
+    SB_HIP(sb::launch_key_rows(nc, ukeys.p, nw, P->d_wrp.p, P->d_wci.p, s));
+    P->nwl = nw;
+    // both triangles: lower row i, then the entries (j, i) below the diagonal as (i, j)
+    SB_HIP(lk.alloc(nw));
+    SB_HIP(slk.alloc(nw));
+    SB_HIP(sb::launch_upper_keys(nc, P->d_wrp.p, P->d_wci.p, lk.p, s));
+    if (nw > 0) {
+      const int bits = std::max(1, sb::ceil_log2_host((int64_t)nn + 1));
+      size_t bytes = 0;
+      SB_HIP(sb::sort_keys_u64(lk.p, slk.p, nw, bits, nullptr, bytes, s));
+      if ((rc = cub_tmp(tmp, bytes))) return rc;
+      SB_HIP(sb::sort_keys_u64(lk.p, slk.p, nw, bits, tmp.p, bytes, s));
+    }
+    SB_HIP(uptr.alloc((size_t)nc + 1));
+    SB_HIP(flen.alloc(nc));
+    SB_HIP(sb::launch_upper_rows(nc, slk.p, nw, P->d_wrp.p, uptr.p, flen.p, s));
+    if ((rc = scan_exclusive(flen.p, frp, nc, tmp, s))) return rc;
+    int nf = 0;
+    if ((rc = read_int(frp.p + nc, &nf, s))) return rc;
+    SB_HIP(fci.alloc(nf));
+    SB_HIP(sb::launch_full_fill(nc, P->d_wrp.p, P->d_wci.p, uptr.p, slk.p, frp.p, fci.p, s));
+    if ((rc = build_tiles_device(P->W, nc, nc, 3, 3, frp.p, fci.p, P->d_wsrc, tmp, s))) return rc;
+    // product terms of every W tile block
+    const int nq = P->W.host.nblocks;
+    SB_HIP(tcnt.alloc(nq));
+    SB_HIP(sb::launch_w_term_count(nq, P->W.brow.p, P->W.bcol.p, P->d_wsrc.p, P->d_hrp.p, P->d_hci.p, tcnt.p, s));
+    if ((rc = scan_exclusive(tcnt.p, P->d_wtptr, nq, tmp, s))) return rc;
+    int nt = 0;
+    if ((rc = read_int(P->d_wtptr.p + nq, &nt, s))) return rc;
+    SB_HIP(P->d_wterm.alloc(std::max<size_t>(2, 2 * (size_t)nt)));
+    SB_HIP(sb::launch_w_term_fill(nq, P->W.brow.p, P->W.bcol.p, P->d_wsrc.p, P->d_hrp.p, P->d_hci.p,
+                                  P->d_wtptr.p, P->d_wterm.p, s));
+    SB_HIP(hipStreamSynchronize(s));  // the local buffers are freed on return
+  }
+  P->devSetup = true;
+  P->devRawValid = true;
+  P->hostStale = true;
+  P->hostPattern = false;
+  // numeric phase
+  SB_HIP(P->d_minv.alloc((size_t)nb * 36));
+  SB_HIP(P->d_minv32.alloc((size_t)nb * 36));
+  SB_HIP(P->d_diag.alloc((size_t)nc * 9));
+  SB_HIP(P->d_scal.alloc(nc));
+  SB_HIP(P->d_ones.alloc(nc));
+  SB_HIP(sb::launch_fill_f64((size_t)nc, 1.0, P->d_ones.p, s));
+  SB_HIP(sb::launch_tile_values(3, 6, false, P->H.host.nblocks, P->d_hsrc.p, P->d_hraw.p, P->H.vals.p,
+                                P->H.vals32.p, s));
+  SB_HIP(sb::launch_tile_values(6, 3, true, P->HT.host.nblocks, P->d_htsrc.p, P->d_hraw.p, P->HT.vals.p,
+                                P->HT.vals32.p, s));
+  SB_HIP(sb::launch_minv_layout(nb, P->d_mraw.p, P->d_minv.p, P->d_minv32.p, s));
+  SB_HIP(sb::launch_diag_scal(nc, P->d_hrp.p, P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->d_diag.p, P->d_scal.p, s));
+  if (P->hasW)
+    SB_HIP(sb::launch_w_numeric(P->W.host.nblocks, P->W.brow.p, P->W.bcol.p, P->d_wtptr.p, P->d_wterm.p,
+                                P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->W.vals.p, P->W.vals32.p, s));
+  if ((rc = alloc_workspace(P))) return rc;
+  SB_HIP(hipStreamSynchronize(s));
+  return SB_OK;
+}
 }  // namespace
 
 namespace {
 int create_problem(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, const int32_t* rp, const int32_t* ci,
                    const double* hv, const double* minv, const double* b, const double* mu, int flags);
+int create_problem_device(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, const int32_t* rp,
+                          const int32_t* ci, const double* hv, const double* minv, const double* b, const double* mu,
+                          int flags);
 }  // namespace
 
 extern "C" {
@@ -926,6 +1241,61 @@ int sb_problem_create(sb_ctx* ctx, sb_problem** out, int nc, int nb, const int32
   return SB_OK;
 }
 
+int sb_problem_create_device(sb_ctx* ctx, sb_problem** out, int nc, int nb, const int32_t* rp, const int32_t* ci,
+                             const double* hv, const double* minv, const double* b, const double* mu, int flags) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!out) return SB_ERR_INVALID;
+  std::unique_ptr<sb_problem> P;
+  rc = create_problem_device(ctx, P, nc, nb, rp, ci, hv, minv, b, mu, flags);
+  if (rc) return rc;
+  *out = P.release();
+  return SB_OK;
+}
+
+int sb_problem_layout(sb_ctx* ctx, const sb_problem* Pc, int which, int64_t* sizes, int32_t* tstart, int32_t* brow,
+                      int32_t* bcol, int32_t* tlong, int32_t* long_rows, int32_t* rowptr, int32_t* src,
+                      double* vals) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (!P || !sizes || which < 0 || which > 2 || (which == 2 && !P->hasW)) return SB_ERR_INVALID;
+  const DevTiles& T = which == 0 ? P->H : which == 1 ? P->HT : P->W;
+  const sbhost::Tiles& h = T.host;
+  sizes[0] = h.nrows;
+  sizes[1] = h.nblocks;
+  sizes[2] = h.ntiles;
+  sizes[3] = T.nlong;
+  sizes[4] = h.scan_steps;
+  sizes[5] = T.pairs ? 1 : 0;
+  hipStream_t s = ctx->stream;
+  auto get = [&](int32_t* dst, const int* d, size_t n) -> int {
+    if (dst && n) SB_HIP(hipMemcpyAsync(dst, d, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    return SB_OK;
+  };
+  const size_t nq = (size_t)h.nblocks, nt = (size_t)h.ntiles, nl = (size_t)T.nlong;
+  if ((rc = get(tstart, T.tstart.p, nt + 1)) || (rc = get(brow, T.brow.p, nq)) || (rc = get(bcol, T.bcol.p, nq)) ||
+      (rc = get(tlong, T.tlong.p, nt)) || (rc = get(rowptr, T.rowptr.p, (size_t)h.nrows + 1)))
+    return rc;
+  if (long_rows && nl) {
+    if ((rc = get(long_rows, T.long_row.p, nl)) || (rc = get(long_rows + nl, T.long_t0.p, nl)) ||
+        (rc = get(long_rows + 2 * nl, T.long_t1.p, nl)))
+      return rc;
+  }
+  if (vals && nq) {
+    const int bs = which == 2 ? 9 : 18;
+    SB_HIP(hipMemcpyAsync(vals, T.vals.p, sizeof(double) * nq * bs, hipMemcpyDeviceToHost, s));
+  }
+  SB_HIP(hipStreamSynchronize(s));
+  if (src) {
+    if ((rc = sync_pattern(P))) return rc;
+    const std::vector<int>& v = which == 0 ? P->H.host.src : which == 1 ? P->htsrc : P->W.host.src;
+    if (v.size() != nq) return SB_ERR_INVALID;
+    std::memcpy(src, v.data(), sizeof(int) * nq);
+  }
+  return SB_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -945,19 +1315,68 @@ int create_problem(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, 
   P->ctx = ctx;
   P->nc = nc;
   P->nb = nb;
+  P->hasW = (flags & SB_PROBLEM_ASSEMBLE_W) != 0;
+  if (!(flags & SB_PROBLEM_HOST_SETUP)) {
+    // raw arrays to the device, set-up there; the host copies are made when needed
+    hipStream_t s = ctx->stream;
+    P->nnz = nnz;
+    SB_HIP(P->d_hrp.upload(rp, (size_t)nc + 1, s));
+    SB_HIP(P->d_hci.upload(ci, (size_t)nnz, s));
+    SB_HIP(P->d_hraw.upload(hv, (size_t)nnz * 18, s));
+    SB_HIP(P->d_mraw.upload(minv, (size_t)nb * 36, s));
+    SB_HIP(P->d_b.upload(b, (size_t)nc * 3, s));
+    SB_HIP(P->d_mu.upload(mu, (size_t)nc, s));
+    return build_device(P.get());
+  }
   P->hrp.assign(rp, rp + nc + 1);
   P->hci.assign(ci, ci + nnz);
   P->hv.assign(hv, hv + (size_t)nnz * 18);
   P->minv.assign(minv, minv + (size_t)nb * 36);
   P->b.assign(b, b + 3 * (size_t)nc);
   P->mu.assign(mu, mu + nc);
-  if (flags & SB_PROBLEM_ASSEMBLE_W) {
+  if (P->hasW)
     sbhost::assembleW(nc, nb, P->hrp.data(), P->hci.data(), P->hv.data(), P->minv.data(), P->wrp, P->wci, P->wv);
-    P->hasW = true;
-  }
   rc = upload_problem(P.get());
   if (rc) return rc;
   return SB_OK;
+}
+
+int create_problem_device(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, const int32_t* rp,
+                          const int32_t* ci, const double* hv, const double* minv, const double* b, const double* mu,
+                          int flags) {
+  int rc;
+  if (nc <= 0 || nb <= 0 || !rp || !ci || !hv || !minv || !b || !mu) return SB_ERR_INVALID;
+  hipStream_t s = ctx->stream;
+  DBuf<int> err;
+  SB_HIP(err.alloc(1));
+  SB_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  SB_HIP(sb::launch_check_rows(nc, rp, err.p, s));
+  int bad = 0, nnz = 0;
+  SB_HIP(hipMemcpyAsync(&nnz, rp + nc, sizeof(int), hipMemcpyDeviceToHost, s));
+  if ((rc = read_int(err.p, &bad, s))) return rc;
+  if (bad || nnz < 0) return SB_ERR_INVALID;
+  SB_HIP(sb::launch_check_cols(nnz, nb, ci, err.p, s));
+  if ((rc = read_int(err.p, &bad, s))) return rc;
+  if (bad) return SB_ERR_INVALID;
+  P.reset(new (std::nothrow) sb_problem());
+  if (!P) return SB_ERR_OOM;
+  P->ctx = ctx;
+  P->nc = nc;
+  P->nb = nb;
+  P->nnz = nnz;
+  P->hasW = (flags & SB_PROBLEM_ASSEMBLE_W) != 0;
+  auto dup = [&](auto& dst, const auto* src, size_t n) -> int {
+    SB_HIP(dst.alloc(n));
+    if (n) SB_HIP(hipMemcpyAsync(dst.p, src, sizeof(*src) * n, hipMemcpyDeviceToDevice, s));
+    return SB_OK;
+  };
+  if ((rc = dup(P->d_hrp, rp, (size_t)nc + 1))) return rc;
+  if ((rc = dup(P->d_hci, ci, (size_t)nnz))) return rc;
+  if ((rc = dup(P->d_hraw, hv, (size_t)nnz * 18))) return rc;
+  if ((rc = dup(P->d_mraw, minv, (size_t)nb * 36))) return rc;
+  if ((rc = dup(P->d_b, b, (size_t)nc * 3))) return rc;
+  if ((rc = dup(P->d_mu, mu, (size_t)nc))) return rc;
+  return build_device(P.get());
 }
 
 }  // namespace
@@ -994,8 +1413,8 @@ int sb_problem_sizes(const sb_problem* P, int* nc, int* nb, int64_t* hblocks, in
   if (!P) return SB_ERR_INVALID;
   if (nc) *nc = P->nc;
   if (nb) *nb = P->nb;
-  if (hblocks) *hblocks = (int64_t)P->hci.size();
-  if (wblocks) *wblocks = P->hasW ? (int64_t)P->wci.size() : 0;
+  if (hblocks) *hblocks = (int64_t)P->nnz;
+  if (wblocks) *wblocks = P->hasW ? P->nwl : 0;
   return SB_OK;
 }
 
@@ -1086,9 +1505,12 @@ int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm
   return upload_problem(P);
 }
 
-int sb_problem_coloring(sb_ctx* ctx, const sb_problem* P, int mode, int64_t* perm, int64_t* colors, int* ncolors) {
-  (void)ctx;
-  if (!P || !perm || !colors || !ncolors || (mode != 0 && mode != 1)) return SB_ERR_INVALID;
+int sb_problem_coloring(sb_ctx* ctx, const sb_problem* Pc, int mode, int64_t* perm, int64_t* colors, int* ncolors) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !perm || !colors || !ncolors || (mode != 0 && mode != 1)) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if ((rc = sync_pattern(P))) return rc;
   std::vector<int64_t> pm, cl;
   // the pattern of W is the contact-body incidence of H: the coloring does not need W
   if (P->hasW)
@@ -1364,8 +1786,9 @@ static int gs_prepare(sb_ctx* ctx, sb_problem* P, const sb_gs_options* o, const 
   }
   hipStream_t s = ctx->stream;
   int rc;
-  if ((rc = sync_host(P))) return rc;
-  std::vector<double> local(P->diag), reg(n, 0.0);
+  std::vector<double> local((size_t)9 * n), reg(n, 0.0);  // the diagonal blocks (current on the device)
+  SB_HIP(hipMemcpyAsync(local.data(), P->d_diag.p, sizeof(double) * local.size(), hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
   if (o->auto_regularization > 0.0)
     for (int i = 0; i < n; ++i) {
       reg[i] = std::max(0.0, o->auto_regularization - min_eig_sym3(&local[(size_t)9 * i]));
@@ -1460,6 +1883,7 @@ int sb_gs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, const
   if (!Pc || !o || !r) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (!P->hasW) return SB_ERR_INVALID;
+  if ((rc = sync_pattern(P))) return rc;
   // Contacts of one color are solved in parallel from each other's previous values, so
   // a color must hold no coupled pair (W_ij != 0). Given colors are checked; without
   // colors the natural order is cut into maximal runs of uncoupled contacts, which
@@ -1548,6 +1972,7 @@ int sb_pgs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, cons
   if (rc) return rc;
   if (!Pc || !o || !r) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
+  if ((rc = sync_pattern(P))) return rc;
   // Contacts of one color update the body vector in parallel, so they must share no
   // body. Given colors are checked; without colors the natural order is cut into
   // maximal runs of contacts that share no body, which sweeps the contacts exactly
@@ -1994,7 +2419,7 @@ int sb_problem_update_values(sb_ctx* ctx, sb_problem* P, const double* hvals, co
   if ((hvals || minv) && P->hasW && !P->wFromH) return SB_ERR_INVALID;  // W was supplied, not H Minv H^T
   hipStream_t s = ctx->stream;
   const int nc = P->nc, nb = P->nb;
-  const size_t nh = P->hv.size();
+  const size_t nh = (size_t)P->nnz * 18;
   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   if ((rc = setup_maps(P))) return rc;
   if (!P->devRawValid) {  // current values from the host copies

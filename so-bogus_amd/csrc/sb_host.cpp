@@ -227,27 +227,32 @@ void assembleW(int nc, int nb, const int* hrowptr, const int* hcolidx, const dou
 
 void diagBlocks(int nc, const int* hrowptr, const int* hcolidx, const double* hvals, const double* minv,
                 std::vector<double>& diag) {
+  // W_ii summed as the assembly sums it: pairs (a, b) of contact i's blocks at the
+  // same body, a outer, b inner; T = Minv_k H_b^T first, then H_a T
   diag.assign((size_t)nc * 9, 0.0);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nc; ++i) {
     double* d = &diag[(size_t)i * 9];
-    for (int p = hrowptr[i]; p < hrowptr[i + 1]; ++p) {
-      const double* h = hvals + (size_t)p * 18;
-      const double* m = minv + (size_t)hcolidx[p] * 36;
-      double t[6][3];
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 3; ++c) {
-          double acc = 0;
-          for (int l = 0; l < 6; ++l) acc += m[r * 6 + l] * h[c * 6 + l];
-          t[r][c] = acc;
-        }
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-          double acc = 0;
-          for (int l = 0; l < 6; ++l) acc += h[r * 6 + l] * t[l][c];
-          d[r * 3 + c] += acc;
-        }
-    }
+    for (int a = hrowptr[i]; a < hrowptr[i + 1]; ++a)
+      for (int b = hrowptr[i]; b < hrowptr[i + 1]; ++b) {
+        if (hcolidx[a] != hcolidx[b]) continue;
+        const double* ha = hvals + (size_t)a * 18;
+        const double* hb = hvals + (size_t)b * 18;
+        const double* m = minv + (size_t)hcolidx[a] * 36;
+        double t[6][3];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double acc = 0;
+            for (int l = 0; l < 6; ++l) acc += m[r * 6 + l] * hb[c * 6 + l];
+            t[r][c] = acc;
+          }
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double acc = 0;
+            for (int l = 0; l < 6; ++l) acc += ha[r * 6 + l] * t[l][c];
+            d[r * 3 + c] += acc;
+          }
+      }
   }
 }
 

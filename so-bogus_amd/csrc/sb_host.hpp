@@ -35,7 +35,8 @@ void expandSymmetric(int n, int bs, const int* rowptr, const int* colidx, const 
 void assembleW(int nc, int nb, const int* hrowptr, const int* hcolidx, const double* hvals, const double* minv,
                std::vector<int>& wrowptr, std::vector<int>& wcolidx, std::vector<double>& wvals);
 
-// Diagonal blocks W_ii = sum_k H_ik Minv_k H_ik^T.
+// Diagonal blocks W_ii: the sum over pairs of contact i's blocks at the same body,
+// as the assembled W sums them.
 void diagBlocks(int nc, const int* hrowptr, const int* hcolidx, const double* hvals, const double* minv,
                 std::vector<double>& diag);
 

@@ -56,6 +56,8 @@ __global__ void __launch_bounds__(256) k_minv_layout(int nb, const double* __res
 // 3x3 block sum_terms H_pi Minv_k H_pj^T, k the body both blocks share.
 __device__ __forceinline__ void w_block(const double* __restrict__ hraw, const double* __restrict__ mraw, int pi,
                                         int pj, int k, double acc[9]) {
+  // No contraction into fused multiply-adds: the same rounding as the host assembly.
+#pragma clang fp contract(off)
   const double* hi = hraw + (size_t)18 * pi;
   const double* hj = hraw + (size_t)18 * pj;
   const double* m = mraw + (size_t)36 * k;
@@ -81,14 +83,18 @@ __device__ __forceinline__ void w_block(const double* __restrict__ hraw, const d
     }
 }
 
-// Diagonal blocks W_ii = sum_k H_ik Minv_k H_ik^T (row-major [nc][9]) and the row
-// scalings s_i = max(1, tr(W_ii) / 3) (ConstrainedSolverBase.impl.hpp:78-118).
+// Diagonal blocks W_ii = sum over pairs (a, b) of contact i's H blocks at the same
+// body k of H_a Minv_k H_b^T (row-major [nc][9]; the diagonal blocks of the
+// assembled W, summed in its order) and the row scalings s_i = max(1, tr(W_ii) / 3)
+// (ConstrainedSolverBase.impl.hpp:78-118).
 __global__ void __launch_bounds__(256) k_diag_scal(int nc, const int* __restrict__ hrp, const int* __restrict__ hci,
                                                    const double* __restrict__ hraw, const double* __restrict__ mraw,
                                                    double* __restrict__ diag, double* __restrict__ scal) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int p = hrp[i]; p < hrp[i + 1]; ++p) w_block(hraw, mraw, p, p, hci[p], acc);
+    for (int a = hrp[i]; a < hrp[i + 1]; ++a)
+      for (int b = hrp[i]; b < hrp[i + 1]; ++b)
+        if (hci[a] == hci[b]) w_block(hraw, mraw, a, b, hci[a], acc);
 #pragma unroll
     for (int e = 0; e < 9; ++e) diag[(size_t)9 * i + e] = acc[e];
     scal[i] = fmax(1.0, (acc[0] + acc[4] + acc[8]) / 3.0);

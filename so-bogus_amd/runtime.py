@@ -35,13 +35,14 @@ BATCH_CALLER_ORDER = 2
 BATCH_SPREAD_TEAMS = 4
 BSR_SYMMETRIC = 1
 PROBLEM_ASSEMBLE_W = 1
+PROBLEM_HOST_SETUP = 8
 BENCH_FLUSH_CACHES = 16
 
 # Every extern "C" symbol of include/sobogus_hip.h
 EXPORTED = [
     "sb_status_string", "sb_pg_options_default", "sb_gs_options_default", "sb_ctx_create", "sb_ctx_destroy",
     "sb_ctx_info", "sb_ctx_launch_counts", "sb_bsr_create", "sb_bsr_destroy", "sb_bsr_multiply", "sb_bsr_split_row_multiply",
-    "sb_problem_create", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
+    "sb_problem_create", "sb_problem_create_device", "sb_problem_layout", "sb_problem_set_w", "sb_problem_destroy", "sb_problem_sizes", "sb_problem_get_w",
     "sb_problem_apply_permutation", "sb_problem_coloring", "sb_problem_apply_w", "sb_fb_residual", "sb_pg_solve",
     "sb_pg_solve_noalloc",
     "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_problem_update_values",
@@ -139,6 +140,9 @@ def lib():
         "sb_bsr_destroy": (C.c_int, [_vp]),
         "sb_bsr_multiply": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int]),
         "sb_bsr_split_row_multiply": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp]),
+        "sb_problem_create_device": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               C.c_int]),
+        "sb_problem_layout": (C.c_int, [_vp, _vp, C.c_int, _lp, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp]),
         "sb_problem_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
                                         C.c_int]),
         "sb_problem_set_w": (C.c_int, [_vp, _vp, _ip, _ip, _dp]),
@@ -268,27 +272,29 @@ class Context:
 
 
 class DeviceArray:
-    """A float64 vector resident in HBM (for solves without host round trips)."""
+    """A vector resident in HBM (float64 by default; int32 for index arrays), for
+    solves and set-ups without host round trips."""
 
-    def __init__(self, ctx: Context, n: int):
-        self.ctx, self.n = ctx, int(n)
+    def __init__(self, ctx: Context, n: int, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.dtype.itemsize * self.n
         p = _vp()
-        _check(lib().sb_device_alloc(ctx.h, C.byref(p), 8 * max(self.n, 1)), "sb_device_alloc")
+        _check(lib().sb_device_alloc(ctx.h, C.byref(p), max(self.nbytes, 8)), "sb_device_alloc")
         self.ptr = p
 
     def upload(self, a):
-        a = _f64(a)
+        a = np.ascontiguousarray(a, dtype=self.dtype).reshape(-1)
         assert a.size == self.n
-        _check(lib().sb_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), 8 * self.n), "h2d")
+        _check(lib().sb_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), self.nbytes), "h2d")
         return self
 
     def zero(self):
-        _check(lib().sb_memset_device(self.ctx.h, self.ptr, 0, 8 * self.n), "sb_memset_device")
+        _check(lib().sb_memset_device(self.ctx.h, self.ptr, 0, self.nbytes), "sb_memset_device")
         return self
 
     def download(self) -> np.ndarray:
-        out = np.empty(self.n)
-        _check(lib().sb_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, 8 * self.n), "d2h")
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(lib().sb_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes), "d2h")
         return out
 
     def free(self):
@@ -377,15 +383,49 @@ class SolveResult:
 class DualFrictionProblem:
     """u = W r + b with W = H Minv H^T on the device (sb_problem)."""
 
-    def __init__(self, ctx: Context, prob, assemble: bool = True):
+    def __init__(self, ctx: Context, prob, assemble: bool = True, host_setup: bool = False):
         self.ctx, self.nc, self.nb = ctx, prob.nc, prob.nb
         rp, ci, hv, mi, b, mu = (_i32(prob.rowptr), _i32(prob.colidx), _f64(prob.hvals).reshape(-1),
                                  _f64(prob.minv).reshape(-1), _f64(prob.b), _f64(prob.mu))
         h = _vp()
+        flags = (PROBLEM_ASSEMBLE_W if assemble else 0) | (PROBLEM_HOST_SETUP if host_setup else 0)
         _check(lib().sb_problem_create(ctx.h, C.byref(h), prob.nc, prob.nb, _i(rp), _i(ci), _d(hv), _d(mi), _d(b),
-                                       _d(mu), PROBLEM_ASSEMBLE_W if assemble else 0), "sb_problem_create")
+                                       _d(mu), flags), "sb_problem_create")
         self.h = h
         self.assembled = assemble
+
+    @classmethod
+    def from_device(cls, ctx: Context, nc: int, nb: int, rowptr: "DeviceArray", colidx: "DeviceArray",
+                    hvals: "DeviceArray", minv: "DeviceArray", b: "DeviceArray", mu: "DeviceArray",
+                    assemble: bool = True) -> "DualFrictionProblem":
+        """A problem from arrays already in HBM (sb_problem_create_device): int32 row
+        pointer and columns, float64 values."""
+        self = cls.__new__(cls)
+        self.ctx, self.nc, self.nb = ctx, int(nc), int(nb)
+        h = _vp()
+        _check(lib().sb_problem_create_device(ctx.h, C.byref(h), self.nc, self.nb, rowptr.ptr, colidx.ptr, hvals.ptr,
+                                              minv.ptr, b.ptr, mu.ptr, PROBLEM_ASSEMBLE_W if assemble else 0),
+               "sb_problem_create_device")
+        self.h = h
+        self.assembled = assemble
+        return self
+
+    def layout(self, which: int) -> dict:
+        """The tile layout of H (0), H^T (1) or W (2) as the solvers use it."""
+        sz = np.zeros(6, np.int64)
+        _check(lib().sb_problem_layout(self.ctx.h, self.h, which, sz.ctypes.data_as(_lp), None, None, None, None,
+                                       None, None, None, None), "sb_problem_layout")
+        nrows, nq, nt, nl, steps, pairs = (int(v) for v in sz)
+        bs = 9 if which == 2 else 18
+        out = dict(nrows=nrows, nblocks=nq, ntiles=nt, nlong=nl, scan_steps=steps, pairs=pairs,
+                   tstart=np.zeros(nt + 1, np.int32), brow=np.zeros(nq, np.int32), bcol=np.zeros(nq, np.int32),
+                   tlong=np.zeros(nt, np.int32), long_rows=np.zeros(max(3 * nl, 1), np.int32),
+                   rowptr=np.zeros(nrows + 1, np.int32), src=np.zeros(nq, np.int32), vals=np.zeros(bs * nq))
+        _check(lib().sb_problem_layout(self.ctx.h, self.h, which, sz.ctypes.data_as(_lp), _i(out["tstart"]),
+                                       _i(out["brow"]), _i(out["bcol"]), _i(out["tlong"]), _i(out["long_rows"]),
+                                       _i(out["rowptr"]), _i(out["src"]), _d(out["vals"])), "sb_problem_layout")
+        out["long_rows"] = out["long_rows"][:3 * nl]
+        return out
 
     def __del__(self):
         try:

@@ -124,3 +124,134 @@ def test_mass_inverse(R, ctx):
     assert np.array_equal(R.mass_inverse(ctx, Mz.reshape(1, 36)).reshape(6, 6), Mz.T / 4.0)
     with pytest.raises(RuntimeError):
         R.mass_inverse(ctx, np.zeros((1, 36)))
+
+
+def _ragged_problem(G, seed=3):
+    """Contacts with no body, one body, three bodies, the same body twice, and one
+    contact with 100 blocks (a long row of H); the last 100 bodies touch nothing
+    (empty rows of H^T)."""
+    base = G.generate(nc=3000, nb=1500, seed=seed)
+    rng = np.random.default_rng(seed)
+    nbu = base.nb - 100
+    rows = []
+    for i in range(base.nc):
+        k = i % 7
+        if k == 0:
+            cols = []
+        elif k == 1:
+            cols = [int(rng.integers(nbu))]
+        elif k == 2:
+            cols = sorted(rng.choice(nbu, size=3, replace=False).tolist())
+        elif k == 3:
+            a = int(rng.integers(nbu))
+            cols = [a, a]
+        else:
+            cols = sorted(rng.choice(nbu, size=2, replace=False).tolist())
+        rows.append(cols)
+    rows[5] = list(range(0, 200, 2))
+    rowptr = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    colidx = np.concatenate([np.asarray(c, np.int32) for c in rows]).astype(np.int32)
+    hvals = rng.standard_normal((len(colidx), 18))
+    return dataclasses.replace(base, rowptr=rowptr, colidx=colidx, hvals=hvals)
+
+
+def _same_layout(a, b, which):
+    la, lb = a.layout(which), b.layout(which)
+    for k in la:
+        assert np.array_equal(la[k], lb[k]), (which, k)
+    return la
+
+
+def _set_up_cases(G):
+    return {
+        "uniform-5k": (G.generate(nc=5000, nb=2000, seed=21), True),
+        "zipf-20k": (G.generate(nc=20000, nb=3000, seed=3, zipf=1.2, mu_range=(0.1, 1.0)), True),
+        "ragged": (_ragged_problem(G), True),
+    }
+
+
+@pytest.mark.parametrize("case", ["uniform-5k", "zipf-20k", "ragged"])
+def test_device_setup_equals_host_setup(R, G, ctx, case):
+    """The set-up on the device (default) builds every array the host set-up builds:
+    tiles of H, H^T and the two-triangle W (indices, long rows, source blocks and
+    values, bit for bit), W's lower triangle, the diagonal blocks and scalings (so
+    every product and residual is bit-identical), and the colorings."""
+    print(f"case: device set-up vs host set-up, {case}", flush=True)
+    p, assemble = _set_up_cases(G)[case]
+    host = R.DualFrictionProblem(ctx, p, assemble=assemble, host_setup=True)
+    dev = R.DualFrictionProblem(ctx, p, assemble=assemble)
+    lh = _same_layout(host, dev, 0)
+    lt = _same_layout(host, dev, 1)
+    if assemble:
+        _same_layout(host, dev, 2)
+    if case == "ragged":
+        assert lh["nlong"] == 1 and (lh["src"] < 0).sum() == (np.diff(p.rowptr) == 0).sum()
+        untouched = (np.bincount(p.colidx, minlength=p.nb) == 0).sum()
+        assert untouched >= 100 and (lt["src"] < 0).sum() == untouched
+    if case == "zipf-20k":
+        assert lt["nlong"] > 0
+    assert host.sizes() == dev.sizes()
+    x = np.random.default_rng(4).standard_normal(3 * p.nc)
+    for op in ((R.OP_ASSEMBLED, R.OP_MATRIX_FREE) if assemble else (R.OP_MATRIX_FREE,)):
+        assert np.array_equal(host.apply_w(x, op=op), dev.apply_w(x, op=op)), op
+        for sc in (R.SCALING_REFERENCE, R.SCALING_DIAGONAL):
+            assert host.eval_with(x, law=R.LAW_COULOMB, op=op, scaling=sc) == \
+                dev.eval_with(x, law=R.LAW_COULOMB, op=op, scaling=sc)
+    if assemble:
+        wh, wd = host.w_lower(), dev.w_lower()
+        for a, b in zip(wh, wd):
+            assert np.array_equal(a, b)
+    for mode in (R.COLORING_REFERENCE, R.COLORING_GREEDY):
+        ph, ch = host.coloring(mode)
+        pd, cd = dev.coloring(mode)
+        assert np.array_equal(ph, pd) and np.array_equal(ch, cd)
+
+
+def test_device_setup_from_device_arrays(R, G, ctx):
+    """sb_problem_create_device: the arrays already in HBM; same layout and products as
+    the host set-up, and invalid patterns are refused."""
+    print("case: set-up from device arrays (uniform-5k, ragged)", flush=True)
+    for p in (G.generate(nc=5000, nb=2000, seed=22), _ragged_problem(G, seed=4)):
+        d = {k: R.DeviceArray(ctx, np.asarray(v).size, np.int32 if k in ("rowptr", "colidx") else np.float64)
+             .upload(np.asarray(v).reshape(-1))
+             for k, v in (("rowptr", p.rowptr), ("colidx", p.colidx), ("hvals", p.hvals), ("minv", p.minv),
+                          ("b", p.b), ("mu", p.mu))}
+        dev = R.DualFrictionProblem.from_device(ctx, p.nc, p.nb, d["rowptr"], d["colidx"], d["hvals"], d["minv"],
+                                                d["b"], d["mu"], assemble=True)
+        host = R.DualFrictionProblem(ctx, p, assemble=True, host_setup=True)
+        for which in (0, 1, 2):
+            _same_layout(host, dev, which)
+        x = np.random.default_rng(8).standard_normal(3 * p.nc)
+        assert np.array_equal(host.apply_w(x, op=R.OP_MATRIX_FREE), dev.apply_w(x, op=R.OP_MATRIX_FREE))
+    p = G.generate(nc=100, nb=50, seed=23)
+    bad_cols = np.array(p.colidx, np.int32)
+    bad_cols[7] = p.nb  # a body index out of range
+    bad_rows = np.array(p.rowptr, np.int32)
+    bad_rows[10] = bad_rows[12]  # a decreasing row pointer
+    for rp, ci in ((p.rowptr, bad_cols), (bad_rows, p.colidx)):
+        arrs = [R.DeviceArray(ctx, np.asarray(v).size, t).upload(np.asarray(v).reshape(-1))
+                for v, t in ((rp, np.int32), (ci, np.int32), (p.hvals, np.float64), (p.minv, np.float64),
+                             (p.b, np.float64), (p.mu, np.float64))]
+        with pytest.raises(RuntimeError):
+            R.DualFrictionProblem.from_device(ctx, p.nc, p.nb, *arrs)
+
+
+def test_device_setup_zipf_1m(R, G, ctx):
+    """The 1M-contact Zipf problem (matrix-free): the device set-up's layouts equal the
+    host's; its creation time is printed beside the host's."""
+    import time
+    print("case: device set-up vs host set-up, zipf-1m", flush=True)
+    p = G.workload("zipf-1m")
+    t0 = time.perf_counter()
+    host = R.DualFrictionProblem(ctx, p, assemble=False, host_setup=True)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    dev = R.DualFrictionProblem(ctx, p, assemble=False)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    print(f"   create: host set-up {1e3 * (t1 - t0):.1f} ms, device set-up {1e3 * (t2 - t1):.1f} ms", flush=True)
+    _same_layout(host, dev, 0)
+    lt = _same_layout(host, dev, 1)
+    assert lt["nlong"] > 0
+    x = np.random.default_rng(9).standard_normal(3 * p.nc)
+    assert np.array_equal(host.apply_w(x, op=R.OP_MATRIX_FREE), dev.apply_w(x, op=R.OP_MATRIX_FREE))
