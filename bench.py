@@ -218,15 +218,19 @@ def tight_50k(ctx, R, G, cpu_threads: int) -> dict:
     # problem (the oracle's too), so the tight tolerance is measured on the SOC problem
     perm, colors = dp.coloring(R.COLORING_GREEDY)
     dp.apply_permutation(perm)
-    for name, fn in (("gs_assembled_w", dp.solve_gs), ("gs_matrix_free", dp.solve_pgs)):
+    for name, fn, prec in (("gs_assembled_w", dp.solve_gs, R.PRECISION_FP64),
+                           ("gs_assembled_w_mixed", dp.solve_gs, R.PRECISION_MIXED),
+                           ("gs_matrix_free", dp.solve_pgs, R.PRECISION_FP64)):
         ts = []
         for _ in range(3):
             r.zero()
-            s = fn(r, law=R.LAW_SOC, tol=1e-8, max_iters=3000, colors=colors, on_device=True)
+            s = fn(r, law=R.LAW_SOC, tol=1e-8, max_iters=3000, colors=colors, on_device=True, precision=prec)
             ts.append(s.device_ms)
         out[name] = {"ms": float(np.median(ts[1:])), "sweeps": s.iterations, "residual": s.residual,
                      "converged": bool(s.converged), "colors": len(colors) - 1,
                      "us_per_sweep": 1e3 * float(np.median(ts[1:])) / max(s.iterations, 1)}
+        if prec == R.PRECISION_MIXED:
+            out[name].update(fp32_sweeps=s.iterations - s.refine_iterations, fp32_residual=s.mixed_residual)
     log("contacts-50k to 1e-8: " + ", ".join(f"{k} {v['ms']:.1f} ms" for k, v in out.items()))
     if cpu_threads > 0:
         sys.path.insert(0, str(ROOT / "oracle"))

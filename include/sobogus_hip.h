@@ -137,6 +137,13 @@ typedef struct sb_gs_options {
   double auto_regularization; /* 0 */
   int grid_blocks;        /* 0 = automatic */
   int block_threads;      /* 0 = automatic (256, the only size of the GS kernel) */
+  int precision;          /* sb_precision, FP64 by default. Mixed (sb_gs_solve only): sweeps whose
+                             off-diagonal gathers W_ij x_j read FP32 copies of W and x (local solve and
+                             FB evaluation in FP64) until tol * mixed_tol_factor or a stall of their
+                             residual, then FP64 sweeps from the best iterate (at least one, the final
+                             refinement sweep, before the first FP64 evaluation) until tol; the returned
+                             residual is an FP64 evaluation */
+  double mixed_tol_factor;/* 0.95 */
 } sb_gs_options;
 
 typedef struct sb_stats {

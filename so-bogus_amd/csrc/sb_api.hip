@@ -1059,6 +1059,8 @@ void sb_gs_options_default(sb_gs_options* o) {
   o->auto_regularization = 0.0;
   o->grid_blocks = 0;
   o->block_threads = 0;
+  o->precision = SB_PRECISION_FP64;
+  o->mixed_tol_factor = 0.95;
 }
 
 int sb_ctx_create(sb_ctx** out, int device) {
@@ -1771,6 +1773,9 @@ int sb_pg_solve_noalloc(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* 
 static int gs_prepare(sb_ctx* ctx, sb_problem* P, const sb_gs_options* o, const int64_t* colors, int ncolors,
                       double* r, int on_device, sb_stats* st, sb::GSArgs& a, int& histCap, int& maxColor) {
   if (o->max_iters < 0 || o->eval_every < 1 || o->strategy < 0 || o->strategy > 3) return SB_ERR_INVALID;
+  if (o->precision != SB_PRECISION_FP64 && o->precision != SB_PRECISION_MIXED) return SB_ERR_INVALID;
+  if (o->precision == SB_PRECISION_MIXED && !(o->mixed_tol_factor > 0.0 && o->mixed_tol_factor <= 1.0))
+    return SB_ERR_INVALID;
   if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   const int n = P->nc;
   std::vector<int> col;
@@ -1829,6 +1834,11 @@ static int gs_prepare(sb_ctx* ctx, sb_problem* P, const sb_gs_options* o, const 
   a.hist = P->hist.p;
   a.hist_cap = histCap;
   a.out = ctx->out.p;
+  a.mixed = o->precision == SB_PRECISION_MIXED ? 1 : 0;
+  a.stop_factor = o->mixed_tol_factor;
+  a.W32 = P->W.view32();
+  a.x32 = P->X32[0].p;
+  a.carry32 = P->carry32.p;
   maxColor = 0;
   for (size_t c = 0; c + 1 < col.size(); ++c) maxColor = std::max(maxColor, col[c + 1] - col[c]);
   return SB_OK;
@@ -1843,9 +1853,10 @@ static int gs_run(sb_ctx* ctx, sb_problem* P, const void* fn, int grid, int thre
   SB_HIP(hipEventRecord(ctx->ev0, s));
   SB_HIP(launch_persistent(ctx, fn, grid, threads, args, s));
   SB_HIP(hipEventRecord(ctx->ev1, s));
-  double out[8];
+  double out[9];
   int rc;
-  if ((rc = read_status(ctx, out, 8))) return rc;
+  if ((rc = read_status(ctx, out, 9))) return rc;
+  const bool mixed = static_cast<sb::GSArgs*>(args[0])->mixed != 0;
   if ((rc = copy_out(ctx, r, P->rio.p, 3 * (size_t)P->nc, on_device))) return rc;
   if (st) {
     float ms = 0.f;
@@ -1859,9 +1870,9 @@ static int gs_run(sb_ctx* ctx, sb_problem* P, const void* fn, int grid, int thre
     st->residual = out[0];
     st->device_ms = ms;
     st->history_length = (int)out[4];
-    st->refine_iterations = 0;
-    st->refine_spmvs = 0;
-    st->mixed_residual = 0.0;
+    st->refine_iterations = mixed ? (int)(out[1] - out[6]) : 0;
+    st->refine_spmvs = mixed ? (int)out[8] : 0;
+    st->mixed_residual = mixed ? out[7] : 0.0;
     st->plain_launch = ctx->lastPlain ? 1 : 0;
     st->mixed_device_ms = 0.0;
     if (histCap > 0) {
@@ -2009,6 +2020,7 @@ int sb_pgs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, cons
   }
   sb::GSArgs a;
   int histCap = 0, maxColor = 0;
+  if (o->precision != SB_PRECISION_FP64) return SB_ERR_UNSUPPORTED;  // mixed precision: sb_gs_solve only
   if ((rc = gs_prepare(ctx, P, o, colors, ncolors, r, on_device, st, a, histCap, maxColor))) return rc;
   if ((rc = pgs_upload(P))) return rc;
   sb::PGSArgs e;

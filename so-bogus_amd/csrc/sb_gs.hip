@@ -35,7 +35,52 @@ __device__ __forceinline__ void cld3(const double* p, int i, double v[3]) {
   v[1] = mld(p + 3 * i + 1);
   v[2] = mld(p + 3 * i + 2);
 }
+// acc += sum over blocks q in [q0, q1) of row i, j = bcol[q] != i, of W_q x_j, block
+// by block in row order. The columns of four blocks, then their values and x_j, are
+// loaded before their products are formed: a row costs about one dependent memory
+// round trip per four blocks rather than one per block. x is updated during the
+// sweep by other blocks of the grid, after a barrier: plain loads (mld).
+template <typename T>
+__device__ __forceinline__ void row_gather(const TileMat<T>& W, size_t nnz, int q0, int q1, int i, const T* x,
+                                           T acc[3]) {
+  for (int qb = q0; qb < q1; qb += 4) {
+    int js[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) js[k] = qb + k < q1 ? W.bcol[qb + k] : i;
+    T xv[4][3], wv[4][9];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (js[k] == i) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xv[k][c] = mld(x + 3 * js[k] + c);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) wv[k][e] = W.vals[(size_t)e * nnz + qb + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (js[k] == i) continue;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        acc[r] += wv[k][3 * r + 0] * xv[k][0] + wv[k][3 * r + 1] * xv[k][1] + wv[k][3 * r + 2] * xv[k][2];
+    }
+  }
+}
 }  // namespace
+
+// One launch runs one or two stages of sweeps. FP64: the reference's loop. Mixed
+// precision: a first stage whose off-diagonal gathers (W_ij x_j) and evaluation
+// product read FP32 copies of W and x (the local right-hand side and the local
+// solve stay FP64, as does the FB evaluation of the FP32 product) runs to
+// tol * stop_factor, or until an evaluation improves on the stage's best by less
+// than 1%; it hands its best iterate to an FP64 stage (skip state and step
+// reference reset), whose first FP64 evaluation follows one sweep (the final
+// refinement sweep), then every eval_every sweeps, within max(eval_every,
+// max_iters - FP32 sweeps) sweeps. The result is the best FP64-evaluated iterate of
+// that stage (the handed-over one included).
+template <bool F32>
+struct StageTag {
+  static constexpr bool value = F32;
+};
 
 __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict__ wrowptr) {
   const int wpb = blockDim.x / SB_WAVE;
@@ -47,153 +92,217 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
   const bool coulomb = a.coulomb != 0;
   const unsigned resMax = a.inf_norm ? 1u : 0u;
   const TileMat<double>& W = a.P.W;
+  const TileMat<float>& W32 = a.W32;
   const size_t nnz = (size_t)W.nblocks;
   Barrier bar;
   bar.gs = a.gs;
   bar.rank = blockIdx.x;
-
-  // y = W x + b ; err = eval(y, x) [and err0 = eval(b, 0) when asked]
-  auto eval_phase = [&](bool withZero, double& err, double& err0) -> bool {
-    double part[2] = {0.0, 0.0};
-    auto ld = [&](int j, double xc[3]) { cld3(a.x, j, xc); };
-    NoPre nopre;
-    auto lead = [&](int i, double v[3], NoPre::type) {
-      double b[3], xi[3];
-      ld3(a.P.b, i, b);
-      cld3(a.x, i, xi);
-      const double s = a.P.scal[i], mu = a.P.mu[i];
-      const double y[3] = {v[0] + b[0], v[1] + b[1], v[2] + b[2]};
-      const double xs[3] = {s * xi[0], s * xi[1], s * xi[2]};
-      const double r = law_eval<double>(coulomb, mu, xs, y);
-      part[0] = resMax ? fmax(part[0], r) : part[0] + r;
-      if (withZero) {
-        const double z[3] = {0.0, 0.0, 0.0};
-        const double r0 = law_eval<double>(coulomb, mu, z, b);
-        part[1] = resMax ? fmax(part[1], r0) : part[1] + r0;
-      }
-    };
-    tile_loop<3, 3, double, double>(W, gw, nwaves, ld, nopre, lead, a.carry);
-    if (W.nlong > 0) {
-      if (!bar.sync()) return false;
-      for (int k = gw; k < W.nlong; k += nwaves) {
-        double v[3];
-        long_row_sum<3, double, double>(W, k, a.carry, v);
-        if (lane_id() == 0) lead(W.long_row[k], v, NoPre::type{});
-      }
-    }
-    if (!bar.reduce<2>(part, resMax | (resMax << 1))) return false;
-    err = bar.red[0];
-    err0 = bar.red[1];
-    if (!a.inf_norm) {
-      err /= (1.0 + n);
-      err0 /= (1.0 + n);
-    }
-    return true;
-  };
-
-  double errBest = 1.7976931348623157e308;
-  double err = 0.0, err0 = 0.0;
-  bool ok = eval_phase(a.try_zero != 0, err, err0);
-  if (ok) {
-    if (err < errBest) {
-      errBest = err;
-      for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
-    }
-    if (a.try_zero && err0 < errBest) {
-      errBest = err0;
-      for (int k = tid; k < n3; k += nthreads) {
-        cst(&a.xbest[k], 0.0);
-        cst(&a.x[k], 0.0);
-      }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.hist_cap > 0) a.hist[0] = errBest;
-    // x/xbest writes above are consumed by other blocks in the color phases
-    ok = bar.sync();
-  }
-
   const double absSkipTol = fmin(a.skip_tol, a.tol);
-  double ndxRef = 0.0;
-  int it = 1, nfail = 0, nEval = 1;
-  bool converged = false;
-  for (; ok && it <= a.max_iters; ++it) {
-    for (int col = 0; col < a.ncolors && ok; ++col) {
-      const int c0 = a.colors[col], c1 = a.colors[col + 1];
-      double part[1] = {0.0};
-      for (int i = c0 + tid; i < c1; i += nthreads) {
-        if (a.skip[i]) {
-          a.skip[i] = (unsigned char)(a.skip[i] - 1);
-          continue;
+  int nfail = 0, nEval = 0;
+
+  // Sweeps until an evaluation is below stopTol or maxIt sweeps; errBest / x_best of
+  // the stage; returns false on a barrier failure.
+  auto stage = [&](auto tag, bool tryZero, int maxIt, double stopTol, int firstEval, int& itDone, double& errBest,
+                   double& errLast, bool& converged) -> bool {
+    constexpr bool kF32 = decltype(tag)::value;
+    // y = W x + b ; err = eval(y, x) [and err0 = eval(b, 0) when asked]
+    auto eval_phase = [&](bool withZero, double& err, double& err0) -> bool {
+      double part[2] = {0.0, 0.0};
+      NoPre nopre;
+      auto lead = [&](int i, auto* v, NoPre::type) {
+        double b[3], xi[3];
+        ld3(a.P.b, i, b);
+        cld3(a.x, i, xi);
+        const double s = a.P.scal[i], mu = a.P.mu[i];
+        const double y[3] = {(double)v[0] + b[0], (double)v[1] + b[1], (double)v[2] + b[2]};
+        const double xs[3] = {s * xi[0], s * xi[1], s * xi[2]};
+        const double r = law_eval<double>(coulomb, mu, xs, y);
+        part[0] = resMax ? fmax(part[0], r) : part[0] + r;
+        if (withZero) {
+          const double z[3] = {0.0, 0.0, 0.0};
+          const double r0 = law_eval<double>(coulomb, mu, z, b);
+          part[1] = resMax ? fmax(part[1], r0) : part[1] + r0;
         }
-        double lx[3], lb[3], x0[3];
-        cld3(a.x, i, lx);
-        x0[0] = lx[0];
-        x0[1] = lx[1];
-        x0[2] = lx[2];
-        const double reg = a.reg[i];
-        ld3(a.P.b, i, lb);
-        lb[0] -= reg * lx[0];
-        lb[1] -= reg * lx[1];
-        lb[2] -= reg * lx[2];
-        for (int q = wrowptr[i]; q < wrowptr[i + 1]; ++q) {
-          const int j = W.bcol[q];
-          if (j == i) continue;
-          double xj[3];
-          cld3(a.x, j, xj);
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-            lb[r] += W.vals[(size_t)(3 * r + 0) * nnz + q] * xj[0] + W.vals[(size_t)(3 * r + 1) * nnz + q] * xj[1] +
-                     W.vals[(size_t)(3 * r + 2) * nnz + q] * xj[2];
+      };
+      if constexpr (kF32) {
+        auto ld = [&](int j, float xc[3]) {
+          xc[0] = mld(a.x32 + 3 * j);
+          xc[1] = mld(a.x32 + 3 * j + 1);
+          xc[2] = mld(a.x32 + 3 * j + 2);
+        };
+        tile_loop<3, 3, float, float>(W32, gw, nwaves, ld, nopre, lead, a.carry32);
+        if (W32.nlong > 0) {
+          if (!bar.sync()) return false;
+          for (int k = gw; k < W32.nlong; k += nwaves) {
+            float v[3];
+            long_row_sum<3, float, float>(W32, k, a.carry32, v);
+            if (lane_id() == 0) lead(W32.long_row[k], v, NoPre::type{});
+          }
         }
-        double A[3][3];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) A[e / 3][e % 3] = a.local[(size_t)9 * i + e];
-        const double ldx0[3] = {-lx[0], -lx[1], -lx[2]};
-        const double s = a.P.scal[i];
-        const double lres = local::solve(coulomb, a.strategy, A, lb, lx, a.P.mu[i], a.local_tol, s);
-        const bool lok = a.local_tol > lres;
-        double ldx[3] = {ldx0[0] + lx[0], ldx0[1] + lx[1], ldx0[2] + lx[2]};
-        if (!lok) {
-          ldx[0] *= 0.5;
-          ldx[1] *= 0.5;
-          ldx[2] *= 0.5;
-          ++nfail;
+      } else {
+        auto ld = [&](int j, double xc[3]) { cld3(a.x, j, xc); };
+        tile_loop<3, 3, double, double>(W, gw, nwaves, ld, nopre, lead, a.carry);
+        if (W.nlong > 0) {
+          if (!bar.sync()) return false;
+          for (int k = gw; k < W.nlong; k += nwaves) {
+            double v[3];
+            long_row_sum<3, double, double>(W, k, a.carry, v);
+            if (lane_id() == 0) lead(W.long_row[k], v, NoPre::type{});
+          }
         }
-        // x_i was read above (lx before the local solve); write x_i + ldx
-        cst(&a.x[3 * i], x0[0] + ldx[0]);
-        cst(&a.x[3 * i + 1], x0[1] + ldx[1]);
-        cst(&a.x[3 * i + 2], x0[2] + ldx[2]);
-        const double s2 = s * s;
-        const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
-        const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
-        part[0] = fmax(part[0], ndx2);
-        if (fmin(nx2, ndx2) < absSkipTol || ndx2 < a.skip_tol * fmin(nx2, ndxRef))
-          a.skip[i] = (unsigned char)a.skip_iters;
       }
-      ok = bar.reduce<1>(part, 1u);
-      if (ok) ndxRef = fmax(ndxRef, bar.red[0]);
-    }
-    if (!ok) break;
-    if (it % a.eval_every == 0) {
-      double e0;
-      ok = eval_phase(false, err, e0);
-      if (!ok) break;
-      if (blockIdx.x == 0 && threadIdx.x == 0 && nEval < a.hist_cap) a.hist[nEval] = err;
-      ++nEval;
+      if (!bar.reduce<2>(part, resMax | (resMax << 1))) return false;
+      err = bar.red[0];
+      err0 = bar.red[1];
+      if (!a.inf_norm) {
+        err /= (1.0 + n);
+        err0 /= (1.0 + n);
+      }
+      return true;
+    };
+
+    errBest = 1.7976931348623157e308;
+    converged = false;
+    itDone = 0;
+    double err = 0.0, err0 = 0.0;
+    bool ok = eval_phase(tryZero, err, err0);
+    if (ok) {
       if (err < errBest) {
         errBest = err;
         for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
       }
-      if (err < a.tol) {
-        converged = true;
-        break;
+      if (tryZero && err0 < errBest) {
+        errBest = err0;
+        for (int k = tid; k < n3; k += nthreads) {
+          cst(&a.xbest[k], 0.0);
+          cst(&a.x[k], 0.0);
+          if constexpr (kF32) cst(&a.x32[k], 0.0f);
+        }
       }
-      ndxRef /= a.eval_every;
-      // xbest must be complete before the next sweep modifies x
+      if (blockIdx.x == 0 && threadIdx.x == 0 && nEval < a.hist_cap) a.hist[nEval] = errBest;
+      // x/xbest writes above are consumed by other blocks in the color phases
       ok = bar.sync();
     }
+    errLast = err;
+    ++nEval;
+
+    double ndxRef = 0.0, prevBest = errBest;
+    int it = 1;
+    for (; ok && it <= maxIt; ++it) {
+      for (int col = 0; col < a.ncolors && ok; ++col) {
+        const int c0 = a.colors[col], c1 = a.colors[col + 1];
+        double part[1] = {0.0};
+        for (int i = c0 + tid; i < c1; i += nthreads) {
+          if (a.skip[i]) {
+            a.skip[i] = (unsigned char)(a.skip[i] - 1);
+            continue;
+          }
+          double lx[3], lb[3], x0[3];
+          cld3(a.x, i, lx);
+          x0[0] = lx[0];
+          x0[1] = lx[1];
+          x0[2] = lx[2];
+          const double reg = a.reg[i];
+          ld3(a.P.b, i, lb);
+          lb[0] -= reg * lx[0];
+          lb[1] -= reg * lx[1];
+          lb[2] -= reg * lx[2];
+          if constexpr (kF32) {
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            row_gather<float>(W32, nnz, wrowptr[i], wrowptr[i + 1], i, a.x32, acc);
+            lb[0] += (double)acc[0];
+            lb[1] += (double)acc[1];
+            lb[2] += (double)acc[2];
+          } else {
+            row_gather<double>(W, nnz, wrowptr[i], wrowptr[i + 1], i, a.x, lb);
+          }
+          double A[3][3];
+#pragma unroll
+          for (int e = 0; e < 9; ++e) A[e / 3][e % 3] = a.local[(size_t)9 * i + e];
+          const double ldx0[3] = {-lx[0], -lx[1], -lx[2]};
+          const double s = a.P.scal[i];
+          const double lres = local::solve(coulomb, a.strategy, A, lb, lx, a.P.mu[i], a.local_tol, s);
+          const bool lok = a.local_tol > lres;
+          double ldx[3] = {ldx0[0] + lx[0], ldx0[1] + lx[1], ldx0[2] + lx[2]};
+          if (!lok) {
+            ldx[0] *= 0.5;
+            ldx[1] *= 0.5;
+            ldx[2] *= 0.5;
+            ++nfail;
+          }
+          // x_i was read above (lx before the local solve); write x_i + ldx
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const double v = x0[r] + ldx[r];
+            cst(&a.x[3 * i + r], v);
+            if constexpr (kF32) cst(&a.x32[3 * i + r], (float)v);
+          }
+          const double s2 = s * s;
+          const double nx2 = s2 * (lx[0] * lx[0] + lx[1] * lx[1] + lx[2] * lx[2]);
+          const double ndx2 = s2 * (ldx[0] * ldx[0] + ldx[1] * ldx[1] + ldx[2] * ldx[2]);
+          part[0] = fmax(part[0], ndx2);
+          if (fmin(nx2, ndx2) < absSkipTol || ndx2 < a.skip_tol * fmin(nx2, ndxRef))
+            a.skip[i] = (unsigned char)a.skip_iters;
+        }
+        ok = bar.reduce<1>(part, 1u);
+        if (ok) ndxRef = fmax(ndxRef, bar.red[0]);
+      }
+      if (!ok) break;
+      if (it % a.eval_every == 0 || it == firstEval) {
+        double e0;
+        ok = eval_phase(false, err, e0);
+        if (!ok) break;
+        errLast = err;
+        if (blockIdx.x == 0 && threadIdx.x == 0 && nEval < a.hist_cap) a.hist[nEval] = err;
+        ++nEval;
+        if (err < errBest) {
+          errBest = err;
+          for (int k = tid; k < n3; k += nthreads) cst(&a.xbest[k], cld(&a.x[k]));
+        }
+        if (err < stopTol) {
+          converged = true;
+          break;
+        }
+        // the FP32 stage also ends when an evaluation no longer improves on the best by
+        // 1% (the rounding floor of its gathers is reached)
+        if (kF32 && !(err < 0.99 * prevBest)) break;
+        prevBest = fmin(prevBest, err);
+        ndxRef /= a.eval_every;
+        // xbest must be complete before the next sweep modifies x
+        ok = bar.sync();
+      }
+    }
+    itDone = it > maxIt ? maxIt : it;
+    return ok;
+  };
+
+  double errBest = 0.0, errLast = 0.0, err1 = 0.0;
+  bool converged = false, ok = true;
+  int it = 0, it1 = 0, nEval1 = 0;
+  if (a.mixed) {
+    for (int k = tid; k < n3; k += nthreads) cst(&a.x32[k], (float)cld(&a.x[k]));
+    ok = bar.sync();
+    bool conv1 = false;
+    double best1 = 0.0;
+    if (ok)
+      ok = stage(StageTag<true>{}, a.try_zero != 0, a.max_iters, a.tol * a.stop_factor, 0, it1, best1, err1, conv1);
+    nEval1 = nEval;
+    // hand over the stage's best iterate; every contact active again
+    if (ok) {
+      for (int k = tid; k < n3; k += nthreads) cst(&a.x[k], cld(&a.xbest[k]));
+      for (int k = tid; k < n; k += nthreads) a.skip[k] = 0;
+      ok = bar.sync();
+    }
+    if (ok)
+      ok = stage(StageTag<false>{}, false, max(a.eval_every, a.max_iters - it1), a.tol, 1, it, errBest, errLast,
+                 converged);
+  } else {
+    ok = stage(StageTag<false>{}, a.try_zero != 0, a.max_iters, a.tol, 0, it, errBest, errLast, converged);
   }
-  // maxIters exhausted: x = x_best (GaussSeidel.impl.hpp:202)
-  if (ok && !converged)
+  // maxIters exhausted: x = x_best (GaussSeidel.impl.hpp:202). Mixed precision always
+  // returns the FP64 stage's best iterate (its verification evaluation included), so
+  // the reported residual is that of the returned r.
+  if (ok && (!converged || a.mixed))
     for (int k = tid; k < n3; k += nthreads) cst(&a.x[k], cld(&a.xbest[k]));
   // count local failures over the grid
   {
@@ -201,11 +310,14 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
     const bool ok2 = ok && bar.reduce<1>(part, 0u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       a.out[0] = errBest;
-      a.out[1] = (double)(it > a.max_iters ? a.max_iters : it);
+      a.out[1] = (double)(it + it1);
       a.out[2] = converged ? 1.0 : 0.0;
       a.out[3] = ok2 ? bar.red[0] : -1.0;
       a.out[4] = (double)nEval;
       a.out[5] = (ok && ok2) ? 0.0 : 1.0;
+      a.out[6] = (double)it1;
+      a.out[7] = err1;
+      a.out[8] = (double)(nEval - nEval1);
     }
   }
 }

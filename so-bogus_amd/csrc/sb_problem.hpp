@@ -163,7 +163,14 @@ struct GSArgs {
   double* hist;
   int hist_cap;
   GridSync gs;
-  double* out;          // [0] residual [1] iterations [2] converged [3] local failures
+  double* out;          // [0] residual [1] iterations [2] converged [3] local failures [4] evaluations
+                        // [5] barrier failure [6] FP32-stage sweeps [7] FP32-stage residual [8] FP64 evaluations
+  // mixed precision (gs_kernel): an FP32 stage (gathers of W32 and x32) to tol * stop_factor first
+  int mixed;
+  double stop_factor;
+  TileMat<float> W32;
+  float* x32;           // FP32 copy of x during the FP32 stage
+  float* carry32;
 };
 
 // Extra inputs of the matrix-free (product) Gauss-Seidel.

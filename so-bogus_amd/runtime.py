@@ -22,7 +22,7 @@ HEADER = PKG_DIR.parent / "include" / "sobogus_hip.h"
 SB_OK, SB_NOT_CONVERGED = 0, 1
 SB_ERR = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "no device", -5: "unsupported",
           -6: "barrier timeout", -7: "non-finite residual", -8: "fixed limit exceeded"}
-SB_ERR_INVALID, SB_ERR_LIMIT = -1, -8
+SB_ERR_INVALID, SB_ERR_UNSUPPORTED, SB_ERR_LIMIT = -1, -5, -8
 PG_STANDARD, PG_DESCENT, PG_CONJUGATED, PG_APGD, PG_SPG = range(5)
 PURE_NEWTON, PURE_ENUMERATIVE, HYBRID, REV_HYBRID = range(4)
 LAW_SOC, LAW_COULOMB = 0, 1
@@ -65,7 +65,7 @@ class GSOptions(C.Structure):
     _fields_ = [("law", C.c_int), ("strategy", C.c_int), ("inf_norm", C.c_int), ("try_zero", C.c_int),
                 ("tol", C.c_double), ("max_iters", C.c_int), ("eval_every", C.c_int), ("skip_tol", C.c_double),
                 ("skip_iters", C.c_int), ("auto_regularization", C.c_double), ("grid_blocks", C.c_int),
-                ("block_threads", C.c_int)]
+                ("block_threads", C.c_int), ("precision", C.c_int), ("mixed_tol_factor", C.c_double)]
 
 
 class Stats(C.Structure):

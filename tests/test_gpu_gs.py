@@ -249,3 +249,60 @@ def test_gs_rejects_coupled_color(R, G, ctx):
     dp = R.DualFrictionProblem(ctx, prob)
     with pytest.raises(RuntimeError):
         dp.solve_gs(np.zeros(3 * prob.nc), law=R.LAW_COULOMB, colors=np.array([0, prob.nc]))
+
+
+# ---------------------------------------------------------------- mixed-precision GS
+# FP32 off-diagonal gathers (W32, x32) with the FP64 local solve and FB evaluation, then
+# FP64 sweeps from the FP32 stage's best iterate. Bar: converged, the returned r
+# re-evaluated by the FP64 oracle meets the tolerance (x1.05), and r agrees with the
+# FP64 GS solution to the tolerance scale (the SOC problem's solution is unique).
+
+@pytest.mark.parametrize("law", ["LAW_COULOMB", "LAW_SOC"])
+def test_mixed_gs_1k(R, O, G, ctx, law):
+    print(f"case: 1k mixed-precision GS {law}", flush=True)
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    perm, colors = dp.coloring(R.COLORING_GREEDY)
+    dp.apply_permutation(perm)
+    op = O.OracleProblem(prob)
+    op.permute(perm.astype(np.int32))
+    coul = law == "LAW_COULOMB"
+    kw = dict(law=getattr(R, law), tol=1e-8, max_iters=2000, colors=colors)
+    m = dp.solve_gs(np.zeros(3 * prob.nc), precision=R.PRECISION_MIXED, **kw)
+    d = dp.solve_gs(np.zeros(3 * prob.nc), precision=R.PRECISION_FP64, **kw)
+    print(f"   mixed {m.iterations} sweeps ({m.iterations - m.refine_iterations} FP32, FP32 res "
+          f"{m.mixed_residual:.2e}), res {m.residual:.3e}; FP64 {d.iterations} sweeps", flush=True)
+    assert m.converged and d.converged
+    assert m.iterations - m.refine_iterations > 0 and m.refine_iterations >= 1
+    assert op.eval(m.r, coulomb=coul) <= 1.05e-8
+    assert m.residual == pytest.approx(op.eval(m.r, coulomb=coul), rel=1e-6)
+    if not coul:
+        assert np.linalg.norm(m.r - d.r) <= 1e-3 * np.linalg.norm(d.r)
+
+
+def test_mixed_gs_50k_soc(R, O, G, ctx):
+    """50,000 contacts, SOC law, tol 1e-8, assembled W, 16 greedy colors: the mixed GS
+    converges to an oracle-verified result; its time is printed beside the FP64 GS."""
+    print("case: 50k mixed-precision GS SOC", flush=True)
+    prob = G.workload("contacts-50k")
+    dp = R.DualFrictionProblem(ctx, prob)
+    perm, colors = dp.coloring(R.COLORING_GREEDY)
+    dp.apply_permutation(perm)
+    kw = dict(law=R.LAW_SOC, tol=1e-8, max_iters=3000, colors=colors)
+    m = dp.solve_gs(np.zeros(3 * prob.nc), precision=R.PRECISION_MIXED, **kw)
+    d = dp.solve_gs(np.zeros(3 * prob.nc), precision=R.PRECISION_FP64, **kw)
+    print(f"   mixed {m.iterations} sweeps ({m.iterations - m.refine_iterations} FP32, FP32 res {m.mixed_residual:.2e}) "
+          f"{m.device_ms:.1f} ms; FP64 {d.iterations} sweeps {d.device_ms:.1f} ms", flush=True)
+    assert m.converged and d.converged
+    op = O.OracleProblem(prob)
+    op.permute(perm.astype(np.int32))
+    assert op.eval(m.r, coulomb=False) <= 1.05e-8
+    assert np.linalg.norm(m.r - d.r) <= 1e-3 * np.linalg.norm(d.r)
+
+
+def test_mixed_gs_refused_where_unsupported(R, G, ctx):
+    prob = G.workload("small-1k")
+    dp = R.DualFrictionProblem(ctx, prob, assemble=False)
+    with pytest.raises(R.SbError) as e:
+        dp.solve_pgs(np.zeros(3 * prob.nc), law=R.LAW_SOC, tol=1e-8, precision=R.PRECISION_MIXED)
+    assert e.value.code == R.SB_ERR_UNSUPPORTED
