@@ -91,7 +91,11 @@ enum sb_bsr_flags { SB_BSR_SYMMETRIC = 1 /* only blocks with col <= row are give
 /* Coloring modes */
 enum sb_coloring_mode {
   SB_COLORING_REFERENCE = 0, /* Coloring::compute as written (lower_bound test; many colors) */
-  SB_COLORING_GREEDY = 1     /* first-fit with an exact membership test (fewer colors) */
+  SB_COLORING_GREEDY = 1,    /* first-fit with an exact membership test (fewer colors) */
+  SB_COLORING_HOST = 16,     /* or'ed into the mode: compute on the host (by default the greedy rule runs on
+                                the device when W is absent or assembled from H; same result) */
+  SB_COLORING_DEVICE = 32    /* or'ed into SB_COLORING_GREEDY: on the device only (SB_ERR_LIMIT where the
+                                host would be needed) */
 };
 
 typedef struct sb_ctx sb_ctx;
@@ -239,7 +243,11 @@ int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm
 /* Coloring of the contacts (Coloring::update(true, W), Coloring.impl.hpp:20-62); perm has
    n_contacts entries and colors n_colors + 1 offsets (allocate n_contacts + 1). Two contacts
    are coupled when they share a body (the block pattern of W), so a problem without an
-   assembled W is colored from H. */
+   assembled W is colored from H. The greedy rule runs on the device (in rounds that give
+   the sequential rule's colors), the reference rule on the host, as does the greedy rule
+   where the device's limits are exceeded (a body with more than 256 contacts, more than
+   256 colors, a chain of more than 4096 coupled contacts) and for a W given by
+   sb_problem_set_w. */
 int sb_problem_coloring(sb_ctx* ctx, const sb_problem* P, int mode, int64_t* perm, int64_t* colors, int* n_colors);
 
 /* y = W x (op = SB_OP_ASSEMBLED) or H (Minv (H^T x)) (SB_OP_MATRIX_FREE). Host or device arrays. */

@@ -1510,8 +1510,91 @@ int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm
 int sb_problem_coloring(sb_ctx* ctx, const sb_problem* Pc, int mode, int64_t* perm, int64_t* colors, int* ncolors) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
+  const bool onHost = (mode & SB_COLORING_HOST) != 0, devOnly = (mode & SB_COLORING_DEVICE) != 0;
+  mode &= ~(SB_COLORING_HOST | SB_COLORING_DEVICE);
+  if (onHost && devOnly) return SB_ERR_INVALID;
   if (!Pc || !perm || !colors || !ncolors || (mode != 0 && mode != 1)) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
+  // First fit on the device when the coupling is the contact-body incidence of H (no
+  // W, or W assembled from H); the host rule otherwise, where the device rule's limits
+  // (colors, bodies per contact) are exceeded, and for the reference rule (whose
+  // thousands of colors at 50k contacts make it a parity tool, not a solver input).
+  if (devOnly && mode != SB_COLORING_GREEDY) return SB_ERR_UNSUPPORTED;
+  if (!onHost && mode == SB_COLORING_GREEDY && (!P->hasW || P->wFromH)) {
+    hipStream_t s = ctx->stream;
+    const int nc = P->nc, nb = P->nb;
+    if (P->d_hrp.n != (size_t)nc + 1) {  // a host set-up uploads the pattern on demand
+      SB_HIP(P->d_hrp.upload(P->hrp, s));
+      SB_HIP(P->d_hci.upload(P->hci, s));
+    }
+    if (P->d_htsrc.n != (size_t)P->HT.host.nblocks) SB_HIP(P->d_htsrc.upload(P->htsrc, s));
+    DBuf<int> color, info, perm32, keys, iota, offs, deg;
+    SB_HIP(color.alloc(nc));
+    SB_HIP(info.alloc(3));  // [0] colors, [1] limit exceeded, [2] contacts left after a round
+    SB_HIP(deg.alloc(1));
+    SB_HIP(hipMemsetAsync(info.p, 0, 3 * sizeof(int), s));
+    SB_HIP(hipMemsetAsync(color.p, 0xff, sizeof(int) * (size_t)nc, s));  // -1: not colored
+    SB_HIP(sb::launch_max_degree(nb, P->HT.rowptr.p, deg.p, s));
+    int hdeg = 0;
+    if ((rc = read_int(deg.p, &hdeg, s))) return rc;
+    int hinfo[3] = {0, 1, 0};  // a body with more contacts than colors: the host rule
+    if (hdeg <= sb::coloring_max_colors()) {
+      hinfo[1] = 0;
+      // rounds in groups of four; the group's last round counts the contacts left.
+      // Every round colors at least the first uncolored contact; a chain of coupled
+      // contacts longer than kMaxColorRounds goes to the host rule.
+      constexpr int kMaxColorRounds = 4096;
+      for (int rounds = 0;; rounds += 4) {
+        if (rounds >= kMaxColorRounds) {
+          hinfo[1] = 1;
+          break;
+        }
+        for (int g = 0; g < 4; ++g) {
+          if (g == 3) SB_HIP(hipMemsetAsync(info.p + 2, 0, sizeof(int), s));
+          SB_HIP(sb::launch_color_round(nc, P->d_hrp.p, P->d_hci.p, P->HT.rowptr.p, P->HT.bcol.p, P->d_htsrc.p,
+                                        color.p, g == 3 ? info.p + 2 : deg.p, info.p + 1, s));
+        }
+        SB_HIP(hipMemcpyAsync(hinfo + 1, info.p + 1, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        SB_HIP(hipStreamSynchronize(s));
+        if (hinfo[1] || hinfo[2] == 0) break;
+      }
+    }
+    if (hinfo[1] == 0) {
+      DBuf<char> tmp;
+      size_t bytes = 0;
+      SB_HIP(sb::reduce_max_int(color.p, info.p, nc, nullptr, bytes, s));
+      if ((rc = cub_tmp(tmp, bytes))) return rc;
+      SB_HIP(sb::reduce_max_int(color.p, info.p, nc, tmp.p, bytes, s));
+      if ((rc = read_int(info.p, &hinfo[0], s))) return rc;
+      const int ncol = hinfo[0] + 1;
+      // perm: the contacts by color, ascending within a color (a stable sort of the colors)
+      SB_HIP(keys.alloc(nc));
+      SB_HIP(iota.alloc(nc));
+      SB_HIP(perm32.alloc(nc));
+      SB_HIP(offs.alloc((size_t)ncol + 1));
+      SB_HIP(sb::launch_iota(nc, iota.p, s));
+      const int bits = std::max(1, sb::ceil_log2_host(ncol));
+      bytes = 0;
+      const unsigned* ck = reinterpret_cast<const unsigned*>(color.p);
+      unsigned* sk = reinterpret_cast<unsigned*>(keys.p);
+      SB_HIP(sb::sort_pairs_u32(ck, sk, iota.p, perm32.p, nc, bits, nullptr, bytes, s));
+      if ((rc = cub_tmp(tmp, bytes))) return rc;
+      SB_HIP(sb::sort_pairs_u32(ck, sk, iota.p, perm32.p, nc, bits, tmp.p, bytes, s));
+      SB_HIP(sb::launch_lower_bounds_u32(ncol + 1, sk, nc, offs.p, s));
+      std::vector<int> hp(nc), ho((size_t)ncol + 1);
+      SB_HIP(hipMemcpyAsync(hp.data(), perm32.p, sizeof(int) * hp.size(), hipMemcpyDeviceToHost, s));
+      SB_HIP(hipMemcpyAsync(ho.data(), offs.p, sizeof(int) * ho.size(), hipMemcpyDeviceToHost, s));
+      SB_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i < nc; ++i) perm[i] = hp[i];
+      for (int c = 0; c <= ncol; ++c) colors[c] = ho[c];
+      *ncolors = ncol;
+      return SB_OK;
+    }
+  }
+  if (devOnly) {
+    last_error_string() = "coloring: the device rule's limits are exceeded, or W was supplied";
+    return SB_ERR_LIMIT;
+  }
   if ((rc = sync_pattern(P))) return rc;
   std::vector<int64_t> pm, cl;
   // the pattern of W is the contact-body incidence of H: the coloring does not need W

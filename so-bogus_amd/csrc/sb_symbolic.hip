@@ -496,4 +496,80 @@ hipError_t launch_w_term_fill(int nq, const int* brow, const int* bcol, const in
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ coloring
+// First-fit coloring of the contacts (coupled: sharing a body): contact i takes the
+// smallest color not used by a contact j < i coupled to it. The rule is sequential
+// in the contacts, but contact i's color depends only on its earlier neighbours'
+// colors, so rounds of a parallel kernel give exactly the sequential result: each
+// round colors every contact whose earlier neighbours are all colored (a color once
+// written is final; a contact that reads an unfinished neighbour waits for the next
+// round). The rounds follow the longest chain of increasing coupled contacts.
+constexpr int kColorWords = 4;  // colors < 256
+
+// deg: the largest number of contacts at one body (atomicMax), from the body rows of H^T
+__global__ void k_max_degree(int nb, const int* __restrict__ trp, int* __restrict__ deg) {
+  int d = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x)
+    d = max(d, trp[k + 1] - trp[k]);
+  atomicMax(deg, d);
+}
+
+// One round. Body k's contacts are tc[q], q in [trp[k], trp[k + 1]), ascending;
+// tsrc[q] < 0 marks an empty body row's zero block. left: contacts still uncolored;
+// info: set if a contact needs more than 64 * kColorWords colors.
+__global__ void __launch_bounds__(256) k_color_round(int nc, const int* __restrict__ hrp, const int* __restrict__ hci,
+                                                     const int* __restrict__ trp, const int* __restrict__ tc,
+                                                     const int* __restrict__ tsrc, int* color, int* left, int* info) {
+  int nleft = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    if (__hip_atomic_load(&color[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0) continue;
+    unsigned long long used[kColorWords] = {0ull, 0ull, 0ull, 0ull};
+    bool ready = true;
+    for (int p = hrp[i]; p < hrp[i + 1] && ready; ++p) {
+      const int k = hci[p];
+      for (int q = trp[k]; q < trp[k + 1]; ++q) {
+        const int j = tc[q];
+        if (j >= i || tsrc[q] < 0) break;
+        const int c = __hip_atomic_load(&color[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c < 0) {
+          ready = false;
+          break;
+        }
+#pragma unroll
+        for (int w = 0; w < kColorWords; ++w)
+          if ((c >> 6) == w) used[w] |= 1ull << (c & 63);
+      }
+    }
+    if (!ready) {
+      ++nleft;
+      continue;
+    }
+    int c = 64 * kColorWords;
+#pragma unroll
+    for (int w = kColorWords - 1; w >= 0; --w)
+      if (~used[w]) c = 64 * w + (int)__builtin_ctzll(~used[w]);
+    if (c >= 64 * kColorWords) {
+      *info = 1;
+      continue;
+    }
+    __hip_atomic_store(&color[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (nleft) atomicAdd(left, nleft);
+}
+
+hipError_t launch_max_degree(int nb, const int* trp, int* deg, hipStream_t s) {
+  (void)hipGetLastError();
+  hipError_t e = hipMemsetAsync(deg, 0, sizeof(int), s);
+  if (e != hipSuccess) return e;
+  k_max_degree<<<grid_n(nb), 256, 0, s>>>(nb, trp, deg);
+  return hipGetLastError();
+}
+hipError_t launch_color_round(int nc, const int* hrp, const int* hci, const int* trp, const int* tc, const int* tsrc,
+                              int* color, int* left, int* info, hipStream_t s) {
+  (void)hipGetLastError();
+  k_color_round<<<grid_n(nc), 256, 0, s>>>(nc, hrp, hci, trp, tc, tsrc, color, left, info);
+  return hipGetLastError();
+}
+int coloring_max_colors() { return 64 * kColorWords; }
+
 }  // namespace sb

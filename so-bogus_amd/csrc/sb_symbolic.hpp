@@ -81,6 +81,16 @@ hipError_t launch_w_term_fill(int nq, const int* brow, const int* bcol, const in
 
 hipError_t launch_fill_f64(size_t n, double v, double* out, hipStream_t s);
 
+// ---- first-fit coloring of the contacts (coupled: sharing a body), in rounds:
+// color[nc] starts at -1; each round colors the contacts whose earlier neighbours are
+// colored and adds the others to *left; *info is set if a contact needs more than
+// coloring_max_colors() colors. trp / tc / tsrc: the body rows of H^T (tiled row
+// pointer, contact per block, source block, -1 for an empty row's zero block).
+hipError_t launch_max_degree(int nb, const int* trp, int* deg, hipStream_t s);
+hipError_t launch_color_round(int nc, const int* hrp, const int* hci, const int* trp, const int* tc, const int* tsrc,
+                              int* color, int* left, int* info, hipStream_t s);
+int coloring_max_colors();
+
 int ceil_log2_host(int64_t v);
 
 }  // namespace sb

@@ -30,7 +30,7 @@ OP_ASSEMBLED, OP_MATRIX_FREE = 0, 1
 SCALING_REFERENCE, SCALING_DIAGONAL, SCALING_NONE = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 ADJOINT_UNROLL, ADJOINT_IMPLICIT = 0, 1
-COLORING_REFERENCE, COLORING_GREEDY = 0, 1
+COLORING_REFERENCE, COLORING_GREEDY, COLORING_HOST, COLORING_DEVICE = 0, 1, 16, 32
 BATCH_CALLER_ORDER = 2
 BATCH_SPREAD_TEAMS = 4
 BSR_SYMMETRIC = 1

@@ -202,9 +202,10 @@ def test_device_setup_equals_host_setup(R, G, ctx, case):
         for a, b in zip(wh, wd):
             assert np.array_equal(a, b)
     for mode in (R.COLORING_REFERENCE, R.COLORING_GREEDY):
-        ph, ch = host.coloring(mode)
-        pd, cd = dev.coloring(mode)
-        assert np.array_equal(ph, pd) and np.array_equal(ch, cd)
+        ph, ch = host.coloring(mode | R.COLORING_HOST)
+        for prob in (host, dev):
+            pd, cd = prob.coloring(mode)
+            assert np.array_equal(ph, pd) and np.array_equal(ch, cd)
 
 
 def test_device_setup_from_device_arrays(R, G, ctx):
@@ -255,3 +256,40 @@ def test_device_setup_zipf_1m(R, G, ctx):
     assert lt["nlong"] > 0
     x = np.random.default_rng(9).standard_normal(3 * p.nc)
     assert np.array_equal(host.apply_w(x, op=R.OP_MATRIX_FREE), dev.apply_w(x, op=R.OP_MATRIX_FREE))
+
+
+@pytest.mark.parametrize("case", ["uniform-5k", "contacts-50k", "ragged", "zipf-1m"])
+def test_device_coloring_equals_host(R, G, ctx, case):
+    """The greedy coloring on the device (rounds that reproduce the sequential rule)
+    gives the host's permutation and color offsets, also for the ragged problem's
+    contacts with no, one, three, twice the same, or a hundred bodies; where the device
+    rule's limits are exceeded (the 1M Zipf problem's hubs: more contacts at a body than
+    colors) the device-only call reports SB_ERR_LIMIT; the reference rule stays on the
+    host."""
+    import time
+    print(f"case: device coloring vs host coloring, {case}", flush=True)
+    if case == "uniform-5k":
+        p, assemble = G.generate(nc=5000, nb=2000, seed=21), True
+    elif case == "contacts-50k":
+        p, assemble = G.workload("contacts-50k"), True
+    elif case == "ragged":
+        p, assemble = _ragged_problem(G), True
+    else:
+        p, assemble = G.workload("zipf-1m"), False
+    dp = R.DualFrictionProblem(ctx, p, assemble=assemble)
+    mode = R.COLORING_GREEDY
+    if case == "zipf-1m":
+        with pytest.raises(R.SbError) as e:
+            dp.coloring(mode | R.COLORING_DEVICE)
+        assert e.value.code == R.SB_ERR_LIMIT
+        return
+    t0 = time.perf_counter()
+    pd, cd = dp.coloring(mode | R.COLORING_DEVICE)
+    t1 = time.perf_counter()
+    ph, ch = dp.coloring(mode | R.COLORING_HOST)
+    t2 = time.perf_counter()
+    print(f"   {len(cd) - 1} colors, device {1e3 * (t1 - t0):.1f} ms, host {1e3 * (t2 - t1):.1f} ms", flush=True)
+    assert np.array_equal(ph, pd) and np.array_equal(ch, cd)
+    with pytest.raises(R.SbError) as e:
+        dp.coloring(R.COLORING_REFERENCE | R.COLORING_DEVICE)
+    assert e.value.code == R.SB_ERR_UNSUPPORTED
