@@ -74,6 +74,10 @@ struct DBuf {
     return hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
   }
   hipError_t upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
 };
 
 std::vector<float> to_float(const std::vector<double>& v) {
@@ -304,9 +308,15 @@ struct sb_problem {
   // the host paths that need them (hostPattern false until then); the lower
   // triangle of W and its tiles' source blocks stay on the device
   bool hostPattern = true;
+  bool hostSetup = false;  // created with SB_PROBLEM_HOST_SETUP: pattern changes stay on the host
   int nnz = 0;        // blocks of H
   int64_t nwl = 0;    // blocks of W's lower triangle (0 without W)
   DBuf<int> d_wrp, d_wci, d_wsrc;
+  // assembly order keys: okey[c] is contact c's index when W was assembled (a contact
+  // permutation keeps each W block's orientation); wtermOkey: the product terms of
+  // the W tiles follow them (device set-up) rather than the indices (host set-up)
+  DBuf<int> d_okey;
+  bool wtermOkey = false;
   DBuf<double> d_hraw, d_mraw;
   // workspace of the FP32 (mixed-precision) projected-gradient solver
   DBuf<float> X32[4], MX32[4], G32[3], xbest32, wbody32, carry32;
@@ -818,8 +828,21 @@ int sync_host(sb_problem* P) {
   SB_HIP(hipMemcpyAsync(P->diag.data(), P->d_diag.p, sizeof(double) * 9 * (size_t)nc, hipMemcpyDeviceToHost, s));
   SB_HIP(hipMemcpyAsync(P->scal.data(), P->d_scal.p, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost, s));
   SB_HIP(hipStreamSynchronize(s));
-  if (P->hasW && P->wFromH)
-    sbhost::assembleW(nc, nb, P->hrp.data(), P->hci.data(), P->hv.data(), P->minv.data(), P->wrp, P->wci, P->wv);
+  if (P->hasW && P->wFromH) {
+    // the lower triangle of W as the solvers hold it: the first blocks of each row of
+    // the two-triangle tiles (a row's lower part precedes its upper part)
+    const sbhost::Tiles& T = P->W.host;
+    const size_t nq = (size_t)T.nblocks;
+    std::vector<double> tv(9 * nq);
+    SB_HIP(hipMemcpyAsync(tv.data(), P->W.vals.p, sizeof(double) * tv.size(), hipMemcpyDeviceToHost, s));
+    SB_HIP(hipStreamSynchronize(s));
+    P->wv.resize((size_t)9 * P->wci.size());
+    for (int i = 0; i < nc; ++i)
+      for (int t = P->wrp[i]; t < P->wrp[i + 1]; ++t) {
+        const size_t q = (size_t)T.rowptr[i] + (size_t)(t - P->wrp[i]);
+        for (int e = 0; e < 9; ++e) P->wv[(size_t)9 * t + e] = tv[(size_t)e * nq + q];
+      }
+  }
   P->hostStale = false;
   return SB_OK;
 }
@@ -963,13 +986,15 @@ int build_device(sb_problem* P) {
     // product terms of every W tile block
     const int nq = P->W.host.nblocks;
     SB_HIP(tcnt.alloc(nq));
-    SB_HIP(sb::launch_w_term_count(nq, P->W.brow.p, P->W.bcol.p, P->d_wsrc.p, P->d_hrp.p, P->d_hci.p, tcnt.p, s));
+    SB_HIP(sb::launch_w_term_count(nq, P->W.brow.p, P->W.bcol.p, P->d_wsrc.p, P->d_okey.p, P->d_hrp.p, P->d_hci.p,
+                                   tcnt.p, s));
     if ((rc = scan_exclusive(tcnt.p, P->d_wtptr, nq, tmp, s))) return rc;
     int nt = 0;
     if ((rc = read_int(P->d_wtptr.p + nq, &nt, s))) return rc;
     SB_HIP(P->d_wterm.alloc(std::max<size_t>(2, 2 * (size_t)nt)));
-    SB_HIP(sb::launch_w_term_fill(nq, P->W.brow.p, P->W.bcol.p, P->d_wsrc.p, P->d_hrp.p, P->d_hci.p,
+    SB_HIP(sb::launch_w_term_fill(nq, P->W.brow.p, P->W.bcol.p, P->d_wsrc.p, P->d_okey.p, P->d_hrp.p, P->d_hci.p,
                                   P->d_wtptr.p, P->d_wterm.p, s));
+    P->wtermOkey = true;
     SB_HIP(hipStreamSynchronize(s));  // the local buffers are freed on return
   }
   P->devSetup = true;
@@ -991,7 +1016,7 @@ int build_device(sb_problem* P) {
   SB_HIP(sb::launch_diag_scal(nc, P->d_hrp.p, P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->d_diag.p, P->d_scal.p, s));
   if (P->hasW)
     SB_HIP(sb::launch_w_numeric(P->W.host.nblocks, P->W.brow.p, P->W.bcol.p, P->d_wtptr.p, P->d_wterm.p,
-                                P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->W.vals.p, P->W.vals32.p, s));
+                                P->d_okey.p, P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->W.vals.p, P->W.vals32.p, s));
   if ((rc = alloc_workspace(P))) return rc;
   SB_HIP(hipStreamSynchronize(s));
   return SB_OK;
@@ -1318,7 +1343,8 @@ int create_problem(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, 
   P->nc = nc;
   P->nb = nb;
   P->hasW = (flags & SB_PROBLEM_ASSEMBLE_W) != 0;
-  if (!(flags & SB_PROBLEM_HOST_SETUP)) {
+  P->hostSetup = (flags & SB_PROBLEM_HOST_SETUP) != 0;
+  if (!P->hostSetup) {
     // raw arrays to the device, set-up there; the host copies are made when needed
     hipStream_t s = ctx->stream;
     P->nnz = nnz;
@@ -1328,6 +1354,8 @@ int create_problem(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, int nb, 
     SB_HIP(P->d_mraw.upload(minv, (size_t)nb * 36, s));
     SB_HIP(P->d_b.upload(b, (size_t)nc * 3, s));
     SB_HIP(P->d_mu.upload(mu, (size_t)nc, s));
+    SB_HIP(P->d_okey.alloc(nc));
+    SB_HIP(sb::launch_iota(nc, P->d_okey.p, s));
     return build_device(P.get());
   }
   P->hrp.assign(rp, rp + nc + 1);
@@ -1378,6 +1406,8 @@ int create_problem_device(sb_ctx* ctx, std::unique_ptr<sb_problem>& P, int nc, i
   if ((rc = dup(P->d_mraw, minv, (size_t)nb * 36))) return rc;
   if ((rc = dup(P->d_b, b, (size_t)nc * 3))) return rc;
   if ((rc = dup(P->d_mu, mu, (size_t)nc))) return rc;
+  SB_HIP(P->d_okey.alloc(nc));
+  SB_HIP(sb::launch_iota(nc, P->d_okey.p, s));
   return build_device(P.get());
 }
 
@@ -1436,13 +1466,53 @@ int sb_problem_apply_permutation(sb_ctx* ctx, sb_problem* P, const int64_t* perm
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!P || !perm) return SB_ERR_INVALID;
-  if ((rc = sync_host(P))) return rc;
   const int n = P->nc;
   std::vector<int> inv(n, -1);
   for (int i = 0; i < n; ++i) {
     if (perm[i] < 0 || perm[i] >= n || inv[perm[i]] >= 0) return SB_ERR_INVALID;
     inv[perm[i]] = i;
   }
+  // On the device (then a new device set-up) unless the problem was set up on the host
+  // or carries a W that was not assembled from H.
+  if (!P->hostSetup && (!P->hasW || P->wFromH)) {
+    hipStream_t s = ctx->stream;
+    if (!P->devRawValid) {
+      SB_HIP(P->d_hraw.upload(P->hv, s));
+      SB_HIP(P->d_mraw.upload(P->minv, s));
+      P->devRawValid = true;
+    }
+    if (P->d_hrp.n != (size_t)n + 1) {
+      SB_HIP(P->d_hrp.upload(P->hrp, s));
+      SB_HIP(P->d_hci.upload(P->hci, s));
+    }
+    std::vector<int> p32(perm, perm + n);
+    DBuf<int> dperm, len, nrp, nci;
+    DBuf<double> nhv, nb, nmu;
+    DBuf<char> tmp;
+    SB_HIP(dperm.upload(p32, s));
+    SB_HIP(len.alloc(n));
+    SB_HIP(sb::launch_perm_lens(n, dperm.p, P->d_hrp.p, len.p, s));
+    if ((rc = scan_exclusive(len.p, nrp, n, tmp, s))) return rc;
+    SB_HIP(nci.alloc((size_t)P->nnz));
+    SB_HIP(nhv.alloc((size_t)P->nnz * 18));
+    SB_HIP(nb.alloc((size_t)n * 3));
+    SB_HIP(nmu.alloc(n));
+    SB_HIP(sb::launch_perm_rows(n, dperm.p, P->d_hrp.p, P->d_hci.p, P->d_hraw.p, P->d_b.p, P->d_mu.p, nrp.p, nci.p,
+                                nhv.p, nb.p, nmu.p, s));
+    SB_HIP(hipStreamSynchronize(s));
+    DBuf<int> nokey;
+    SB_HIP(nokey.alloc(n));
+    SB_HIP(sb::launch_gather_int(n, dperm.p, P->d_okey.p, nokey.p, s));
+    SB_HIP(hipStreamSynchronize(s));
+    P->d_hrp.swap(nrp);
+    P->d_hci.swap(nci);
+    P->d_hraw.swap(nhv);
+    P->d_b.swap(nb);
+    P->d_mu.swap(nmu);
+    P->d_okey.swap(nokey);
+    return build_device(P);
+  }
+  if ((rc = sync_host(P))) return rc;
   // H rows, b, mu: new i <- old perm[i]
   std::vector<int> rp(n + 1, 0), ci;
   std::vector<double> hv, b(3 * (size_t)n), mu(n);
@@ -2537,7 +2607,8 @@ int sb_problem_update_values(sb_ctx* ctx, sb_problem* P, const double* hvals, co
     SB_HIP(sb::launch_diag_scal(nc, P->d_hrp.p, P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->d_diag.p, P->d_scal.p, s));
     if (P->hasW)
       SB_HIP(sb::launch_w_numeric(P->W.host.nblocks, P->W.brow.p, P->W.bcol.p, P->d_wtptr.p, P->d_wterm.p,
-                                  P->d_hci.p, P->d_hraw.p, P->d_mraw.p, P->W.vals.p, P->W.vals32.p, s));
+                                  P->wtermOkey ? P->d_okey.p : nullptr, P->d_hci.p, P->d_hraw.p, P->d_mraw.p,
+                                  P->W.vals.p, P->W.vals32.p, s));
   }
   SB_HIP(hipStreamSynchronize(s));
   ++P->version;

@@ -102,13 +102,16 @@ __global__ void __launch_bounds__(256) k_diag_scal(int nc, const int* __restrict
 }
 
 // Numeric phase of W = H Minv H^T on the expanded (both triangles) tiled pattern.
-// A block (i, j) below or on the diagonal is the sum over its terms
-// wterm[wtptr[q] .. wtptr[q+1]) = (p_i, p_j) of H_pi Minv_k H_pj^T (k the shared
-// body); a block above it is the transpose of block (j, i), whose terms it holds,
-// as the lower-triangle storage of the symmetric W implies.
+// Block q is the sum over its terms wterm[wtptr[q] .. wtptr[q+1]) = (p_a, p_b) of
+// H_pa Minv_k H_pb^T (k the shared body), p_a of contact hi and p_b of contact lo;
+// it is stored transposed where the block's row is lo (the transpose of block
+// (hi, lo), whose terms it holds, as the lower-triangle storage of the symmetric W
+// implies). hi is the contact with the larger index, or with okey the larger
+// assembly order key (okey[c]: contact c's index when W was assembled).
 __global__ void __launch_bounds__(256) k_w_numeric(int nq, const int* __restrict__ brow, const int* __restrict__ bcol,
                                                    const int* __restrict__ wtptr, const int2* __restrict__ wterm,
-                                                   const int* __restrict__ hci, const double* __restrict__ hraw,
+                                                   const int* __restrict__ okey, const int* __restrict__ hci,
+                                                   const double* __restrict__ hraw,
                                                    const double* __restrict__ mraw, double* __restrict__ vals,
                                                    float* __restrict__ vals32) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
@@ -117,7 +120,7 @@ __global__ void __launch_bounds__(256) k_w_numeric(int nq, const int* __restrict
       const int2 pq = wterm[t];
       w_block(hraw, mraw, pq.x, pq.y, hci[pq.x], acc);
     }
-    const bool upper = bcol[q] > brow[q];
+    const bool upper = okey ? okey[bcol[q]] > okey[brow[q]] : bcol[q] > brow[q];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -311,12 +314,12 @@ hipError_t launch_diag_scal(int nc, const int* hrp, const int* hci, const double
 }
 
 hipError_t launch_w_numeric(int nq, const int* brow, const int* bcol, const int* wtptr, const int* wterm,
-                            const int* hci, const double* hraw, const double* mraw, double* vals, float* vals32,
-                            hipStream_t s) {
+                            const int* okey, const int* hci, const double* hraw, const double* mraw, double* vals,
+                            float* vals32, hipStream_t s) {
   (void)hipGetLastError();
   if (nq > 0)
-    k_w_numeric<<<grid_for_n(nq), 256, 0, s>>>(nq, brow, bcol, wtptr, reinterpret_cast<const int2*>(wterm), hci, hraw,
-                                                mraw, vals, vals32);
+    k_w_numeric<<<grid_for_n(nq), 256, 0, s>>>(nq, brow, bcol, wtptr, reinterpret_cast<const int2*>(wterm), okey,
+                                                hci, hraw, mraw, vals, vals32);
   return hipGetLastError();
 }
 

@@ -22,9 +22,10 @@ hipError_t launch_tile_values(int br, int bc, bool transposed, int nq, const int
 hipError_t launch_minv_layout(int nb, const double* raw, double* em, float* em32, hipStream_t s);
 hipError_t launch_diag_scal(int nc, const int* hrp, const int* hci, const double* hraw, const double* mraw,
                             double* diag, double* scal, hipStream_t s);
+// okey: the contacts' assembly order keys (nullptr: their indices)
 hipError_t launch_w_numeric(int nq, const int* brow, const int* bcol, const int* wtptr, const int* wterm,
-                            const int* hci, const double* hraw, const double* mraw, double* vals, float* vals32,
-                            hipStream_t s);
+                            const int* okey, const int* hci, const double* hraw, const double* mraw, double* vals,
+                            float* vals32, hipStream_t s);
 hipError_t launch_mass_inverse(int nb, const double* M, double* Minv, int* err, hipStream_t s);
 hipError_t launch_permute3(int64_t n, const int* perm, const double* src, double* dst, bool scatter, hipStream_t s);
 hipError_t launch_cadoux_rhs(int nc, const double* mu, const double* wx, const double* b0, double* bout,

@@ -451,35 +451,45 @@ hipError_t launch_full_fill(int nc, const int* wrp, const int* wci, const int* u
   return hipGetLastError();
 }
 
-// Terms of block (i, j), i >= j: a over contact i's H blocks, b over contact j's, at
-// the same body, in that nesting order (the order the host assembly sums them).
+// Terms of block (i, j): a over the H blocks of contact hi, b over those of contact lo,
+// at the same body, in that nesting order (the order the host assembly sums them),
+// where hi is the one of i, j with the larger assembly order key okey (the contact's
+// index when W was assembled: a later permutation keeps each block's orientation,
+// as a symmetric permutation of the assembled W does).
+__device__ __forceinline__ void w_block_rows(const int* __restrict__ okey, int i, int j, int& hi, int& lo) {
+  const bool iHi = okey[i] >= okey[j];
+  hi = iHi ? i : j;
+  lo = iHi ? j : i;
+}
 __global__ void k_w_term_count(int nq, const int* __restrict__ brow, const int* __restrict__ bcol,
-                               const int* __restrict__ src, const int* __restrict__ hrp, const int* __restrict__ hci,
-                               int* __restrict__ cnt) {
+                               const int* __restrict__ src, const int* __restrict__ okey, const int* __restrict__ hrp,
+                               const int* __restrict__ hci, int* __restrict__ cnt) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
     int c = 0;
     if (src[q] >= 0) {
-      const int i = max(brow[q], bcol[q]), j = min(brow[q], bcol[q]);
+      int i, j;
+      w_block_rows(okey, brow[q], bcol[q], i, j);
       for (int a = hrp[i]; a < hrp[i + 1]; ++a)
         for (int b = hrp[j]; b < hrp[j + 1]; ++b) c += hci[a] == hci[b] ? 1 : 0;
     }
     cnt[q] = c;
   }
 }
-hipError_t launch_w_term_count(int nq, const int* brow, const int* bcol, const int* src, const int* hrp,
-                               const int* hci, int* cnt, hipStream_t s) {
+hipError_t launch_w_term_count(int nq, const int* brow, const int* bcol, const int* src, const int* okey,
+                               const int* hrp, const int* hci, int* cnt, hipStream_t s) {
   (void)hipGetLastError();
-  if (nq > 0) k_w_term_count<<<grid_n(nq), 256, 0, s>>>(nq, brow, bcol, src, hrp, hci, cnt);
+  if (nq > 0) k_w_term_count<<<grid_n(nq), 256, 0, s>>>(nq, brow, bcol, src, okey, hrp, hci, cnt);
   return hipGetLastError();
 }
 // tptr: exclusive prefix sums (nq + 1 entries)
 __global__ void k_w_term_fill(int nq, const int* __restrict__ brow, const int* __restrict__ bcol,
-                              const int* __restrict__ src, const int* __restrict__ hrp, const int* __restrict__ hci,
-                              const int* __restrict__ tptr, int* __restrict__ term) {
+                              const int* __restrict__ src, const int* __restrict__ okey, const int* __restrict__ hrp,
+                              const int* __restrict__ hci, const int* __restrict__ tptr, int* __restrict__ term) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
     if (src[q] < 0) continue;
     int o = tptr[q];
-    const int i = max(brow[q], bcol[q]), j = min(brow[q], bcol[q]);
+    int i, j;
+    w_block_rows(okey, brow[q], bcol[q], i, j);
     for (int a = hrp[i]; a < hrp[i + 1]; ++a)
       for (int b = hrp[j]; b < hrp[j + 1]; ++b)
         if (hci[a] == hci[b]) {
@@ -489,10 +499,53 @@ __global__ void k_w_term_fill(int nq, const int* __restrict__ brow, const int* _
         }
   }
 }
-hipError_t launch_w_term_fill(int nq, const int* brow, const int* bcol, const int* src, const int* hrp,
-                              const int* hci, const int* tptr, int* term, hipStream_t s) {
+hipError_t launch_w_term_fill(int nq, const int* brow, const int* bcol, const int* src, const int* okey,
+                              const int* hrp, const int* hci, const int* tptr, int* term, hipStream_t s) {
   (void)hipGetLastError();
-  if (nq > 0) k_w_term_fill<<<grid_n(nq), 256, 0, s>>>(nq, brow, bcol, src, hrp, hci, tptr, term);
+  if (nq > 0) k_w_term_fill<<<grid_n(nq), 256, 0, s>>>(nq, brow, bcol, src, okey, hrp, hci, tptr, term);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ permutation
+// New contact i is old contact perm[i] (DualFrictionProblem::applyPermutation): row
+// lengths, then each row's columns and 18 values, b and mu.
+__global__ void k_perm_lens(int nc, const int* __restrict__ perm, const int* __restrict__ rp, int* __restrict__ len) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
+    len[i] = rp[perm[i] + 1] - rp[perm[i]];
+}
+__global__ void k_perm_rows(int nc, const int* __restrict__ perm, const int* __restrict__ rp,
+                            const int* __restrict__ ci, const double* __restrict__ hv, const double* __restrict__ b,
+                            const double* __restrict__ mu, const int* __restrict__ nrp, int* __restrict__ nci,
+                            double* __restrict__ nhv, double* __restrict__ nb, double* __restrict__ nmu) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const int src = perm[i];
+    int dst = nrp[i];
+    for (int p = rp[src]; p < rp[src + 1]; ++p, ++dst) {
+      nci[dst] = ci[p];
+      for (int e = 0; e < 18; ++e) nhv[(size_t)dst * 18 + e] = hv[(size_t)p * 18 + e];
+    }
+    for (int k = 0; k < 3; ++k) nb[3 * (size_t)i + k] = b[3 * (size_t)src + k];
+    nmu[i] = mu[src];
+  }
+}
+__global__ void k_gather_int(int n, const int* __restrict__ idx, const int* __restrict__ src, int* __restrict__ dst) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[idx[i]];
+}
+hipError_t launch_gather_int(int n, const int* idx, const int* src, int* dst, hipStream_t s) {
+  (void)hipGetLastError();
+  if (n > 0) k_gather_int<<<grid_n(n), 256, 0, s>>>(n, idx, src, dst);
+  return hipGetLastError();
+}
+hipError_t launch_perm_lens(int nc, const int* perm, const int* rp, int* len, hipStream_t s) {
+  (void)hipGetLastError();
+  if (nc > 0) k_perm_lens<<<grid_n(nc), 256, 0, s>>>(nc, perm, rp, len);
+  return hipGetLastError();
+}
+hipError_t launch_perm_rows(int nc, const int* perm, const int* rp, const int* ci, const double* hv, const double* b,
+                            const double* mu, const int* nrp, int* nci, double* nhv, double* nb, double* nmu,
+                            hipStream_t s) {
+  (void)hipGetLastError();
+  if (nc > 0) k_perm_rows<<<grid_n(nc), 256, 0, s>>>(nc, perm, rp, ci, hv, b, mu, nrp, nci, nhv, nb, nmu);
   return hipGetLastError();
 }
 

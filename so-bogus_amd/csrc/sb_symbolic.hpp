@@ -73,13 +73,23 @@ hipError_t launch_upper_rows(int nc, const uint64_t* ukeys, int n, const int* wr
 // full rows: lower row i (columns <= i), then its upper entries (columns > i), both ascending
 hipError_t launch_full_fill(int nc, const int* wrp, const int* wci, const int* uptr, const uint64_t* ukeys,
                             const int* frp, int* fci, hipStream_t s);
-// product terms of every W tile block (i, j): (a, b) over H blocks a of max(i, j) and b of min(i, j) at the same body
-hipError_t launch_w_term_count(int nq, const int* brow, const int* bcol, const int* src, const int* hrp,
-                               const int* hci, int* cnt, hipStream_t s);
-hipError_t launch_w_term_fill(int nq, const int* brow, const int* bcol, const int* src, const int* hrp,
-                              const int* hci, const int* tptr, int* term, hipStream_t s);
+// product terms of every W tile block (i, j): (a, b) over the H blocks a of hi and b of lo at
+// the same body, hi / lo the one of i, j with the larger / smaller assembly order key okey
+hipError_t launch_w_term_count(int nq, const int* brow, const int* bcol, const int* src, const int* okey,
+                               const int* hrp, const int* hci, int* cnt, hipStream_t s);
+hipError_t launch_w_term_fill(int nq, const int* brow, const int* bcol, const int* src, const int* okey,
+                              const int* hrp, const int* hci, const int* tptr, int* term, hipStream_t s);
 
 hipError_t launch_fill_f64(size_t n, double v, double* out, hipStream_t s);
+
+// dst[i] = src[idx[i]]
+hipError_t launch_gather_int(int n, const int* idx, const int* src, int* dst, hipStream_t s);
+// ---- contact permutation (new i <- old perm[i]): row lengths, then rows (nrp: the new
+// row pointer), b and mu
+hipError_t launch_perm_lens(int nc, const int* perm, const int* rp, int* len, hipStream_t s);
+hipError_t launch_perm_rows(int nc, const int* perm, const int* rp, const int* ci, const double* hv, const double* b,
+                            const double* mu, const int* nrp, int* nci, double* nhv, double* nb, double* nmu,
+                            hipStream_t s);
 
 // ---- first-fit coloring of the contacts (coupled: sharing a body), in rounds:
 // color[nc] starts at -1; each round colors the contacts whose earlier neighbours are

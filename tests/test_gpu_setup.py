@@ -293,3 +293,31 @@ def test_device_coloring_equals_host(R, G, ctx, case):
     with pytest.raises(R.SbError) as e:
         dp.coloring(R.COLORING_REFERENCE | R.COLORING_DEVICE)
     assert e.value.code == R.SB_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("assemble", [True, False])
+def test_device_permutation_equals_host(R, G, ctx, assemble):
+    """sb_problem_apply_permutation on a device set-up permutes H, b and mu on the device
+    and sets the problem up again there; the result equals the host path's (a problem
+    created with the host set-up) array for array, and the colored GS agrees."""
+    print(f"case: device permutation vs host permutation, assemble={assemble}", flush=True)
+    p = _ragged_problem(G, seed=5)
+    host = R.DualFrictionProblem(ctx, p, assemble=assemble, host_setup=True)
+    dev = R.DualFrictionProblem(ctx, p, assemble=assemble)
+    perm, colors = dev.coloring(R.COLORING_GREEDY)
+    for prob in (host, dev):
+        prob.apply_permutation(perm)
+    for which in ((0, 1, 2) if assemble else (0, 1)):
+        _same_layout(host, dev, which)
+    assert host.sizes() == dev.sizes()
+    x = np.random.default_rng(10).standard_normal(3 * p.nc)
+    for op in ((R.OP_ASSEMBLED, R.OP_MATRIX_FREE) if assemble else (R.OP_MATRIX_FREE,)):
+        assert np.array_equal(host.apply_w(x, op=op), dev.apply_w(x, op=op))
+        assert host.eval_with(x, law=R.LAW_SOC, op=op, scaling=R.SCALING_DIAGONAL) == \
+            dev.eval_with(x, law=R.LAW_SOC, op=op, scaling=R.SCALING_DIAGONAL)
+    if assemble:
+        for a, b in zip(host.w_lower(), dev.w_lower()):
+            assert np.array_equal(a, b)
+        gh = host.solve_gs(np.zeros(3 * p.nc), law=R.LAW_SOC, tol=1e-8, max_iters=500, colors=colors)
+        gd = dev.solve_gs(np.zeros(3 * p.nc), law=R.LAW_SOC, tol=1e-8, max_iters=500, colors=colors)
+        assert gh.iterations == gd.iterations and np.array_equal(gh.r, gd.r)
