@@ -35,35 +35,30 @@ __device__ __forceinline__ void cld3(const double* p, int i, double v[3]) {
   v[1] = mld(p + 3 * i + 1);
   v[2] = mld(p + 3 * i + 2);
 }
-// acc += sum over blocks q in [q0, q1) of row i, j = bcol[q] != i, of W_q x_j, block
-// by block in row order. The columns of four blocks, then their values and x_j, are
-// loaded before their products are formed: a row costs about one dependent memory
-// round trip per four blocks rather than one per block. x is updated during the
-// sweep by other blocks of the grid, after a barrier: plain loads (mld).
+// Lanes per contact in a color phase: the g lanes of a group (g a power of two, at
+// most kGroup, as many as the grid holds for the color in one pass) gather the blocks
+// of the contact's W row together (lane k: blocks k, k + g, ...), so a row costs one
+// dependent memory round trip for its columns and one for its values and x_j; the
+// group's sums are combined by butterfly shuffles, and its first lane solves.
+constexpr int kGroup = 16;
 template <typename T>
-__device__ __forceinline__ void row_gather(const TileMat<T>& W, size_t nnz, int q0, int q1, int i, const T* x,
-                                           T acc[3]) {
-  for (int qb = q0; qb < q1; qb += 4) {
-    int js[4];
+__device__ __forceinline__ void group_gather(const TileMat<T>& W, size_t nnz, int q0, int q1, int i, int sub, int g,
+                                             const T* x, T acc[3]) {
+  acc[0] = acc[1] = acc[2] = T(0);
+  for (int q = q0 + sub; q < q1; q += g) {
+    const int j = W.bcol[q];
+    if (j == i) continue;
+    T xj[3], w[9];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) js[k] = qb + k < q1 ? W.bcol[qb + k] : i;
-    T xv[4][3], wv[4][9];
+    for (int c = 0; c < 3; ++c) xj[c] = mld(x + 3 * j + c);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (js[k] == i) continue;
+    for (int e = 0; e < 9; ++e) w[e] = W.vals[(size_t)e * nnz + q];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xv[k][c] = mld(x + 3 * js[k] + c);
-#pragma unroll
-      for (int e = 0; e < 9; ++e) wv[k][e] = W.vals[(size_t)e * nnz + qb + k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (js[k] == i) continue;
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-        acc[r] += wv[k][3 * r + 0] * xv[k][0] + wv[k][3 * r + 1] * xv[k][1] + wv[k][3 * r + 2] * xv[k][2];
-    }
+    for (int r = 0; r < 3; ++r) acc[r] += w[3 * r + 0] * xj[0] + w[3 * r + 1] * xj[1] + w[3 * r + 2] * xj[2];
   }
+  for (int m = g >> 1; m >= 1; m >>= 1)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) acc[r] += __shfl_xor(acc[r], m, kGroup);
 }
 }  // namespace
 
@@ -192,33 +187,42 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
       for (int col = 0; col < a.ncolors && ok; ++col) {
         const int c0 = a.colors[col], c1 = a.colors[col + 1];
         double part[1] = {0.0};
-        for (int i = c0 + tid; i < c1; i += nthreads) {
-          if (a.skip[i]) {
-            a.skip[i] = (unsigned char)(a.skip[i] - 1);
+        // g consecutive lanes per contact (g divides 64 and nthreads is a multiple of 64:
+        // a group never straddles a wave and leaves the loop as a whole)
+        int g = kGroup;
+        while (g > 1 && (c1 - c0) * g > nthreads) g >>= 1;
+        for (int sg = tid; sg < (c1 - c0) * g; sg += nthreads) {
+          const int i = c0 + sg / g, sub = sg % g;
+          const unsigned char sk = a.skip[i];
+          if (sk) {
+            if (sub == 0) a.skip[i] = (unsigned char)(sk - 1);
             continue;
           }
-          double lx[3], lb[3], x0[3];
-          cld3(a.x, i, lx);
+          // the solving lane's own data first, then the group's gather
+          double lx[3] = {0.0, 0.0, 0.0}, bi[3] = {0.0, 0.0, 0.0}, reg = 0.0, A[3][3];
+          if (sub == 0) {
+            cld3(a.x, i, lx);
+            ld3(a.P.b, i, bi);
+            reg = a.reg[i];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) A[e / 3][e % 3] = a.local[(size_t)9 * i + e];
+          }
+          double lb[3], x0[3];
+          if constexpr (kF32) {
+            float acc[3];
+            group_gather<float>(W32, nnz, wrowptr[i], wrowptr[i + 1], i, sub, g, a.x32, acc);
+            lb[0] = (double)acc[0];
+            lb[1] = (double)acc[1];
+            lb[2] = (double)acc[2];
+          } else {
+            group_gather<double>(W, nnz, wrowptr[i], wrowptr[i + 1], i, sub, g, a.x, lb);
+          }
+          if (sub != 0) continue;
           x0[0] = lx[0];
           x0[1] = lx[1];
           x0[2] = lx[2];
-          const double reg = a.reg[i];
-          ld3(a.P.b, i, lb);
-          lb[0] -= reg * lx[0];
-          lb[1] -= reg * lx[1];
-          lb[2] -= reg * lx[2];
-          if constexpr (kF32) {
-            float acc[3] = {0.0f, 0.0f, 0.0f};
-            row_gather<float>(W32, nnz, wrowptr[i], wrowptr[i + 1], i, a.x32, acc);
-            lb[0] += (double)acc[0];
-            lb[1] += (double)acc[1];
-            lb[2] += (double)acc[2];
-          } else {
-            row_gather<double>(W, nnz, wrowptr[i], wrowptr[i + 1], i, a.x, lb);
-          }
-          double A[3][3];
 #pragma unroll
-          for (int e = 0; e < 9; ++e) A[e / 3][e % 3] = a.local[(size_t)9 * i + e];
+          for (int r = 0; r < 3; ++r) lb[r] += bi[r] - reg * lx[r];
           const double ldx0[3] = {-lx[0], -lx[1], -lx[2]};
           const double s = a.P.scal[i];
           const double lres = local::solve(coulomb, a.strategy, A, lb, lx, a.P.mu[i], a.local_tol, s);
