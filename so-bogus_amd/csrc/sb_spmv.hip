@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 
 #include "sb_law.hpp"
 #include "sb_problem.hpp"
@@ -194,42 +195,84 @@ hipError_t launch_w(const DevProblem<double>& P, const double* x, double* y, dou
 // y = H w for an H whose rows all hold two blocks (a contact between two bodies),
 // blocks 2i and 2i + 1 for row i: one lane per row, no scan. Element e of the two
 // blocks is adjacent in the element-major layout, so a lane reads both with one
-// 16-byte non-temporal load (a wave: 1 KB contiguous per element). The row sum is
-// formed as in the tiled kernel (each block's products in order, then the second
-// block's sum plus the first's): the same bits.
+// 16-byte non-temporal load (a wave: 1 KB contiguous per element); a body vector is
+// three 16-byte loads. The row sum is formed as in the tiled kernel (each block's
+// products in order, then the second block's sum plus the first's): the same bits.
+// A wave's 64 results (1536 contiguous bytes of y) go through LDS and are written
+// with 16-byte non-temporal stores. Aligned: w and y 16-byte aligned, else 8-byte
+// accesses.
 typedef double dbl2 __attribute__((ext_vector_type(2)));
+template <bool Aligned>
 __global__ void __launch_bounds__(256) k_h_pairs(TileMat<double> A, const double* __restrict__ w,
                                                  double* __restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.nrows) return;
-  const size_t nb2 = (size_t)A.nblocks / 2;
-  const dbl2* v2 = reinterpret_cast<const dbl2*>(A.vals) + i;
-  const int ca = A.bcol[2 * i], cb = A.bcol[2 * i + 1];
-  dbl2 e[18];
+  __shared__ double sy[4][3 * 64];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int nc = A.nrows;
+  if (i < nc) {
+    const size_t nb2 = (size_t)A.nblocks / 2;
+    const dbl2* v2 = reinterpret_cast<const dbl2*>(A.vals) + i;
+    const int ca = A.bcol[2 * i], cb = A.bcol[2 * i + 1];
+    dbl2 e[18];
 #pragma unroll
-  for (int k = 0; k < 18; ++k) e[k] = __builtin_nontemporal_load(v2 + (size_t)k * nb2);
-  double xa[6], xb[6];
+    for (int k = 0; k < 18; ++k) e[k] = __builtin_nontemporal_load(v2 + (size_t)k * nb2);
+    double xa[6], xb[6];
+    if (Aligned) {
+      const dbl2* wa = reinterpret_cast<const dbl2*>(w + 6 * (size_t)ca);
+      const dbl2* wb = reinterpret_cast<const dbl2*>(w + 6 * (size_t)cb);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    xa[c] = w[6 * (size_t)ca + c];
-    xb[c] = w[6 * (size_t)cb + c];
-  }
+      for (int k = 0; k < 3; ++k) {
+        const dbl2 a2 = wa[k], b2 = wb[k];
+        xa[2 * k] = a2.x;
+        xa[2 * k + 1] = a2.y;
+        xb[2 * k] = b2.x;
+        xb[2 * k + 1] = b2.y;
+      }
+    } else {
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    double aa = 0.0, ab = 0.0;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      aa += e[r * 6 + c].x * xa[c];
-      ab += e[r * 6 + c].y * xb[c];
+      for (int c = 0; c < 6; ++c) {
+        xa[c] = w[6 * (size_t)ca + c];
+        xb[c] = w[6 * (size_t)cb + c];
+      }
     }
-    y[3 * (size_t)i + r] = ab + aa;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double aa = 0.0, ab = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        aa += e[r * 6 + c].x * xa[c];
+        ab += e[r * 6 + c].y * xb[c];
+      }
+      sy[wv][3 * l + r] = ab + aa;
+    }
+  }
+  __syncthreads();  // every thread of the block reaches it
+  const int i0 = i - l;
+  if (i0 < nc) {
+    const int nd = 3 * min(64, nc - i0);  // the wave's results
+    double* yo = y + 3 * (size_t)i0;
+    if (Aligned) {
+      dbl2* yo2 = reinterpret_cast<dbl2*>(yo);
+      for (int k = l; k < nd / 2; k += 64) {
+        const dbl2 v = {sy[wv][2 * k], sy[wv][2 * k + 1]};
+        __builtin_nontemporal_store(v, yo2 + k);
+      }
+      if ((nd & 1) && l == 0) yo[nd - 1] = sy[wv][nd - 1];
+    } else {
+      for (int k = l; k < nd; k += 64) yo[k] = sy[wv][k];
+    }
   }
 }
 
 hipError_t launch_h(const DevProblem<double>& P, const double* w, double* y, double* carry, hipStream_t s) {
   (void)hipGetLastError();  // clear a stale error: report this launch only
   if (P.H.pairs) {
-    if (P.H.nrows > 0) k_h_pairs<<<(P.H.nrows + 255) / 256, 256, 0, s>>>(P.H, w, y);
+    const bool aligned = ((uintptr_t)w % 16 == 0) && ((uintptr_t)y % 16 == 0);
+    const unsigned grid = (unsigned)((P.H.nrows + 255) / 256);
+    if (P.H.nrows > 0) {
+      if (aligned) k_h_pairs<true><<<grid, 256, 0, s>>>(P.H, w, y);
+      else k_h_pairs<false><<<grid, 256, 0, s>>>(P.H, w, y);
+    }
   } else {
     run_tiles<3, 6>(P.H, VecLoader6{w}, Store3{y}, carry, s);
   }
