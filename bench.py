@@ -308,15 +308,24 @@ def adjoint_100k(ctx, R, G, n_dirs: int = 64) -> dict:
                       "fd_h": h, "fd_max_rel_error": float(max(errs)), "fd_median_rel_error": float(np.median(errs)),
                       "fd_map": "the map replayed on the recorded piece (projection branches and De Saxce "
                                 "directions of the unperturbed run)"},
-           "implicit": {"solver": "restarted GMRES(40) on (I - dT/dr^T) lambda = g", "ms": imp["device_ms"],
-                        "arnoldi_steps": imp["iterations"], "relative_residual": imp["change"],
-                        "converged": bool(imp["converged"])},
+           "implicit": {"solver": "restarted GMRES(40) on (I - dT/dr^T) lambda = g",
+                        "at": f"the FP64 fixed point (FB 1e-10, {s64.iterations} iterations)",
+                        "ms": imp64["device_ms"], "arnoldi_steps": imp64["iterations"],
+                        "relative_residual": imp64["change"], "converged": bool(imp64["converged"])},
+           "implicit_at_tolerance_solution": {
+               "at": f"the mixed-precision solution (FB {sol.residual:.1e})", "ms": imp["device_ms"],
+               "arnoldi_steps": imp["iterations"], "relative_residual": imp["change"],
+               "converged": bool(imp["converged"]),
+               "note": "the adjoint system is taken at an iterate that meets the tolerance, not at the fixed "
+                       "point; its conditioning depends on that iterate's branches"},
            "implicit_vs_unroll_limit": {"fixed_point": f"FP64 APGD to FB 1e-10 ({s64.iterations} iterations)",
                                         "unroll_steps": K_long, "relative_difference": agree,
                                         "implicit_ms": imp64["device_ms"], "unroll_ms": unr["device_ms"]}}
     log(f"adjoint-100k (Coulomb): gradient of {K} steps {ad['device_ms']:.1f} ms ({out['unroll']['gradient_vs_forward']:.2f}x "
-        f"the forward steps), FD max rel error {max(errs):.1e} on {n_dirs} directions; implicit (GMRES) "
-        f"{imp['device_ms']:.0f} ms, {imp['iterations']} steps; implicit vs {K_long}-step unroll: "
+        f"the forward steps), FD max rel error {max(errs):.1e} on {n_dirs} directions; implicit (GMRES) at the "
+        f"FB 1e-10 point {imp64['device_ms']:.0f} ms, {imp64['iterations']} steps (at the 1e-6 solution: "
+        f"{imp['iterations']} steps, {'converged' if imp['converged'] else 'not converged'}); implicit vs "
+        f"{K_long}-step unroll: "
         + ", ".join(f"{k} {v:.1e}" for k, v in agree.items()))
     return out
 
@@ -546,17 +555,27 @@ def main() -> int:
     dp50 = R.DualFrictionProblem(ctx, p50, assemble=True)
     r50 = R.DeviceArray(ctx, 3 * p50.nc)
     single = {}
+    # APGD's trajectory on this problem turns on rounding-level differences (line-search
+    # decisions near ties), the FP32 stage's most of all: the solve is timed from r0 = 0
+    # and from four starts perturbed at 1e-8, and the median reported
+    starts = [np.zeros(3 * p50.nc)] + [1e-8 * np.random.default_rng(k).standard_normal(3 * p50.nc)
+                                        for k in range(1, 5)]
     for name, prec in (("fp64", R.PRECISION_FP64), ("mixed", R.PRECISION_MIXED)):
         o = R.pg_options(precision=prec, **{**kw, "max_iters": 20000})
-        times = []
-        for _ in range(4):
-            r50.zero()
+        r50.zero()
+        dp50.solve_pg(r50, o, on_device=True)  # warm-up
+        times, its = [], []
+        for x0 in starts:
+            r50.upload(x0)
             s1 = dp50.solve_pg(r50, o, on_device=True)
             times.append(s1.device_ms)
-        single[name] = {"ms": float(np.median(times[1:])), "iterations": s1.iterations, "spmvs": s1.spmvs,
+            its.append(s1.iterations)
+        single[name] = {"ms": float(np.median(times)), "ms_per_start": times, "iterations_per_start": its,
                         "residual": s1.residual, "refine_iterations": s1.refine_iterations,
-                        "contact_solves_per_s": p50.nc / (np.median(times[1:]) * 1e-3)}
-    log(f"contacts-50k: FP64 {single['fp64']['ms']:.2f} ms, mixed {single['mixed']['ms']:.2f} ms per solve")
+                        "contact_solves_per_s": p50.nc / (np.median(times) * 1e-3),
+                        "starts": "r0 = 0 and four starts perturbed by 1e-8 N(0, 1); median"}
+    log(f"contacts-50k: FP64 {single['fp64']['ms']:.2f} ms (iterations {single['fp64']['iterations_per_start']}), "
+        f"mixed {single['mixed']['ms']:.2f} ms (iterations {single['mixed']['iterations_per_start']}) per solve, median")
     cpu_threads = args.cpu_threads or cpu_share()
     tight = tight_50k(ctx, R, G, 0 if args.no_cpu else cpu_threads)
     adj = adjoint_100k(ctx, R, G)

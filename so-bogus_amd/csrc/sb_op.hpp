@@ -9,6 +9,9 @@
 
 namespace sb {
 
+template <typename T>
+constexpr bool kPairsPass = sizeof(T) == 4;
+
 // Minv_k v for the element-major [36][nb] storage.
 template <typename T>
 __device__ __forceinline__ void minv_apply(const T* minv, int nb, int k, const T v[6], T w[6]) {
@@ -60,7 +63,12 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
 #pragma unroll
     for (int r = 0; r < 6; ++r) xc[r] = Team::ld(&wbody[6 * k + r]);
   };
-  tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, carry);
+  // FP32: one lane per contact when every row of H holds two blocks (FP64 keeps the
+  // tiled pass: its 36 values per lane would cost the kernels their occupancy)
+  if (kPairsPass<T> && P.H.pairs)
+    pairs_loop<T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead);
+  else
+    tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, carry);
   return true;
 }
 

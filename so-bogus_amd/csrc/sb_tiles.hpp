@@ -154,6 +154,44 @@ __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int st
   }
 }
 
+// One lane per row for a matrix whose rows all hold two blocks, blocks 2i and 2i + 1
+// of row i (TileMat::pairs): rows 64 u .. 64 u + 63 for units u = first, first +
+// stride, ... Element e of a row's two blocks is one 2-vector load; every lane
+// does its row's leader work (no scan, no idle lanes). The sums equal tile_gather's
+// (each block's products in order, then the second block's sum plus the first's).
+template <typename T, typename Loader, typename Pre, typename Leader>
+__device__ __forceinline__ void pairs_loop(const TileMat<T>& A, int first, int stride, Loader& load, Pre& pre,
+                                           Leader& lead) {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  const int nunits = (A.nrows + SB_WAVE - 1) / SB_WAVE;
+  const size_t nb2 = (size_t)A.nblocks / 2;
+  for (int u = first; u < nunits; u += stride) {
+    const int i = u * SB_WAVE + lane_id();
+    if (i >= A.nrows) continue;
+    auto rp = pre(i);
+    const T2* v2 = reinterpret_cast<const T2*>(A.vals) + i;
+    T2 e[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) e[k] = gld(v2 + (size_t)k * nb2);
+    const int ca = gld(A.bcol + 2 * i), cb = gld(A.bcol + 2 * i + 1);
+    T xa[6], xb[6];
+    load(ca, xa);
+    load(cb, xb);
+    T v[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      T aa = T(0), ab = T(0);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        aa += e[r * 6 + c].x * xa[c];
+        ab += e[r * 6 + c].y * xb[c];
+      }
+      v[r] = ab + aa;
+    }
+    lead(i, v, rp);
+  }
+}
+
 // Sum of the carries of long row k (one wave), fixed order: lane-strided partial
 // sums (four tiles in flight per lane) then the xor tree. Result valid in every
 // lane. Coherent: the carries were written by other blocks of the same kernel and
