@@ -421,9 +421,21 @@ int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, const sb
    iteration count in one full solve (long-running problems then get larger
    teams), times the two best geometries of each team sizing on full solves, and keeps
    the fastest for later solves of this batch. Solves are reproducible between
-   calls to this function: only it changes the schedule. */
+   calls to this function and sb_batch_set_iteration_hints: only they change the
+   schedule. */
 int sb_batch_autotune(sb_ctx* ctx, sb_batch* B, const sb_pg_options* opts, int probe_iters, sb_batch_config* best,
                       double* best_ms);
+/* Expected iteration count of every problem (hints[n_problems], both stages together;
+   0 = unknown), used by the planner as sb_batch_autotune's recorded counts are: a
+   problem expected to iterate longer gets a larger team and starts earlier. Callers
+   pass, for example, the counts of the previous time step's batch, or counts predicted
+   from the problem sizes by a model fitted on another batch. They change the schedule
+   only (every result still meets the tolerance). hints = NULL forgets them (every
+   problem weighted alike). sb_batch_get_iteration_hints reads the counts in use
+   (recorded by sb_batch_autotune or set here). No counterpart in the reference: it
+   describes the device schedule of a loop of independent solveWith calls. */
+int sb_batch_set_iteration_hints(sb_batch* B, const int* hints);
+int sb_batch_get_iteration_hints(const sb_batch* B, int* hints);
 
 /* ---------------------------------------------------------------- raw device memory helpers
    (for callers that keep vectors resident in HBM between calls) */

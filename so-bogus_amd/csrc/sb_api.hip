@@ -3288,12 +3288,31 @@ int sb_batch_sizes(const sb_batch* B, int* n, int64_t* total) {
   return SB_OK;
 }
 
+int sb_batch_set_iteration_hints(sb_batch* B, const int* hints) {
+  if (!B) return SB_ERR_INVALID;
+  if (!hints) {
+    B->iterHint.assign(B->n, 0);
+    return SB_OK;
+  }
+  for (int k = 0; k < B->n; ++k)
+    if (hints[k] < 0) return SB_ERR_INVALID;
+  B->iterHint.assign(hints, hints + B->n);
+  return SB_OK;
+}
+
+int sb_batch_get_iteration_hints(const sb_batch* B, int* hints) {
+  if (!B || !hints) return SB_ERR_INVALID;
+  std::copy(B->iterHint.begin(), B->iterHint.end(), hints);
+  return SB_OK;
+}
+
 int sb_batch_solve(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, double* r,
                    int on_device, sb_batch_stats* st) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!B || !o || !r || B->ctx != ctx) return SB_ERR_INVALID;
   // the plan (and so every problem's arithmetic) changes only through sb_batch_autotune
+  // and sb_batch_set_iteration_hints
   return batch_run(ctx, B, o, cfg, r, on_device, st, false);
 }
 

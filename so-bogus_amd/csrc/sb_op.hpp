@@ -12,6 +12,16 @@ namespace sb {
 template <typename T>
 constexpr bool kPairsPass = sizeof(T) == 4;
 
+// H^T pass with two tiles per wave in flight (tile_loop_pairs)
+#ifndef SB_HT_TILE_PAIRS32
+#define SB_HT_TILE_PAIRS32 1
+#endif
+#ifndef SB_HT_TILE_PAIRS64
+#define SB_HT_TILE_PAIRS64 0
+#endif
+template <typename T>
+constexpr bool kHtTilePairs = sizeof(T) == 4 ? SB_HT_TILE_PAIRS32 != 0 : SB_HT_TILE_PAIRS64 != 0;
+
 // Minv_k v for the element-major [36][nb] storage.
 template <typename T>
 __device__ __forceinline__ void minv_apply(const T* minv, int nb, int k, const T v[6], T w[6]) {
@@ -49,7 +59,10 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
   }
   NoPre nopre;
   auto leadA = [&](int k, T v[6], NoPre::type) { body(k, v); };
-  tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry);
+  if constexpr (kHtTilePairs<T>)
+    tile_loop_pairs<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry);
+  else
+    tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry);
   if (!tm.sync()) return false;
   if (P.HT.nlong > 0) {
     for (int k = tm.gw; k < P.HT.nlong; k += tm.nwaves) {

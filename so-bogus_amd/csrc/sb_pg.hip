@@ -116,6 +116,30 @@ __device__ __forceinline__ void proj_step(T mu, T al, const T x[3], const T g[3]
   project<T>(mu, xs[0], xs[1], xs[2]);
 }
 
+// Column loader of a line-search trial's H^T pass: the candidate
+// P(x_j - alpha g_j) of contact j (split into the loads and the projection, so
+// that a pass with two tiles in flight issues both tiles' loads first).
+template <typename Team, typename T>
+struct CandidateLoader {
+  static constexpr bool kSplit = true;
+  const T* X;
+  const T* G;
+  const double* mu;
+  T al;
+  struct Raw {
+    T x[3], g[3], mu;
+  };
+  __device__ __forceinline__ Raw fetch(int j) const {
+    Raw r;
+    tld3<Team, T>(X, j, r.x);
+    tld3<Team, T>(G, j, r.g);
+    r.mu = (T)gld(mu + j);
+    return r;
+  }
+  __device__ __forceinline__ void finish(const Raw& r, T xc[3]) const { proj_step<T>(r.mu, al, r.x, r.g, xc); }
+  __device__ __forceinline__ void operator()(int j, T xc[3]) const { finish(fetch(j), xc); }
+};
+
 struct InitRow {
   double x[3], b[3], mu, s;
 };
@@ -305,12 +329,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
       for (int ls = 0; ls < prm.ls_iters && ok; ++ls) {
         double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const T al = (T)alpha;
-        auto ld = [&](int j, T xc[3]) {
-          T xv[3], gv[3];
-          tld3<Team, T>(Xx, j, xv);
-          tld3<Team, T>(Gx, j, gv);
-          proj_step<T>((T)gld(P.mu + j), al, xv, gv, xc);
-        };
+        CandidateLoader<Team, T> ld{Xx, Gx, P.mu, al};
         auto lead = [&](int i, T v[3], const TrialRow<T>& r) {
           T xst[3];
           proj_step<T>((T)r.mu, al, r.x, r.g, xst);
@@ -421,12 +440,7 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         r.s = rowScale(i);
         return r;
       };
-      auto ld = [&](int j, T xc[3]) {
-        T xv[3], gv[3];
-        tld3<Team, T>(Xx, j, xv);
-        tld3<Team, T>(Gx, j, gv);
-        proj_step<T>((T)gld(P.mu + j), al, xv, gv, xc);
-      };
+      CandidateLoader<Team, T> ld{Xx, Gx, P.mu, al};
       auto lead = [&](int i, T v[3], const SpgRow<T>& r) {
         T xst[3];
         proj_step<T>((T)r.mu, al, r.x, r.g, xst);

@@ -16,6 +16,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sb_device.hpp"
 
 namespace sb {
@@ -64,6 +66,10 @@ __device__ __forceinline__ TileIdx tile_indices(const TileMat<T>& A, int t) {
   return ix;
 }
 
+template <int BR, typename Acc, typename Leader, typename Row>
+__device__ __forceinline__ void tile_finish(int t, const TileIdx& ix, Acc v[BR], Leader& lead, const Row& rp,
+                                            Acc* carry);
+
 // Processes tile t of A whose indices are ix. load(col, xc[BC]) returns the
 // column vector (in the accumulation type Acc). pre(row) is evaluated by every
 // active lane (so the row's own data is fetched while the gathers are in flight);
@@ -101,6 +107,17 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
       v[r] = acc;
     }
   }
+  tile_finish<BR>(t, ix, v, lead, rp, carry);
+}
+
+// The end of a tile: the segmented scan over the lanes' block products v, then the
+// row leaders (or, for a piece of a long row, its carry).
+template <int BR, typename Acc, typename Leader, typename Row>
+__device__ __forceinline__ void tile_finish(int t, const TileIdx& ix, Acc v[BR], Leader& lead, const Row& rp,
+                                            Acc* carry) {
+  const int l = lane_id();
+  const int row = ix.row;
+  const bool act = row >= 0;
   const int steps = ix.tl >= 0 ? 6 : -ix.tl - 1;  // a long tile is one row across the wave
   for (int s = 0; s < steps; ++s) {
     const int d = 1 << s;
@@ -122,6 +139,96 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
     } else {
       lead(row, v, rp);
     }
+  }
+}
+
+// A loader may split its work into fetch(j), which issues the loads of column j's
+// vector and returns them, and finish(raw, xc), which uses them (declared by a
+// member kSplit = true); tile_pair issues both tiles' fetches before either finish.
+template <typename L, typename = void>
+struct SplitLoader {
+  static constexpr bool value = false;
+};
+template <typename L>
+struct SplitLoader<L, std::void_t<decltype(L::kSplit)>> {
+  static constexpr bool value = L::kSplit;
+};
+
+// Tiles ta and tb of A together (a pass without row data, e.g. the H^T pass of the
+// matrix-free operator): both tiles' block values and column vectors are requested
+// before either tile's products, scan and leaders, so a wave has two tiles' loads in
+// flight. An index set with row -1 in every lane (past the last tile) does nothing.
+template <int BR, int BC, typename T, typename Acc, typename Loader, typename Leader>
+__device__ __forceinline__ void tile_pair(const TileMat<T>& A, int ta, const TileIdx& ia, int tb, const TileIdx& ib,
+                                          Loader& load, Leader& lead, Acc* carry) {
+  const int l = lane_id();
+  const bool aa = ia.row >= 0, ab = ib.row >= 0;
+  const size_t nb = (size_t)A.nblocks;
+  T ba[BR * BC], bb[BR * BC];
+  if (aa) {
+    const T* vq = A.vals + ia.q0 + l;
+#pragma unroll
+    for (int e = 0; e < BR * BC; ++e) ba[e] = gld(vq + (size_t)e * nb);
+  }
+  if (ab) {
+    const T* vq = A.vals + ib.q0 + l;
+#pragma unroll
+    for (int e = 0; e < BR * BC; ++e) bb[e] = gld(vq + (size_t)e * nb);
+  }
+  Acc xa[BC], xb[BC];
+  if constexpr (SplitLoader<Loader>::value) {
+    decltype(load.fetch(0)) ra{}, rb{};
+    if (aa) ra = load.fetch(ia.col);
+    if (ab) rb = load.fetch(ib.col);
+    if (aa) load.finish(ra, xa);
+    if (ab) load.finish(rb, xb);
+  } else {
+    if (aa) load(ia.col, xa);
+    if (ab) load(ib.col, xb);
+  }
+  Acc va[BR], vb[BR];
+#pragma unroll
+  for (int r = 0; r < BR; ++r) {
+    Acc acc_a = Acc(0), acc_b = Acc(0);
+#pragma unroll
+    for (int c = 0; c < BC; ++c) {
+      acc_a += Acc(ba[r * BC + c]) * xa[c];
+      acc_b += Acc(bb[r * BC + c]) * xb[c];
+    }
+    va[r] = aa ? acc_a : Acc(0);
+    vb[r] = ab ? acc_b : Acc(0);
+  }
+  const NoPre::type none{};
+  tile_finish<BR>(ta, ia, va, lead, none, carry);
+  tile_finish<BR>(tb, ib, vb, lead, none, carry);
+}
+
+// Tiles first, first + stride, ... of A two at a time (tile_pair), with the indices
+// of the next two loaded before the current two are processed.
+template <int BR, int BC, typename T, typename Acc, typename Loader, typename Leader>
+__device__ __forceinline__ void tile_loop_pairs(const TileMat<T>& A, int first, int stride, Loader& load,
+                                                Leader& lead, Acc* carry) {
+  if (first >= A.ntiles) return;
+  auto idx = [&](int t) {
+    if (t < A.ntiles) return tile_indices(A, t);
+    TileIdx none;
+    none.q0 = none.q1 = 0;
+    none.row = -1;
+    none.col = 0;
+    none.tl = -1;
+    return none;
+  };
+  TileIdx ca = idx(first), cb = idx(first + stride);
+  for (int t = first; t < A.ntiles; t += 2 * stride) {
+    const int tn = t + 2 * stride;
+    TileIdx na = ca, nb = cb;
+    if (tn < A.ntiles) {
+      na = idx(tn);
+      nb = idx(tn + stride);
+    }
+    tile_pair<BR, BC, T, Acc>(A, t, ca, t + stride, cb, load, lead, carry);
+    ca = na;
+    cb = nb;
   }
 }
 

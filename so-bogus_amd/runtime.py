@@ -48,7 +48,8 @@ EXPORTED = [
     "sb_gs_solve", "sb_pgs_solve", "sb_pg_solve_cadoux", "sb_gs_solve_cadoux", "sb_problem_update_values",
     "sb_mass_inverse", "sb_device_alloc", "sb_device_free", "sb_memcpy_h2d", "sb_memcpy_d2h", "sb_synchronize",
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
-    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_memset_device", "sb_adjoint_options_default",
+    "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_batch_set_iteration_hints",
+    "sb_batch_get_iteration_hints", "sb_memset_device", "sb_adjoint_options_default",
     "sb_adjoint", "sb_problem_update",
 ]
 
@@ -186,6 +187,8 @@ def lib():
         "sb_batch_solve": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), C.POINTER(BatchConfig), _vp, C.c_int,
                                      C.POINTER(BatchStats)]),
         "sb_batch_autotune": (C.c_int, [_vp, _vp, C.POINTER(PGOptions), C.c_int, C.POINTER(BatchConfig), _dp]),
+        "sb_batch_set_iteration_hints": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+        "sb_batch_get_iteration_hints": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -709,6 +712,24 @@ class Batch:
                            total_spmvs=st.total_spmvs, device_ms=st.device_ms, stage_spmvs=stage, phase_us=phase,
                            config=(st.config.block_threads, st.config.blocks_per_cu,
                                    st.config.contacts_per_block))
+
+    def set_iteration_hints(self, hints=None):
+        """Expected iteration count of every problem (0 = unknown; None forgets them):
+        the planner sizes teams and orders queues with them."""
+        if hints is None:
+            _check(lib().sb_batch_set_iteration_hints(self.h, None), "sb_batch_set_iteration_hints")
+            return
+        h = np.ascontiguousarray(hints, dtype=np.int32)
+        if h.shape != (self.n,):
+            raise ValueError(f"need {self.n} hints")
+        _check(lib().sb_batch_set_iteration_hints(self.h, h.ctypes.data_as(C.POINTER(C.c_int))),
+               "sb_batch_set_iteration_hints")
+
+    def iteration_hints(self) -> np.ndarray:
+        h = np.zeros(self.n, dtype=np.int32)
+        _check(lib().sb_batch_get_iteration_hints(self.h, h.ctypes.data_as(C.POINTER(C.c_int))),
+               "sb_batch_get_iteration_hints")
+        return h
 
     def autotune(self, opts: PGOptions | None = None, probe_iters: int = 50, **kw):
         o = opts if opts is not None else pg_options(**kw)

@@ -188,3 +188,34 @@ def test_batch_locality_order_is_transparent(R, O, ctx, small_batch):
     for k, (a, c) in enumerate(zip(B.split(back), C.split(rc.r))):
         if rc.converged[k]:
             assert np.linalg.norm(a - c) <= 1e-3 * max(np.linalg.norm(c), 1e-12), k
+
+
+def test_batch_iteration_hints(R, O, ctx, small_batch):
+    """Caller-provided iteration counts change the schedule (team sizes, queue order),
+    not the quality: every result still meets the tolerance in the oracle. They read
+    back as set; None forgets them; a negative count is refused."""
+    print("[batch] iteration hints", flush=True)
+    B = R.Batch(ctx, small_batch)
+    kw = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL, tol=1e-6,
+              max_iters=5000, precision=R.PRECISION_MIXED)
+    cfg = (256, 1, 250)  # teams of several workgroups for the larger problems
+    assert np.all(B.iteration_hints() == 0)
+    base = B.solve(config=cfg, **kw)
+    # the largest problem expected to iterate longest, the others short: its team grows
+    hints = np.full(B.n, 50, dtype=np.int32)
+    hints[int(np.argmax(B.ncs))] = 2000
+    B.set_iteration_hints(hints)
+    assert np.array_equal(B.iteration_hints(), hints)
+    res = B.solve(config=cfg, **kw)
+    assert res.n_converged == base.n_converged
+    _check_each(O, small_batch, res, B.split(res.r), 1e-6, 5000)
+    # the counts of a previous solve (what a simulation passes from its last step)
+    B.set_iteration_hints(base.iterations)
+    res2 = B.solve(config=cfg, **kw)
+    _check_each(O, small_batch, res2, B.split(res2.r), 1e-6, 5000)
+    B.set_iteration_hints(None)
+    assert np.all(B.iteration_hints() == 0)
+    again = B.solve(config=cfg, **kw)
+    assert np.array_equal(again.r, base.r)
+    with pytest.raises(R.SbError):
+        B.set_iteration_hints(np.full(B.n, -1, dtype=np.int32))
