@@ -603,6 +603,22 @@ def main() -> int:
                                              "frac_of_peak": nbytes / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                              "note": "each launch timed alone after a 768 MB copy (L2 and "
                                                      "Infinity Cache evicted)"}}
+        # the same products with the contacts listed body by body (as a simulator that
+        # collects contacts per body passes them): the H^T gathers of contact vectors then
+        # run along memory for each contact's first body instead of at random
+        p1b = G.body_order(p1m)
+        dp1b = R.DualFrictionProblem(ctx, p1b, assemble=False)
+        for kind, name in ((1, "h_x_1m"), (2, "ht_minv_1m"), (3, "matrix_free_wx_1m")):
+            nbytes = spmv[name]["bytes"]
+            ms = dp1b.bench_spmv(kind, reps=20)
+            msf = dp1b.bench_spmv(kind | R.BENCH_FLUSH_CACHES, reps=10)
+            spmv[name + "_body_order"] = {
+                "ms": ms, "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9,
+                "frac_of_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "caches_flushed": {"ms": msf, "GBs": nbytes / (msf * 1e-3) / 1e9,
+                                   "frac_of_peak": nbytes / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                "note": "zipf-1m with its contacts in body order (generator.body_order)"}
+        del dp1b, p1b
         # the 1M-contact Zipf problem has no solution at FB 1e-6 (its hubs over-constrain the
         # bodies); FP64 vs mixed is compared on a fixed number of APGD iterations
         zipf = {}
@@ -633,7 +649,7 @@ def main() -> int:
         zipf["fp64_in_solver_vs_stand_alone"] = (zipf["fp64"]["us_per_operator_application"] /
                                                  zipf["stand_alone_matrix_free_wx_us"])
         del r1m, dp1m
-        log("1M-contact SpMV: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in spmv.items() if k.endswith("1m")))
+        log("1M-contact SpMV: " + ", ".join(f"{k} {v['GBs']:.0f} GB/s" for k, v in spmv.items() if "_1m" in k))
         log(f"zipf-1m, 100 APGD iterations: FP64 {zipf['fp64']['us_per_operator_application']:.0f} us, FP32 stage "
             f"{zipf['fp32_stage']['us_per_operator_application']:.0f} us per operator application "
             f"(stand-alone FP64 W x {zipf['stand_alone_matrix_free_wx_us']:.0f} us)")

@@ -686,8 +686,13 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
   return p;
 }
 
+// Single-problem matrix-free APGD / Descent / SPG solves form each line-search
+// candidate in a phase of its own (pg_kernel_cand) from this many contacts on
+constexpr int kCandPhaseContacts = 200000;
+
 template <typename T>
-const void* pg_grid_fn(bool mf) {
+const void* pg_grid_fn(bool mf, bool candPhase = false) {
+  if (mf && candPhase) return (const void*)&sb::pg_kernel_cand<T>;
   return mf ? (const void*)&sb::pg_kernel<true, T> : (const void*)&sb::pg_kernel<false, T>;
 }
 
@@ -1781,6 +1786,11 @@ static int factor_prepare(sb_problem* P) {
 }  // extern "C"
 
 namespace {
+bool cand_phase_for(const sb_problem* P, const sb_pg_options* o) {
+  const bool lineSearch = o->variant == SB_PG_APGD || o->variant == SB_PG_DESCENT || o->variant == SB_PG_SPG;
+  return o->op == SB_OP_MATRIX_FREE && lineSearch && P->nc >= kCandPhaseContacts;
+}
+
 // The projected-gradient solve of a prepared problem on device vector x (warm start in,
 // result out): one persistent launch per stage (FP32 then FP64 in mixed precision), grid
 // state from the context or the fixed array (fixed != null), results into stats.
@@ -1807,7 +1817,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     P->fill(a.pb, true);
     a.pb.x_io = x;
     a.pb.out = dout;
-    const void* fn = pg_grid_fn<float>(mf);
+    const void* fn = pg_grid_fn<float>(mf, cand_phase_for(P, o));
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
     SB_HIP(hipEventRecord(ctx->evm, s));
   }
@@ -1822,7 +1832,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     a.pb.hist = histCap > 0 ? P->hist.p : nullptr;
     a.pb.hist_cap = histCap;
     a.pb.out = dout + 16;
-    const void* fn = pg_grid_fn<double>(mf);
+    const void* fn = pg_grid_fn<double>(mf, cand_phase_for(P, o));
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
   }
   SB_HIP(hipEventRecord(ctx->ev1, s));

@@ -140,6 +140,23 @@ struct CandidateLoader {
   __device__ __forceinline__ void operator()(int j, T xc[3]) const { finish(fetch(j), xc); }
 };
 
+// Contact-major phase of a line-search trial (pg_body's CandPhase): the candidate
+// P(x_i - alpha g_i) of every contact into Xs. Team-synchronised by the caller.
+template <typename Team, typename T>
+__device__ __forceinline__ void candidate_phase(const Team& tm, int nc, const CandidateLoader<Team, T>& ld, T* Xs) {
+  for (int i = tm.tid; i < nc; i += tm.nthreads) {
+    T xs[3];
+    ld(i, xs);
+    tst3<Team, T>(Xs, i, xs);
+  }
+}
+// Column loader of an H^T pass over a stored vector.
+template <typename Team, typename T>
+struct StoredLoader {
+  const T* X;
+  __device__ __forceinline__ void operator()(int j, T xc[3]) const { tld3<Team, T>(X, j, xc); }
+};
+
 struct InitRow {
   double x[3], b[3], mu, s;
 };
@@ -174,7 +191,11 @@ __device__ __forceinline__ void start_point(const PGProblem<T>& pb, int i, T xc[
 // (no update phase; pays where a team barrier is expensive: the whole grid), or by
 // a separate element-wise phase after the accepted trial (cheaper per trial and in
 // registers: teams synchronised by __syncthreads or few blocks).
-template <bool MF, typename T, typename Team, bool Fuse>
+// CandPhase (matrix-free): each line-search trial first stores its candidate
+// P(x - alpha g) in a contact-major phase, and the H^T pass gathers that one vector
+// instead of x, g and mu (fewer cache lines where the vectors do not fit the L2
+// caches); otherwise the candidate is formed inside the gather. Same arithmetic.
+template <bool MF, typename T, typename Team, bool Fuse, bool CandPhase = false>
 __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>& pb, Team& tm, uint64_t tsum[4]) {
   const DevProblem<T>& P = pb.P;
   const int nc = P.nc;
@@ -330,10 +351,11 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const T al = (T)alpha;
         CandidateLoader<Team, T> ld{Xx, Gx, P.mu, al};
+        constexpr bool cph = MF && CandPhase;
         auto lead = [&](int i, T v[3], const TrialRow<T>& r) {
           T xst[3];
           proj_step<T>((T)r.mu, al, r.x, r.g, xst);
-          tst3<Team, T>(pb.X[rxs], i, xst);
+          if (!cph) tst3<Team, T>(pb.X[rxs], i, xst);
           tst3<Team, T>(pb.MX[rxs], i, v);
           double xs[3], vd[3], x[3], g[3], p[3], m[3];
           widen3<T>(xst, xs);
@@ -363,7 +385,13 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
           }
         };
         tp0 = realtime();
-        ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
+        if constexpr (cph) {
+          candidate_phase<Team, T>(tm, nc, ld, pb.X[rxs]);
+          StoredLoader<Team, T> lds{pb.X[rxs]};
+          ok = tm.sync() && apply_op<MF, T>(pb, tm, lds, pre, lead);
+        } else {
+          ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
+        }
         tp1 = realtime();
         ok = ok && tm.template reduce<8>(part, (resMax << 4) | (resMax << 6));
         tsum[0] += tp1 - tp0;
@@ -441,10 +469,11 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         return r;
       };
       CandidateLoader<Team, T> ld{Xx, Gx, P.mu, al};
+      constexpr bool cph = MF && CandPhase;
       auto lead = [&](int i, T v[3], const SpgRow<T>& r) {
         T xst[3];
         proj_step<T>((T)r.mu, al, r.x, r.g, xst);
-        tst3<Team, T>(pb.X[rxs], i, xst);
+        if (!cph) tst3<Team, T>(pb.X[rxs], i, xst);
         tst3<Team, T>(pb.MX[rxs], i, v);
         double xs[3], vd[3], x[3], g[3], p[3], m[3], mp[3];
         widen3<T>(xst, xs);
@@ -486,7 +515,13 @@ __device__ __forceinline__ void pg_body(const PGParams& prm, const PGProblem<T>&
         }
       };
       tp0 = realtime();
-      ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
+      if constexpr (cph) {
+        candidate_phase<Team, T>(tm, nc, ld, pb.X[rxs]);
+        StoredLoader<Team, T> lds{pb.X[rxs]};
+        ok = tm.sync() && apply_op<MF, T>(pb, tm, lds, pre, lead);
+      } else {
+        ok = apply_op<MF, T>(pb, tm, ld, pre, lead);
+      }
       tp1 = realtime();
       ok = ok && tm.template reduce<9>(part, (resMax << 7) | (resMax << 8));
       tsum[0] += tp1 - tp0;
@@ -935,6 +970,17 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
   }
 }
 
+// The matrix-free single-problem solver (APGD, Descent, SPG) with each line-search
+// candidate formed in a contact-major phase of its own.
+template <typename T>
+__global__ void __launch_bounds__(512) pg_kernel_cand(PGGridArgs<T> a) {
+  GridTeam tm(a.gs, blockIdx.x);
+  uint64_t tsum[4] = {0, 0, 0, 0};
+  pg_body<true, T, GridTeam, true, true>(a.prm, a.pb, tm, tsum);
+}
+
+template __global__ void pg_kernel_cand<double>(PGGridArgs<double>);
+template __global__ void pg_kernel_cand<float>(PGGridArgs<float>);
 template __global__ void pg_kernel<false, double>(PGGridArgs<double>);
 template __global__ void pg_kernel<true, double>(PGGridArgs<double>);
 template __global__ void pg_kernel<false, float>(PGGridArgs<float>);

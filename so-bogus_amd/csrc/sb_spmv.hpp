@@ -40,6 +40,8 @@ hipError_t launch_reset_sync(unsigned* words, int n, int* error, hipStream_t s);
 
 template <bool MF, typename T>
 __global__ void pg_kernel(PGGridArgs<T> a);
+template <typename T>
+__global__ void pg_kernel_cand(PGGridArgs<T> a);
 template <bool MF, typename T, int NT>
 __global__ void pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs);
 __global__ void gs_kernel(GSArgs a, const int* wrowptr);

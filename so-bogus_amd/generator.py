@@ -187,6 +187,43 @@ def generate(nc: int, nb: int, seed: int, zipf: float | None = None,
                            meta={"zipf": zipf, "mu_range": mu_range})
 
 
+def body_order(p: FrictionProblem) -> FrictionProblem:
+    """The same problem with its contacts listed in body order, as a simulator that
+    collects contacts body by body passes them: bodies renumbered by first touch along
+    the contacts sorted by their (lower, higher) body, then contacts sorted by their
+    renumbered bodies (each row's blocks kept in increasing body order). W becomes
+    P W P^T for the contact permutation P; the solution is permuted alike. Returns the
+    reordered problem; its meta holds 'contact_perm' (new contact -> old contact) and
+    'body_map' (old body -> new body)."""
+    nc, nb = p.nc, p.nb
+    if not np.all(np.diff(p.rowptr) == 2):
+        raise ValueError("body_order expects two blocks per contact")
+    col = p.colidx.reshape(nc, 2).astype(np.int64)
+    lo, hi = col.min(1), col.max(1)
+    perm0 = np.lexsort((hi, lo))
+    seq = col[perm0].reshape(-1)
+    _, first = np.unique(seq, return_index=True)
+    touched = seq[np.sort(first)]
+    bmap = np.full(nb, -1, dtype=np.int64)
+    bmap[touched] = np.arange(touched.size)
+    rest = np.nonzero(bmap < 0)[0]
+    bmap[rest] = touched.size + np.arange(rest.size)
+    c2 = bmap[col]
+    cperm = np.lexsort((c2.max(1), c2.min(1)))
+    newcol = c2[cperm]
+    hv = p.hvals.reshape(nc, 2, 18)[cperm].copy()
+    swap = newcol[:, 0] > newcol[:, 1]
+    newcol[swap] = newcol[swap][:, ::-1]
+    hv[swap] = hv[swap][:, ::-1]
+    minv = np.empty_like(p.minv)
+    minv[bmap] = p.minv
+    meta = dict(p.meta, contact_perm=cperm, body_map=bmap)
+    return FrictionProblem(nc=nc, nb=nb, rowptr=p.rowptr.copy(), colidx=newcol.reshape(-1).astype(np.int32),
+                           hvals=np.ascontiguousarray(hv.reshape(2 * nc, 18)), minv=minv,
+                           b=p.b.reshape(nc, 3)[cperm].reshape(-1).copy(), mu=p.mu[cperm].copy(), seed=p.seed,
+                           meta=meta)
+
+
 # Named workloads (sizes and seeds of the synthetic benchmark set)
 WORKLOADS = {
     "small-1k": dict(nc=1000, nb=500, seed=1),

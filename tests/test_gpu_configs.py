@@ -141,6 +141,42 @@ def test_zipf_1m_apgd_first_residuals(R, O, ctx, zipf1m):
 
 
 @pytest.fixture(scope="module")
+def c250k(G):
+    # 250,000 contacts / 100,000 bodies (the oracle's APGD passes FB 1e-6 at iteration 1220)
+    return G.generate(250_000, 100_000, seed=12)
+
+
+@pytest.mark.parametrize("variant,precision", [("PG_APGD", "PRECISION_FP64"), ("PG_APGD", "PRECISION_MIXED"),
+                                               ("PG_SPG", "PRECISION_FP64")])
+def test_candidate_phase_solver_250k(R, O, ctx, c250k, variant, precision):
+    """From 200,000 contacts on, matrix-free single-problem solves form each line-search
+    candidate in a contact-major phase of its own (the H^T pass then gathers one
+    vector). At 250,000 contacts: the first residuals follow the oracle's same-algorithm
+    run (1e-6), APGD reaches FB 1e-6 with a result the oracle re-evaluates below
+    1.05e-6, and an SPG run of 60 iterations returns an iterate whose residual the
+    oracle confirms."""
+    print(f"[configs] 250k contacts, candidate phase, {variant} {precision}", flush=True)
+    O.set_threads(16)
+    p = c250k
+    dp = R.DualFrictionProblem(ctx, p, assemble=False)
+    spg = variant == "PG_SPG"
+    tol, iters = (1e-30, 60) if spg else (1e-6, 10000)
+    g = dp.solve_pg(np.zeros(3 * p.nc), variant=getattr(R, variant), law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE,
+                    scaling=R.SCALING_DIAGONAL, tol=tol, max_iters=iters, precision=getattr(R, precision),
+                    hist_cap=64)
+    op = O.OracleProblem(p, assemble=False)
+    chk = op.eval(g.r, coulomb=True, assembled=False, diag_scaling=True)
+    print(f"   {g.iterations} iterations, residual {g.residual:.3e}, oracle {chk:.3e}", flush=True)
+    assert chk == pytest.approx(g.residual, rel=1e-6)
+    if not spg:
+        assert g.converged and chk <= 1.05 * tol
+    if precision == "PRECISION_FP64":
+        ores, ox, oinfo = op.pg_solve(np.zeros(3 * p.nc), tol=1e-30, max_iters=10, coulomb=True, assembled=False,
+                                      diag_scaling=True, variant=getattr(O, variant))
+        assert np.allclose(g.history[:10], oinfo["history"][:10], rtol=1e-6)
+
+
+@pytest.fixture(scope="module")
 def c50k(G):
     return G.workload("contacts-50k")
 
