@@ -397,6 +397,9 @@ def main() -> int:
                          "H^T r + Minv, matrix-free W x; 20 launches each after one warm-up), no JSON")
     ap.add_argument("--flush", action="store_true",
                     help="with --profile-spmv: evict L2 and the Infinity Cache before every launch (768 MB copy)")
+    ap.add_argument("--profile-grid", action="store_true",
+                    help="short run for rocprofv3: the single-problem solver on zipf-1m, 100 APGD iterations in "
+                         "FP64 and in mixed precision (FP32 stage, then the FP64 stage), twice each, no JSON")
     ap.add_argument("--cpu-baseline-child", type=str, default="", help=argparse.SUPPRESS)
     ap.add_argument("--profile-config", type=str, default="",
                     help="with --profile-run: solve with this launch shape (threads,blocks_per_cu,contacts_per_block) "
@@ -430,6 +433,25 @@ def main() -> int:
                 f"{nbytes} compulsory bytes, {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
         ctx.bench_copy(COPY_BYTES, reps=2)
         del dp1m
+        ctx.close()
+        return 0
+    if args.profile_grid:
+        p1m = G.workload("zipf-1m")
+        dp1m = R.DualFrictionProblem(ctx, p1m, assemble=False)
+        r1m = R.DeviceArray(ctx, 3 * p1m.nc)
+        kwg = dict(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL,
+                   tol=args.tol, max_iters=100)
+        for name, prec in (("FP64", R.PRECISION_FP64), ("mixed", R.PRECISION_MIXED)):
+            o = R.pg_options(precision=prec, mixed_stall_iters=0, **kwg)
+            for _ in range(2):
+                r1m.zero()
+                s1 = dp1m.solve_pg(r1m, o, on_device=True)
+                n32 = s1.spmvs - s1.refine_spmvs
+                log(f"profile grid {name}: {s1.device_ms:.1f} ms, {s1.iterations} iterations, {s1.spmvs} operator "
+                    f"applications ({n32 if name == 'mixed' else 0} in the FP32 stage), "
+                    f"{1e3 * s1.device_ms / s1.spmvs:.0f} us per application")
+        ctx.bench_copy(COPY_BYTES, reps=2)
+        del r1m, dp1m
         ctx.close()
         return 0
     t0 = time.perf_counter()

@@ -129,6 +129,7 @@ struct GridSync {
   uint64_t timeout_ticks;  // s_memrealtime ticks (100 MHz)
 };
 constexpr unsigned kSyncGroups = 16;  // blocks are grouped by blockIdx % kSyncGroups
+constexpr unsigned kXcdGroups = 8;    // XCDs of the device: with round-robin dispatch, block b runs on XCD b % 8
 constexpr unsigned kSyncStride = 32;  // one 128-byte line per counter
 constexpr int kMaxPartials = 12;
 
@@ -147,6 +148,9 @@ struct Barrier {
                              // reduction's partial sums go through that L2 too
   bool plain_stores = false; // vectors are written with plain stores: unless one_xcd,
                              // the L2 is written back before arriving (release)
+  bool xcd_wb = false;       // with plain_stores: block b runs on XCD b mod 8 (checked by the
+                             // caller); blocks are grouped by XCD and the last block of each
+                             // XCD to arrive writes that XCD's L2 back, once for all its blocks
   unsigned epoch = 0;
   unsigned nred = 0;         // reductions done (selects the partials buffer)
   double red[kMaxPartials];  // results of the last reduction (identical in every block)
@@ -160,13 +164,20 @@ struct Barrier {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (plain_stores && !one_xcd) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const unsigned G = gs.nblocks < kSyncGroups ? gs.nblocks : kSyncGroups;
+      if (plain_stores && !one_xcd && !xcd_wb) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      // the grouping must stay the same for every barrier of a launch (the counters are
+      // monotonic): plain-store teams are grouped by rank % 8 from their first barrier on,
+      // before xcd_wb or one_xcd is known
+      const unsigned ng = plain_stores ? kXcdGroups : kSyncGroups;
+      const unsigned G = gs.nblocks < ng ? gs.nblocks : ng;
       const unsigned g = rank % G;
       const unsigned gsize = gs.nblocks / G + ((g < gs.nblocks % G) ? 1u : 0u);
       const unsigned old = __hip_atomic_fetch_add(&gs.group[g * kSyncStride], 1u, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
       if (old + 1 == (epoch + 1) * gsize) {
+        // every block of this XCD has stored its data (each waited for its stores
+        // before arriving): one write-back of the XCD's L2 publishes them all
+        if (xcd_wb) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned t = __hip_atomic_fetch_add(gs.top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (t + 1 == (epoch + 1) * G)
           for (unsigned k = 0; k < G; ++k)

@@ -32,6 +32,8 @@
 //    tolerance therefore costs one operator application instead of two.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sb_law.hpp"
 #include "sb_op.hpp"
 #include "sb_problem.hpp"
@@ -868,18 +870,27 @@ __device__ __forceinline__ void pg_body_sc(const PGParams& prm, const PGProblem<
 
 // ------------------------------------------------------------------ kernels
 
+// Single-problem teams: vectors written with plain stores and each
+// XCD's L2 written back once per barrier (GridTeamWB), or written through to memory
+// store by store (GridTeam)
+#ifndef SB_GRID_XCD_WRITEBACK
+#define SB_GRID_XCD_WRITEBACK 1
+#endif
+using GridTeamSel = std::conditional_t<SB_GRID_XCD_WRITEBACK != 0, GridTeamWB, GridTeam>;
+
 // One problem on the whole co-resident grid (the solver state stays in vector
 // registers: at one workgroup per CU the scalar variant's spills cost 3-4 %, and more
 // workgroups per CU did not shorten an operator application).
 template <bool MF, typename T>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
-  GridTeam tm(a.gs, blockIdx.x);
+  GridTeamSel tm(a.gs, blockIdx.x);
+  if (SB_GRID_XCD_WRITEBACK && !tm.group_by_xcd()) return;
   if (a.prm.variant == 0 || a.prm.variant == 2) {  // Standard, Conjugated
-    pg_body_sc<MF, T, GridTeam>(a.prm, a.pb, tm);
+    pg_body_sc<MF, T, GridTeamSel>(a.prm, a.pb, tm);
     return;
   }
   uint64_t tsum[4] = {0, 0, 0, 0};
-  pg_body<MF, T, GridTeam, true>(a.prm, a.pb, tm, tsum);
+  pg_body<MF, T, GridTeamSel, true>(a.prm, a.pb, tm, tsum);
 }
 
 // Teams of several workgroups are placed on one XCD. They write their vectors (and
@@ -974,9 +985,10 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
 // candidate formed in a contact-major phase of its own.
 template <typename T>
 __global__ void __launch_bounds__(512) pg_kernel_cand(PGGridArgs<T> a) {
-  GridTeam tm(a.gs, blockIdx.x);
+  GridTeamSel tm(a.gs, blockIdx.x);
+  if (SB_GRID_XCD_WRITEBACK && !tm.group_by_xcd()) return;
   uint64_t tsum[4] = {0, 0, 0, 0};
-  pg_body<true, T, GridTeam, true, true>(a.prm, a.pb, tm, tsum);
+  pg_body<true, T, GridTeamSel, true, true>(a.prm, a.pb, tm, tsum);
 }
 
 template __global__ void pg_kernel_cand<double>(PGGridArgs<double>);

@@ -53,6 +53,21 @@ struct GridTeamT {
   }
   __device__ bool leader() const { return lead; }
   __device__ __forceinline__ bool sync() { return bar.sync(); }
+  // Plain-store team over a whole grid: checks through one barrier that block b runs
+  // on XCD b mod 8 (round-robin dispatch; the word after the top counter collects
+  // mismatches) and, if every block does, lets each barrier write every XCD's L2 back
+  // once, from the XCD's last arriving block, instead of once per block.
+  __device__ __forceinline__ bool group_by_xcd() {
+    __shared__ unsigned s_mis;
+    unsigned* mis = bar.gs.top + 1;
+    if (threadIdx.x == 0 && xcd_id() != bar.rank % kXcdGroups)
+      __hip_atomic_fetch_or(mis, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!sync()) return false;
+    if (threadIdx.x == 0) s_mis = __hip_atomic_load(mis, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    bar.xcd_wb = Plain && s_mis == 0u;
+    return true;
+  }
   template <int NV>
   __device__ __forceinline__ bool reduce(const double (&v)[NV], unsigned max_mask) {
     if (!bar.reduce<NV>(v, max_mask)) return false;
@@ -78,6 +93,8 @@ struct GridTeamT {
   }
 };
 using GridTeam = GridTeamT<true>;
+// Plain stores; L2 write-back at each barrier, once per XCD after group_by_xcd().
+using GridTeamWB = GridTeamT<true, true>;
 
 // dst[k] = src[k] (converted) for k = tid, tid + nthreads, ... < n, by a team; src
 // was written before the team's last barrier (or by the same thread). Groups of
