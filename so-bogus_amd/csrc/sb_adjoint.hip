@@ -539,9 +539,12 @@ __device__ void adjoint_body(const AdjArgs& a, Team& tm) {
   }
 }
 
+// Plain vector stores, each XCD's L2 written back once per barrier (as the
+// single-problem solvers).
 __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
-  GridTeam tm(a.gs, blockIdx.x);
-  adjoint_body<GridTeam>(a, tm);
+  GridTeamWB tm(a.gs, blockIdx.x);
+  if (!tm.group_by_xcd()) return;
+  adjoint_body<GridTeamWB>(a, tm);
 }
 
 }  // namespace sb
