@@ -35,12 +35,16 @@ __device__ __forceinline__ void cld3(const double* p, int i, double v[3]) {
   v[1] = mld(p + 3 * i + 1);
   v[2] = mld(p + 3 * i + 2);
 }
-// Lanes per contact in a color phase: the g lanes of a group (g a power of two, at
-// most kGroup, as many as the grid holds for the color in one pass) gather the blocks
-// of the contact's W row together (lane k: blocks k, k + g, ...), so a row costs one
-// dependent memory round trip for its columns and one for its values and x_j; the
-// group's sums are combined by butterfly shuffles, and its first lane solves.
+// Lanes per contact in a color phase: a contact gets g lanes (a power of two, at most
+// kSpread, as many as the grid holds for the color in one pass), of which the first
+// min(g, kGroup) gather the blocks of the contact's W row together (lane k: blocks k,
+// k + g', ...), so a row costs one dependent memory round trip for its columns and
+// one for its values and x_j; their sums are combined by butterfly shuffles, and the
+// first lane solves. Spreading the contacts over every wave of the grid (one per wave
+// when the color is small enough) keeps the local solves of different contacts out of
+// one another's divergent paths: a wave runs the union of its solving lanes' paths.
 constexpr int kGroup = 16;
+constexpr int kSpread = 64;
 template <typename T>
 __device__ __forceinline__ void group_gather(const TileMat<T>& W, size_t nnz, int q0, int q1, int i, int sub, int g,
                                              const T* x, T acc[3]) {
@@ -187,12 +191,15 @@ __global__ void __launch_bounds__(256) gs_kernel(GSArgs a, const int* __restrict
       for (int col = 0; col < a.ncolors && ok; ++col) {
         const int c0 = a.colors[col], c1 = a.colors[col + 1];
         double part[1] = {0.0};
-        // g consecutive lanes per contact (g divides 64 and nthreads is a multiple of 64:
-        // a group never straddles a wave and leaves the loop as a whole)
-        int g = kGroup;
-        while (g > 1 && (c1 - c0) * g > nthreads) g >>= 1;
-        for (int sg = tid; sg < (c1 - c0) * g; sg += nthreads) {
-          const int i = c0 + sg / g, sub = sg % g;
+        // gs consecutive lanes per contact (gs divides 64 and nthreads is a multiple of 64:
+        // a group never straddles a wave and leaves the loop as a whole); the gather's
+        // width g and so its sums do not depend on gs
+        int gs = kSpread;
+        while (gs > 1 && (c1 - c0) * gs > nthreads) gs >>= 1;
+        const int g = gs < kGroup ? gs : kGroup;  // the gathering lanes of a contact
+        for (int sg = tid; sg < (c1 - c0) * gs; sg += nthreads) {
+          const int i = c0 + sg / gs, sub = sg % gs;
+          if (sub >= g) continue;
           const unsigned char sk = a.skip[i];
           if (sk) {
             if (sub == 0) a.skip[i] = (unsigned char)(sk - 1);
@@ -413,7 +420,12 @@ __global__ void __launch_bounds__(256) pgs_kernel(GSArgs a, PGSArgs e) {
     for (int col = 0; col < a.ncolors && ok; ++col) {
       const int c0 = a.colors[col], c1 = a.colors[col + 1];
       double part[1] = {0.0};
-      for (int i = c0 + tid; i < c1; i += nthreads) {
+      // one lane per contact, the contacts spread over the waves (see kSpread)
+      int gs = kSpread;
+      while (gs > 1 && (c1 - c0) * gs > nthreads) gs >>= 1;
+      for (int sg = tid; sg < (c1 - c0) * gs; sg += nthreads) {
+        if (sg % gs != 0) continue;
+        const int i = c0 + sg / gs;
         if (a.skip[i]) {
           a.skip[i] = (unsigned char)(a.skip[i] - 1);
           continue;
