@@ -188,15 +188,24 @@ __device__ __forceinline__ void tile_pair(const TileMat<T>& A, int ta, const Til
   }
   Acc va[BR], vb[BR];
 #pragma unroll
-  for (int r = 0; r < BR; ++r) {
-    Acc acc_a = Acc(0), acc_b = Acc(0);
+  for (int r = 0; r < BR; ++r) va[r] = vb[r] = Acc(0);
+  if (aa) {
 #pragma unroll
-    for (int c = 0; c < BC; ++c) {
-      acc_a += Acc(ba[r * BC + c]) * xa[c];
-      acc_b += Acc(bb[r * BC + c]) * xb[c];
+    for (int r = 0; r < BR; ++r) {
+      Acc acc = Acc(0);
+#pragma unroll
+      for (int c = 0; c < BC; ++c) acc += Acc(ba[r * BC + c]) * xa[c];
+      va[r] = acc;
     }
-    va[r] = aa ? acc_a : Acc(0);
-    vb[r] = ab ? acc_b : Acc(0);
+  }
+  if (ab) {
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+      Acc acc = Acc(0);
+#pragma unroll
+      for (int c = 0; c < BC; ++c) acc += Acc(bb[r * BC + c]) * xb[c];
+      vb[r] = acc;
+    }
   }
   const NoPre::type none{};
   tile_finish<BR>(ta, ia, va, lead, none, carry);
