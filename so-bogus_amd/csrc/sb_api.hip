@@ -2788,6 +2788,7 @@ struct sb_batch {
   DBuf<int> gperm;
   DBuf<double> stage;
   int maxTeams = 0;
+  size_t maxSyncWords = 0;
   int descScaling = -1;       // scaling mode the descriptor arrays were built for
   bool tuned = false;
   sb_batch_config tunedCfg{0, 0, 0};
@@ -2855,9 +2856,25 @@ struct BatchPlan {
   bool mf, mixed;
   int threads, blocksPerCU, contactsPerBlock, grid;
   int nclass, nteams;
+  int poolPerXcd;  // helper teams of class 0's size per XCD (0: none)
   sb::BatchClass cls[sb::kMaxBatchClasses];
   sb::PGParams prm32, prm64;
 };
+
+// Helper teams (blocks whose class has drained join teams of class 0's size on their
+// XCD and take class 0's problems) unless built with SB_BATCH_HELPERS=0.
+#ifndef SB_BATCH_HELPERS
+#define SB_BATCH_HELPERS 1
+#endif
+constexpr int kXcdCount = 8;
+// Team ids (sync words, partials, slots) of a plan: its own teams, then the helpers'.
+int plan_team_ids(const BatchPlan& pl) { return pl.nteams + kXcdCount * pl.poolPerXcd; }
+// Words of the batch's sync buffer: class queues, every team's barrier words, the
+// helpers' join counters (one line per XCD) and states (one line per helper team).
+size_t plan_sync_words(const BatchPlan& pl) {
+  return kQueueWords + (size_t)plan_team_ids(pl) * sb::kTeamSyncWords +
+         (size_t)kXcdCount * sb::kSyncStride * (1 + pl.poolPerXcd);
+}
 
 int pow2ceil(int64_t v) {
   int k = 1;
@@ -2986,7 +3003,7 @@ void batch_classes(const sb_batch* B, BatchPlan& pl, std::vector<int>& order) {
 // Places the teams on the grid (sbhost::batchBlockMap): the k blocks of a team on
 // one XCD; the batch kernel checks this on the device and falls back to
 // L2-writing-back barriers for a team that is not.
-constexpr int kXcds = 8;
+constexpr int kXcds = kXcdCount;
 std::vector<int> batch_block_map(const BatchPlan& pl, bool spread) {
   int k[sb::kMaxBatchClasses], nt[sb::kMaxBatchClasses];
   for (int c = 0; c < pl.nclass; ++c) {
@@ -3023,6 +3040,7 @@ int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, 
   pl.contactsPerBlock = c.contacts_per_block;
   pl.grid = c.blocks_per_cu * ctx->ncu;
   batch_classes(B, pl, order);
+  pl.poolPerXcd = (SB_BATCH_HELPERS && pl.cls[0].k >= 2) ? (pl.grid / kXcdCount) / pl.cls[0].k : 0;
   pl.prm64 = make_params(o, pl.mf);
   pl.prm32 = pl.prm64;
   pl.prm32.tol = o->tol * (o->mixed_tol_factor > 0 ? o->mixed_tol_factor : 0.95);
@@ -3030,12 +3048,15 @@ int batch_plan(sb_batch* B, const sb_pg_options* o, const sb_batch_config* cfg, 
   return SB_OK;
 }
 
-int batch_buffers(sb_batch* B, int nteams) {
-  if (nteams <= B->maxTeams) return SB_OK;
-  SB_HIP(B->syncw.alloc(kQueueWords + (size_t)nteams * sb::kTeamSyncWords));
+int batch_buffers(sb_batch* B, const BatchPlan& pl) {
+  const int nteams = plan_team_ids(pl);
+  const size_t words = plan_sync_words(pl);
+  if (nteams <= B->maxTeams && words <= B->maxSyncWords) return SB_OK;
+  SB_HIP(B->syncw.alloc(words));
   SB_HIP(B->teamPartials.alloc((size_t)nteams * 2 * sb::kMaxTeamBlocks * sb::kMaxPartials));
   SB_HIP(B->teamSlot.alloc(nteams));
   B->maxTeams = nteams;
+  B->maxSyncWords = words;
   // the graph holds the old addresses
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
@@ -3060,6 +3081,10 @@ void batch_args(sb::PGBatchArgs<T>& a, const sb_batch* B, const BatchPlan& pl, c
   a.team_slot = B->teamSlot.p;
   a.error = B->error.p;
   a.timeout_ticks = 100ull * 1000 * 1000 * 20;  // 20 s at 100 MHz
+  a.nteams = pl.nteams;
+  a.pool_per_xcd = pl.poolPerXcd;
+  a.pool_count = B->syncw.p + kQueueWords + (size_t)plan_team_ids(pl) * sb::kTeamSyncWords;
+  a.pool_state = a.pool_count + (size_t)kXcdCount * sb::kSyncStride;
 }
 
 // Captures [reset, FP32 stage, reset, FP64 stage] (or [reset, FP64 stage]) into the
@@ -3079,7 +3104,7 @@ int batch_graph(sb_batch* B, const BatchPlan& pl) {
   batch_args(a64, B, pl, pl.prm64);
   const sb::PGProblem<float>* p32 = B->d32.p;
   const sb::PGProblem<double>* p64 = B->d64.p;
-  const int nwords = (int)(kQueueWords + (size_t)pl.nteams * sb::kTeamSyncWords);
+  const int nwords = (int)plan_sync_words(pl);
   {
     const std::vector<int> bmap = batch_block_map(pl, B->spreadTeams);
     if (bmap.empty() || (int)bmap.size() > 8 * B->ctx->ncu) return SB_ERR_INVALID;
@@ -3119,7 +3144,7 @@ int batch_run(sb_ctx* ctx, sb_batch* B, const sb_pg_options* o, const sb_batch_c
   if ((rc = batch_plan(B, o, cfg, pl, order))) return rc;
   const bool useScal = (o->scaling == SB_SCALING_DIAGONAL) || (o->scaling == SB_SCALING_REFERENCE && !pl.mf);
   if ((rc = batch_descriptors(B, useScal))) return rc;
-  if ((rc = batch_buffers(B, pl.nteams))) return rc;
+  if ((rc = batch_buffers(B, pl))) return rc;
   if ((rc = batch_graph(B, pl))) return rc;
   hipStream_t s = ctx->stream;
   SB_HIP(hipMemcpyAsync(B->d_order.p, order.data(), sizeof(int) * B->n, hipMemcpyHostToDevice, s));

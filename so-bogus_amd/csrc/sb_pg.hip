@@ -914,8 +914,95 @@ using BatchTeamSel =
 // never wait for other teams. A team of k > 1 blocks needs its blocks resident
 // together (the grid is sized to the occupancy limit); its barrier waits are
 // bounded and set the sticky error flag on timeout.
+template <typename T>
+__device__ __forceinline__ GridSync batch_team_sync(const PGBatchArgs<T>& a, int gt, int k) {
+  GridSync g;
+  g.group = a.team_sync + (size_t)gt * kTeamSyncWords;
+  g.release = g.group + kSyncGroups * kSyncStride;
+  g.top = g.group + 2 * kSyncGroups * kSyncStride;
+  g.error = a.error;
+  g.partials = a.team_partials + (size_t)gt * 2 * kMaxTeamBlocks * kMaxPartials;
+  g.nblocks = (unsigned)k;
+  g.timeout_ticks = a.timeout_ticks;
+  return g;
+}
+
+// A block whose own class has no problems left joins a helper team of class 0's size
+// on its XCD (the j-th block of the XCD to join: team j / K, rank j % K). The team's
+// last member starts it (forming -> running) unless a waiting member has found class
+// 0's queue empty and cancelled it (forming -> cancelled); each state change is one
+// compare-and-swap, so every member sees the same outcome. A running helper team
+// solves class 0's problems exactly as class 0's own teams do (same size, same rank
+// order: the same arithmetic), with its blocks on one XCD. Waits are bounded: a team
+// still forming after the deadline is cancelled (nothing is lost: its blocks had no
+// work of their own). Returns whether this block joined a running team (gt, rank).
+template <typename T>
+__device__ __forceinline__ bool batch_helper_join(const PGBatchArgs<T>& a, int& s_next, int& gt, unsigned& rank) {
+  const BatchClass cl = a.cls[0];
+  const int K = cl.k;
+  const int nq = cl.q1 - cl.q0;
+  if (a.pool_per_xcd <= 0 || K < 2) return false;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    s_next = (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+              (int)__hip_atomic_load(&a.queue[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nq)
+                 ? 1
+                 : 0;
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(s_next) == 0) return false;
+  const unsigned x = xcd_id() & 7u;
+  __syncthreads();
+  if (threadIdx.x == 0) s_next = (int)atomicAdd(&a.pool_count[x * kSyncStride], 1u);
+  __syncthreads();
+  const int j = __builtin_amdgcn_readfirstlane(s_next);
+  const int t = j / K;
+  rank = (unsigned)(j % K);
+  if (t >= a.pool_per_xcd) return false;
+  const int pt = (int)x * a.pool_per_xcd + t;
+  unsigned* state = a.pool_state + (size_t)pt * kSyncStride;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int go = 0;
+    if ((int)rank == K - 1) {
+      unsigned expected = 0u;
+      go = __hip_atomic_compare_exchange_strong(state, &expected, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+               ? 1
+               : 0;
+    } else {
+      const uint64_t t0 = realtime();
+      for (;;) {
+        const unsigned v = __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v == 1u) {
+          go = 1;
+          break;
+        }
+        if (v == 2u) break;
+        const bool drained =
+            (int)__hip_atomic_load(&a.queue[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nq ||
+            __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            realtime() - t0 > a.timeout_ticks;
+        if (drained) {
+          unsigned expected = 0u;
+          if (__hip_atomic_compare_exchange_strong(state, &expected, 2u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+            break;  // cancelled by this member
+          continue;  // the last member started the team first: go
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    s_next = go;
+  }
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(s_next) == 0) return false;
+  gt = a.nteams + pt;
+  return true;
+}
+
 template <bool MF, typename T, int NT>
-__global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT <= 256 ? 3 : 1)))
+pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
   // the Nesterov point is formed inside the line-search trials in both stages (with the
   // uniform state in scalar registers the FP32 stage keeps 3 waves per SIMD with it:
   // 153 VGPRs; mixed batch 103.5 -> 99.6 ms against a separate update phase)
@@ -923,61 +1010,58 @@ __global__ void __launch_bounds__(NT) pg_batch_kernel(PGBatchArgs<T> a, const PG
   __shared__ int s_next;
   const int code = a.bmap[blockIdx.x];
   if (code < 0) return;  // a block no team uses
-  const int c = code >> 20;
-  const BatchClass cl = a.cls[c];
-  const int team = (code >> 4) & 0xffff;
-  const unsigned rank = (unsigned)(code & 15);
-  const int nq = cl.q1 - cl.q0;
-  if (cl.k == 1) {
-    for (;;) {
-      if (threadIdx.x == 0) s_next = (int)atomicAdd(&a.queue[c], 1u);
-      __syncthreads();
-      const int k = __builtin_amdgcn_readfirstlane(s_next);
-      __syncthreads();
-      if (k >= nq) break;
-      const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
-      BlockTeam tm;
-      uint64_t tsum[4] = {0, 0, 0, 0};
-      pg_body<MF, T, BlockTeam, kFuse>(a.prm, probs[p], tm, tsum);
-      __syncthreads();
-    }
-    return;
-  }
-  const int gt = cl.team0 + team;
-  GridSync g;
-  g.group = a.team_sync + (size_t)gt * kTeamSyncWords;
-  g.release = g.group + kSyncGroups * kSyncStride;
-  g.top = g.group + 2 * kSyncGroups * kSyncStride;
-  g.error = a.error;
-  g.partials = a.team_partials + (size_t)gt * 2 * kMaxTeamBlocks * kMaxPartials;
-  g.nblocks = (unsigned)cl.k;
-  g.timeout_ticks = a.timeout_ticks;
-  BatchTeamSel tm(g, rank);
-  // Which XCDs the team's blocks run on: one bit each, gathered through the first
-  // barrier. A team on one XCD shares that XCD's L2: its plain stores are visible
-  // to its other blocks without an L2 write-back.
-  unsigned* xcds = g.top + 1;
-  if (threadIdx.x == 0)
-    __hip_atomic_fetch_or(xcds, 1u << xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!tm.sync()) return;
-  if (threadIdx.x == 0) s_next = (int)__hip_atomic_load(xcds, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const unsigned xm = (unsigned)__builtin_amdgcn_readfirstlane(s_next);
-  tm.bar.one_xcd = (xm & (xm - 1)) == 0;
+  // this block's team: the plan's (class, team, rank), later a helper team of class 0
+  int c = code >> 20;
+  unsigned rank = (unsigned)(code & 15);
+  int gt = a.cls[c].team0 + ((code >> 4) & 0xffff);
   for (;;) {
-    // the team leader draws the next problem and hands it over through the barrier
-    if (rank == 0 && threadIdx.x == 0) cst(&a.team_slot[gt], (int)atomicAdd(&a.queue[c], 1u));
-    if (!tm.sync()) break;
-    if (threadIdx.x == 0) s_next = __hip_atomic_load(&a.team_slot[gt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int k = __builtin_amdgcn_readfirstlane(s_next);
-    // the slot is rewritten only after the next problem's first barrier, which every
-    // member passes after this read
-    if (k >= nq) break;
-    const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + k]);
-    uint64_t tsum[4] = {0, 0, 0, 0};
-    pg_body<MF, T, BatchTeamSel, kFuse>(a.prm, probs[p], tm, tsum);
-    if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    const BatchClass cl = a.cls[c];
+    const int nq = cl.q1 - cl.q0;
+    if (cl.k == 1) {
+      for (;;) {
+        if (threadIdx.x == 0) s_next = (int)atomicAdd(&a.queue[c], 1u);
+        __syncthreads();
+        const int q = __builtin_amdgcn_readfirstlane(s_next);
+        __syncthreads();
+        if (q >= nq) break;
+        const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + q]);
+        BlockTeam tm;
+        uint64_t tsum[4] = {0, 0, 0, 0};
+        pg_body<MF, T, BlockTeam, kFuse>(a.prm, probs[p], tm, tsum);
+        __syncthreads();
+      }
+    } else {
+      BatchTeamSel tm(batch_team_sync(a, gt, cl.k), rank);
+      // Which XCDs the team's blocks run on: one bit each, gathered through the first
+      // barrier. A team on one XCD (a helper team always is) shares that XCD's L2: its
+      // plain stores are visible to its other blocks without an L2 write-back.
+      unsigned* xcds = tm.bar.gs.top + 1;
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_or(xcds, 1u << xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!tm.sync()) return;
+      if (threadIdx.x == 0) s_next = (int)__hip_atomic_load(xcds, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      const unsigned xm = (unsigned)__builtin_amdgcn_readfirstlane(s_next);
+      tm.bar.one_xcd = (xm & (xm - 1)) == 0;
+      for (;;) {
+        // the team leader draws the next problem and hands it over through the barrier
+        if (rank == 0 && threadIdx.x == 0) cst(&a.team_slot[gt], (int)atomicAdd(&a.queue[c], 1u));
+        if (!tm.sync()) return;
+        if (threadIdx.x == 0) s_next = __hip_atomic_load(&a.team_slot[gt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int q = __builtin_amdgcn_readfirstlane(s_next);
+        // the slot is rewritten only after the next problem's first barrier, which every
+        // member passes after this read
+        if (q >= nq) break;
+        const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + q]);
+        uint64_t tsum[4] = {0, 0, 0, 0};
+        pg_body<MF, T, BatchTeamSel, kFuse>(a.prm, probs[p], tm, tsum);
+        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+      }
+    }
+    // this block's class has drained: help class 0 (the largest teams) if it has not
+    if (c == 0 || !batch_helper_join(a, s_next, gt, rank)) return;
+    c = 0;
   }
 }
 

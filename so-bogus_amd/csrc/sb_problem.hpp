@@ -100,6 +100,13 @@ struct PGBatchArgs {
   int* team_slot;             // [nteams] problem handed to the team
   int* error;                 // sticky barrier-timeout flag
   uint64_t timeout_ticks;
+  // Helper teams: blocks whose own queue has drained form further teams of class 0's
+  // size on their XCD and take class 0's problems. Team ids from nteams on (sync
+  // words, partials, slot); pool_per_xcd = 0 turns helpers off.
+  int nteams;
+  int pool_per_xcd;           // helper teams per XCD at most
+  unsigned* pool_count;       // [8][kSyncStride] blocks that joined a helper team, per XCD
+  unsigned* pool_state;       // [8 * pool_per_xcd][kSyncStride] 0 forming, 1 running, 2 cancelled
 };
 
 // Reverse-mode derivatives of the projected fixed-point map (matrix-free, FP64).
