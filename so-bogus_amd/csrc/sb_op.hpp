@@ -78,9 +78,12 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
   };
   // FP32: one lane per contact when every row of H holds two blocks (FP64 keeps the
   // tiled pass: its 36 values per lane would cost the kernels their occupancy)
-  if (kPairsPass<T> && P.H.pairs)
-    pairs_loop<T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead);
-  else
+  if (kPairsPass<T> && P.H.pairs) {
+    if constexpr (Team::kPairsCache && sizeof(T) == 4)
+      pairs_loop_cached<T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, tm.pcache);
+    else
+      pairs_loop<T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead);
+  } else
     tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, carry);
   return true;
 }

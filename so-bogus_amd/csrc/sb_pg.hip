@@ -1010,6 +1010,15 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
   __shared__ int s_next;
   const int code = a.bmap[blockIdx.x];
   if (code < 0) return;  // a block no team uses
+  // FP32: the first unit of H-pass block values and column indices of each of the
+  // first kPairsCacheWaves waves stays in LDS while the team solves a problem (9.5 KB
+  // per wave)
+  PairsCache<float> pc{nullptr, false};
+  if constexpr (sizeof(T) == 4 && BlockTeam::kPairsCache) {
+    __shared__ PairsCache<float>::T2 s_pc[kPairsCacheWaves][PairsCache<float>::kWords];
+    const int w = wave_in_block();
+    if (w < kPairsCacheWaves) pc.lds = s_pc[w];
+  }
   // this block's team: the plan's (class, team, rank), later a helper team of class 0
   int c = code >> 20;
   unsigned rank = (unsigned)(code & 15);
@@ -1026,6 +1035,7 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
         if (q >= nq) break;
         const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + q]);
         BlockTeam tm;
+        tm.pcache = pc;
         uint64_t tsum[4] = {0, 0, 0, 0};
         pg_body<MF, T, BlockTeam, kFuse>(a.prm, probs[p], tm, tsum);
         __syncthreads();
@@ -1054,6 +1064,7 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
         // member passes after this read
         if (q >= nq) break;
         const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + q]);
+        tm.pcache = pc;
         uint64_t tsum[4] = {0, 0, 0, 0};
         pg_body<MF, T, BatchTeamSel, kFuse>(a.prm, probs[p], tm, tsum);
         if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;

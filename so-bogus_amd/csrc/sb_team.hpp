@@ -16,8 +16,16 @@
 #include <hip/hip_runtime.h>
 
 #include "sb_device.hpp"
+#include "sb_tiles.hpp"
 
 namespace sb {
+
+// Batch teams keep each wave's first unit of FP32 H-pass block values in LDS for the
+// duration of a problem (pairs_loop_cached); the grid kernels do not.
+#ifndef SB_PAIRS_CACHE
+#define SB_PAIRS_CACHE 1
+#endif
+constexpr int kPairsCacheWaves = 4;  // 256-thread workgroups: 38 KB of LDS
 
 // Acquire = true: the barrier invalidates this XCD's caches and vectors written by
 // other blocks are read with plain loads. Acquire = false: no invalidation; such
@@ -33,6 +41,8 @@ struct GridTeamT {
   // vectors written by other blocks must be read with coherent loads
   static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
   static constexpr bool kScalarState = Scalar;
+  static constexpr bool kPairsCache = Scalar && SB_PAIRS_CACHE != 0;
+  PairsCache<float> pcache{nullptr, false};
   Barrier bar;
   int gw, nwaves, tid, nthreads;
   bool lead;
@@ -118,6 +128,8 @@ __device__ __forceinline__ void team_copy(const Team& tm, TD* dst, const TS* src
 struct BlockTeam {
   static constexpr bool kCoherentReads = false;
   static constexpr bool kScalarState = true;  // used by the batch kernel only
+  static constexpr bool kPairsCache = SB_PAIRS_CACHE != 0;
+  PairsCache<float> pcache{nullptr, false};
   int gw, nwaves, tid, nthreads;
   double red[kMaxPartials];
   __device__ BlockTeam() {

@@ -308,6 +308,77 @@ __device__ __forceinline__ void pairs_loop(const TileMat<T>& A, int first, int s
   }
 }
 
+// LDS copy of the block values and column indices of the first unit of rows that a
+// wave takes in pairs_loop (64 rows: [18][64] 2-vectors, then [64] index pairs), kept
+// across the operator applications of one problem: the units a wave takes depend only
+// on the matrix, the team's wave count and the wave's index. lds == nullptr: no copy.
+template <typename T>
+struct PairsCache {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  static constexpr int kWords = 18 * 64 + 64 * 8 / (int)sizeof(T2);  // T2 units per wave
+  T2* lds;
+  bool valid;  // lds holds the current matrix's values
+};
+
+// pairs_loop with the first unit's values read from (or, on the first call for a
+// matrix, copied into) the wave's LDS slice; the values, and so the sums, are those
+// of pairs_loop.
+template <typename T, typename Loader, typename Pre, typename Leader>
+__device__ __forceinline__ void pairs_loop_cached(const TileMat<T>& A, int first, int stride, Loader& load, Pre& pre,
+                                                  Leader& lead, PairsCache<T>& pc) {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  const int nunits = (A.nrows + SB_WAVE - 1) / SB_WAVE;
+  const size_t nb2 = (size_t)A.nblocks / 2;
+  for (int u = first; u < nunits; u += stride) {
+    const int i = u * SB_WAVE + lane_id();
+    if (i >= A.nrows) continue;
+    auto rp = pre(i);
+    const T2* v2 = reinterpret_cast<const T2*>(A.vals) + i;
+    T2 e[18];
+    int ca, cb;
+    if (u == first && pc.lds != nullptr) {
+      T2* c = pc.lds + lane_id();
+      int2* ci = reinterpret_cast<int2*>(pc.lds + 18 * SB_WAVE) + lane_id();
+      if (pc.valid) {
+        const int2 cc = *ci;
+        ca = cc.x;
+        cb = cc.y;
+#pragma unroll
+        for (int k = 0; k < 18; ++k) e[k] = c[k * SB_WAVE];
+      } else {
+        ca = gld(A.bcol + 2 * i);
+        cb = gld(A.bcol + 2 * i + 1);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) e[k] = gld(v2 + (size_t)k * nb2);
+        *ci = make_int2(ca, cb);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) c[k * SB_WAVE] = e[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 18; ++k) e[k] = gld(v2 + (size_t)k * nb2);
+      ca = gld(A.bcol + 2 * i);
+      cb = gld(A.bcol + 2 * i + 1);
+    }
+    T xa[6], xb[6];
+    load(ca, xa);
+    load(cb, xb);
+    T v[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      T aa = T(0), ab = T(0);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        aa += e[r * 6 + c].x * xa[c];
+        ab += e[r * 6 + c].y * xb[c];
+      }
+      v[r] = ab + aa;
+    }
+    lead(i, v, rp);
+  }
+  pc.valid = true;
+}
+
 // Sum of the carries of long row k (one wave), fixed order: lane-strided partial
 // sums (four tiles in flight per lane) then the xor tree. Result valid in every
 // lane. Coherent: the carries were written by other blocks of the same kernel and
