@@ -1010,10 +1010,10 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
   __shared__ int s_next;
   const int code = a.bmap[blockIdx.x];
   if (code < 0) return;  // a block no team uses
-  // FP32: the first unit of H-pass block values and column indices of each of the
-  // first kPairsCacheWaves waves stays in LDS while the team solves a problem (9.5 KB
-  // per wave)
-  PairsCache<float> pc{nullptr, false};
+  // FP32: the first unit of H-pass block values and column indices, and the indices of
+  // the first two H^T tiles, of each of the first kPairsCacheWaves waves stay in LDS
+  // while the team solves a problem (10.5 KB per wave)
+  PairsCache<float> pc{nullptr, false, false};
   if constexpr (sizeof(T) == 4 && BlockTeam::kPairsCache) {
     __shared__ PairsCache<float>::T2 s_pc[kPairsCacheWaves][PairsCache<float>::kWords];
     const int w = wave_in_block();

@@ -25,7 +25,7 @@ namespace sb {
 #ifndef SB_PAIRS_CACHE
 #define SB_PAIRS_CACHE 1
 #endif
-constexpr int kPairsCacheWaves = 4;  // 256-thread workgroups: 38 KB of LDS
+constexpr int kPairsCacheWaves = 4;  // 256-thread workgroups: 42 KB of LDS
 
 // Acquire = true: the barrier invalidates this XCD's caches and vectors written by
 // other blocks are read with plain loads. Acquire = false: no invalidation; such
@@ -42,7 +42,7 @@ struct GridTeamT {
   static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
   static constexpr bool kScalarState = Scalar;
   static constexpr bool kPairsCache = Scalar && SB_PAIRS_CACHE != 0;
-  PairsCache<float> pcache{nullptr, false};
+  PairsCache<float> pcache{nullptr, false, false};
   Barrier bar;
   int gw, nwaves, tid, nthreads;
   bool lead;
@@ -129,7 +129,7 @@ struct BlockTeam {
   static constexpr bool kCoherentReads = false;
   static constexpr bool kScalarState = true;  // used by the batch kernel only
   static constexpr bool kPairsCache = SB_PAIRS_CACHE != 0;
-  PairsCache<float> pcache{nullptr, false};
+  PairsCache<float> pcache{nullptr, false, false};
   int gw, nwaves, tid, nthreads;
   double red[kMaxPartials];
   __device__ BlockTeam() {

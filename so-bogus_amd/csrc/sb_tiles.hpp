@@ -212,11 +212,30 @@ __device__ __forceinline__ void tile_pair(const TileMat<T>& A, int ta, const Til
   tile_finish<BR>(tb, ib, vb, lead, none, carry);
 }
 
+// LDS copy of the block values and column indices of the first unit of rows that a
+// wave takes in pairs_loop (64 rows: [18][64] 2-vectors, then [64] index pairs), kept
+// across the operator applications of one problem: the units a wave takes depend only
+// on the matrix, the team's wave count and the wave's index. lds == nullptr: no copy.
+// After them, the indices of the first two tiles the wave takes in tile_loop_pairs
+// (its H^T pass): per tile q0, q1, tl, then each lane's row and column.
+template <typename T>
+struct PairsCache {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  static constexpr int kHWords = 18 * 64 + 64 * 8 / (int)sizeof(T2);  // H-pass unit, T2 units
+  static constexpr int kTInts = 8 + 4 * 64;                            // H^T first pair's indices
+  static constexpr int kWords = kHWords + kTInts * 4 / (int)sizeof(T2);
+  T2* lds;
+  bool valid;   // lds holds the current H's unit
+  bool tvalid;  // lds holds the current H^T's first pair of tile indices
+};
+
 // Tiles first, first + stride, ... of A two at a time (tile_pair), with the indices
-// of the next two loaded before the current two are processed.
-template <int BR, int BC, typename T, typename Acc, typename Loader, typename Leader>
+// of the next two loaded before the current two are processed. Cached: the first
+// pair's indices come from (or, on the first call for a matrix, go to) the wave's
+// PairsCache pc (pc->lds == nullptr: no copy).
+template <int BR, int BC, typename T, typename Acc, bool Cached = false, typename Loader, typename Leader>
 __device__ __forceinline__ void tile_loop_pairs(const TileMat<T>& A, int first, int stride, Loader& load,
-                                                Leader& lead, Acc* carry) {
+                                                Leader& lead, Acc* carry, PairsCache<T>* pc = nullptr) {
   if (first >= A.ntiles) return;
   auto idx = [&](int t) {
     if (t < A.ntiles) return tile_indices(A, t);
@@ -227,7 +246,44 @@ __device__ __forceinline__ void tile_loop_pairs(const TileMat<T>& A, int first, 
     none.tl = -1;
     return none;
   };
-  TileIdx ca = idx(first), cb = idx(first + stride);
+  TileIdx ca, cb;
+  if constexpr (Cached) {
+    int* ti = pc->lds != nullptr ? reinterpret_cast<int*>(pc->lds + PairsCache<T>::kHWords) : nullptr;
+    const int l = lane_id();
+    if (ti != nullptr && pc->tvalid) {
+      ca.q0 = uni(ti[0]);
+      ca.q1 = uni(ti[1]);
+      ca.tl = uni(ti[2]);
+      cb.q0 = uni(ti[3]);
+      cb.q1 = uni(ti[4]);
+      cb.tl = uni(ti[5]);
+      ca.row = ti[8 + l];
+      ca.col = ti[8 + SB_WAVE + l];
+      cb.row = ti[8 + 2 * SB_WAVE + l];
+      cb.col = ti[8 + 3 * SB_WAVE + l];
+    } else {
+      ca = idx(first);
+      cb = idx(first + stride);
+      if (ti != nullptr) {
+        if (l == 0) {
+          ti[0] = ca.q0;
+          ti[1] = ca.q1;
+          ti[2] = ca.tl;
+          ti[3] = cb.q0;
+          ti[4] = cb.q1;
+          ti[5] = cb.tl;
+        }
+        ti[8 + l] = ca.row;
+        ti[8 + SB_WAVE + l] = ca.col;
+        ti[8 + 2 * SB_WAVE + l] = cb.row;
+        ti[8 + 3 * SB_WAVE + l] = cb.col;
+        pc->tvalid = true;
+      }
+    }
+  } else {
+    ca = idx(first);
+    cb = idx(first + stride);
+  }
   for (int t = first; t < A.ntiles; t += 2 * stride) {
     const int tn = t + 2 * stride;
     TileIdx na = ca, nb = cb;
@@ -307,18 +363,6 @@ __device__ __forceinline__ void pairs_loop(const TileMat<T>& A, int first, int s
     lead(i, v, rp);
   }
 }
-
-// LDS copy of the block values and column indices of the first unit of rows that a
-// wave takes in pairs_loop (64 rows: [18][64] 2-vectors, then [64] index pairs), kept
-// across the operator applications of one problem: the units a wave takes depend only
-// on the matrix, the team's wave count and the wave's index. lds == nullptr: no copy.
-template <typename T>
-struct PairsCache {
-  typedef T T2 __attribute__((ext_vector_type(2)));
-  static constexpr int kWords = 18 * 64 + 64 * 8 / (int)sizeof(T2);  // T2 units per wave
-  T2* lds;
-  bool valid;  // lds holds the current matrix's values
-};
 
 // pairs_loop with the first unit's values read from (or, on the first call for a
 // matrix, copied into) the wave's LDS slice; the values, and so the sums, are those
