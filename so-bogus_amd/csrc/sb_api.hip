@@ -691,7 +691,9 @@ sb::PGParams make_params(const sb_pg_options* o, bool mf) {
 constexpr int kCandPhaseContacts = 200000;
 
 template <typename T>
-const void* pg_grid_fn(bool mf, bool candPhase = false) {
+const void* pg_grid_fn(bool mf, bool candPhase, int variant) {
+  if (variant == SB_PG_STANDARD || variant == SB_PG_CONJUGATED)
+    return mf ? (const void*)&sb::pg_kernel_sc<true, T> : (const void*)&sb::pg_kernel_sc<false, T>;
   if (mf && candPhase) return (const void*)&sb::pg_kernel_cand<T>;
   return mf ? (const void*)&sb::pg_kernel<true, T> : (const void*)&sb::pg_kernel<false, T>;
 }
@@ -1817,7 +1819,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     P->fill(a.pb, true);
     a.pb.x_io = x;
     a.pb.out = dout;
-    const void* fn = pg_grid_fn<float>(mf, cand_phase_for(P, o));
+    const void* fn = pg_grid_fn<float>(mf, cand_phase_for(P, o), o->variant);
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
     SB_HIP(hipEventRecord(ctx->evm, s));
   }
@@ -1832,7 +1834,7 @@ int pg_run(sb_ctx* ctx, sb_problem* P, const sb_pg_options* o, double* x, int hi
     a.pb.hist = histCap > 0 ? P->hist.p : nullptr;
     a.pb.hist_cap = histCap;
     a.pb.out = dout + 16;
-    const void* fn = pg_grid_fn<double>(mf, cand_phase_for(P, o));
+    const void* fn = pg_grid_fn<double>(mf, cand_phase_for(P, o), o->variant);
     if ((rc = launch_pg_grid(ctx, fixed, fn, &a, threads, o->grid_blocks, workWaves))) return rc;
   }
   SB_HIP(hipEventRecord(ctx->ev1, s));

@@ -43,7 +43,10 @@ template <bool MF, typename T, typename Team, typename Loader, typename Pre, typ
 __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* wbody, T* carry, Team& tm, Loader& ld,
                                                Pre& pre, Leader& lead, Body& body) {
   if (!MF) {
-    tile_loop<3, 3, T, T>(P.W, tm.gw, tm.nwaves, ld, pre, lead, carry);
+    if constexpr (Team::kTileCache && sizeof(T) == 8)
+      tile_loop_cached<3, 3, T, T>(P.W, tm.gw, tm.nwaves, ld, pre, lead, carry, tm.tcache, 0);
+    else
+      tile_loop<3, 3, T, T>(P.W, tm.gw, tm.nwaves, ld, pre, lead, carry);
     if (P.W.nlong > 0) {
       if (!tm.sync()) return false;
       for (int k = tm.gw; k < P.W.nlong; k += tm.nwaves) {
@@ -63,6 +66,8 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
     tile_loop_pairs<6, 3, T, T, true>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry, &tm.pcache);
   else if constexpr (kHtTilePairs<T>)
     tile_loop_pairs<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry);
+  else if constexpr (Team::kTileCache && sizeof(T) == 8)
+    tile_loop_cached<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry, tm.tcache, 1);
   else
     tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry);
   if (!tm.sync()) return false;
@@ -85,8 +90,11 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
       pairs_loop_cached<T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, tm.pcache);
     else
       pairs_loop<T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead);
-  } else
+  } else if constexpr (Team::kTileCache && sizeof(T) == 8) {
+    tile_loop_cached<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, carry, tm.tcache, 0);
+  } else {
     tile_loop<3, 6, T, T>(P.H, tm.gw, tm.nwaves, ldB, pre, lead, carry);
+  }
   return true;
 }
 

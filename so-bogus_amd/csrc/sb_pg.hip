@@ -876,21 +876,41 @@ __device__ __forceinline__ void pg_body_sc(const PGParams& prm, const PGProblem<
 #ifndef SB_GRID_XCD_WRITEBACK
 #define SB_GRID_XCD_WRITEBACK 1
 #endif
-using GridTeamSel = std::conditional_t<SB_GRID_XCD_WRITEBACK != 0, GridTeamWB, GridTeam>;
+using GridTeamSel =
+    std::conditional_t<SB_GRID_XCD_WRITEBACK != 0, GridTeamT<true, true, false, true>, GridTeamT<true, false, false, true>>;
+
+// The grid team's per-wave LDS caches of operator data (waves beyond kGridCacheWaves,
+// or a team without caches, read everything from memory)
+#define SB_GRID_CACHE_ATTACH(MFV, T, tm)                                                      \
+  if constexpr ((MFV) && sizeof(T) == 4 && GridTeamSel::kPairsCache) {                          \
+    __shared__ PairsCache<float>::T2 s_pc[kGridCacheWaves][PairsCache<float>::kWords];          \
+    if (wave_in_block() < kGridCacheWaves) tm.pcache.lds = s_pc[wave_in_block()];               \
+  } else if constexpr (sizeof(T) == 8 && GridTeamSel::kTileCache) {                             \
+    __shared__ double s_tc[kGridCacheWaves][TileCache<double>::kBytes / sizeof(double)];        \
+    if (wave_in_block() < kGridCacheWaves) tm.tcache.lds = reinterpret_cast<char*>(s_tc[wave_in_block()]); \
+  }
 
 // One problem on the whole co-resident grid (the solver state stays in vector
 // registers: at one workgroup per CU the scalar variant's spills cost 3-4 %, and more
 // workgroups per CU did not shorten an operator application).
+// APGD, Descent, SPG.
 template <bool MF, typename T>
 __global__ void __launch_bounds__(512) pg_kernel(PGGridArgs<T> a) {
   GridTeamSel tm(a.gs, blockIdx.x);
+  SB_GRID_CACHE_ATTACH(MF, T, tm)
   if (SB_GRID_XCD_WRITEBACK && !tm.group_by_xcd()) return;
-  if (a.prm.variant == 0 || a.prm.variant == 2) {  // Standard, Conjugated
-    pg_body_sc<MF, T, GridTeamSel>(a.prm, a.pb, tm);
-    return;
-  }
   uint64_t tsum[4] = {0, 0, 0, 0};
   pg_body<MF, T, GridTeamSel, true>(a.prm, a.pb, tm, tsum);
+}
+
+// Standard, Conjugated (a kernel of their own: with both solver bodies in one kernel
+// the FP64 matrix-free kernel spilled registers).
+template <bool MF, typename T>
+__global__ void __launch_bounds__(512) pg_kernel_sc(PGGridArgs<T> a) {
+  GridTeamSel tm(a.gs, blockIdx.x);
+  SB_GRID_CACHE_ATTACH(MF, T, tm)
+  if (SB_GRID_XCD_WRITEBACK && !tm.group_by_xcd()) return;
+  pg_body_sc<MF, T, GridTeamSel>(a.prm, a.pb, tm);
 }
 
 // Teams of several workgroups are placed on one XCD. They write their vectors (and
@@ -1081,6 +1101,7 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
 template <typename T>
 __global__ void __launch_bounds__(512) pg_kernel_cand(PGGridArgs<T> a) {
   GridTeamSel tm(a.gs, blockIdx.x);
+  SB_GRID_CACHE_ATTACH(true, T, tm)
   if (SB_GRID_XCD_WRITEBACK && !tm.group_by_xcd()) return;
   uint64_t tsum[4] = {0, 0, 0, 0};
   pg_body<true, T, GridTeamSel, true, true>(a.prm, a.pb, tm, tsum);
@@ -1092,6 +1113,10 @@ template __global__ void pg_kernel<false, double>(PGGridArgs<double>);
 template __global__ void pg_kernel<true, double>(PGGridArgs<double>);
 template __global__ void pg_kernel<false, float>(PGGridArgs<float>);
 template __global__ void pg_kernel<true, float>(PGGridArgs<float>);
+template __global__ void pg_kernel_sc<false, double>(PGGridArgs<double>);
+template __global__ void pg_kernel_sc<true, double>(PGGridArgs<double>);
+template __global__ void pg_kernel_sc<false, float>(PGGridArgs<float>);
+template __global__ void pg_kernel_sc<true, float>(PGGridArgs<float>);
 
 #define SB_BATCH_INST(MF, T, NT) \
   template __global__ void pg_batch_kernel<MF, T, NT>(PGBatchArgs<T>, const PGProblem<T>* __restrict__);

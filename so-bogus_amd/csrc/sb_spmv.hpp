@@ -40,6 +40,8 @@ hipError_t launch_reset_sync(unsigned* words, int n, int* error, hipStream_t s);
 
 template <bool MF, typename T>
 __global__ void pg_kernel(PGGridArgs<T> a);
+template <bool MF, typename T>
+__global__ void pg_kernel_sc(PGGridArgs<T> a);
 template <typename T>
 __global__ void pg_kernel_cand(PGGridArgs<T> a);
 template <bool MF, typename T, int NT>

@@ -26,6 +26,7 @@ namespace sb {
 #define SB_PAIRS_CACHE 1
 #endif
 constexpr int kPairsCacheWaves = 4;  // 256-thread workgroups: 42 KB of LDS
+constexpr int kGridCacheWaves = 8;   // 512-thread workgroups (FP64: 156 KB of LDS)
 
 // Acquire = true: the barrier invalidates this XCD's caches and vectors written by
 // other blocks are read with plain loads. Acquire = false: no invalidation; such
@@ -36,13 +37,17 @@ constexpr int kPairsCacheWaves = 4;  // 256-thread workgroups: 42 KB of LDS
 // registers (uni): fewer vector registers, one more wave per SIMD for the batch kernel,
 // at the cost of scalar spills (the single-problem grid kernel, one block per CU, keeps
 // them in vector registers).
-template <bool Acquire, bool Plain = false, bool Scalar = false>
+// Cache = true: the single-problem grid kernels' teams, whose waves keep operator data
+// in LDS for the whole solve (FP32: PairsCache; FP64: TileCache).
+template <bool Acquire, bool Plain = false, bool Scalar = false, bool Cache = false>
 struct GridTeamT {
   // vectors written by other blocks must be read with coherent loads
   static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
   static constexpr bool kScalarState = Scalar;
-  static constexpr bool kPairsCache = Scalar && SB_PAIRS_CACHE != 0;
+  static constexpr bool kPairsCache = (Scalar || Cache) && SB_PAIRS_CACHE != 0;
+  static constexpr bool kTileCache = Cache && SB_PAIRS_CACHE != 0;
   PairsCache<float> pcache{nullptr, false, false};
+  TileCache<double> tcache{nullptr, {false, false}};
   Barrier bar;
   int gw, nwaves, tid, nthreads;
   bool lead;
@@ -129,6 +134,7 @@ struct BlockTeam {
   static constexpr bool kCoherentReads = false;
   static constexpr bool kScalarState = true;  // used by the batch kernel only
   static constexpr bool kPairsCache = SB_PAIRS_CACHE != 0;
+  static constexpr bool kTileCache = false;
   PairsCache<float> pcache{nullptr, false, false};
   int gw, nwaves, tid, nthreads;
   double red[kMaxPartials];

@@ -75,9 +75,10 @@ __device__ __forceinline__ void tile_finish(int t, const TileIdx& ix, Acc v[BR],
 // active lane (so the row's own data is fetched while the gathers are in flight);
 // lead(row, v[BR], pre) consumes complete row sums in the row's last lane.
 // Partial sums of long tiles go to carry[t * BR + r].
+// cv != nullptr: the block values come from the wave's LDS copy cv[e * 64 + lane].
 template <int BR, int BC, typename T, typename Acc, bool NT = false, typename Loader, typename Pre, typename Leader>
 __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const TileIdx& ix, Loader& load, Pre& pre,
-                                            Leader& lead, Acc* carry) {
+                                            Leader& lead, Acc* carry, const T* cv = nullptr) {
   const int l = lane_id();
   const int q = ix.q0 + l;
   const bool act = ix.row >= 0;
@@ -94,9 +95,14 @@ __device__ __forceinline__ void tile_gather(const TileMat<T>& A, int t, const Ti
     const T* vq = A.vals + q;
     const size_t nb = (size_t)A.nblocks;
     T blk[BR * BC];
+    if (cv != nullptr) {
 #pragma unroll
-    for (int e = 0; e < BR * BC; ++e)
-      blk[e] = NT ? __builtin_nontemporal_load((const gmem<T>*)(vq + (size_t)e * nb)) : gld(vq + (size_t)e * nb);
+      for (int e = 0; e < BR * BC; ++e) blk[e] = cv[e * SB_WAVE + l];
+    } else {
+#pragma unroll
+      for (int e = 0; e < BR * BC; ++e)
+        blk[e] = NT ? __builtin_nontemporal_load((const gmem<T>*)(vq + (size_t)e * nb)) : gld(vq + (size_t)e * nb);
+    }
     Acc xc[BC];
     load(ix.col, xc);
 #pragma unroll
@@ -318,6 +324,72 @@ __device__ __forceinline__ void tile_loop(const TileMat<T>& A, int first, int st
   if (first >= A.ntiles) return;
   TileIdx cur = tile_indices(A, first);
   for (int t = first; t < A.ntiles; t += stride) {
+    const int tn = t + stride;
+    TileIdx nxt = cur;
+    if (tn < A.ntiles) nxt = tile_indices(A, tn);
+    tile_gather<BR, BC, T, Acc, SB_SOLVER_NT>(A, t, cur, load, pre, lead, carry);
+    cur = nxt;
+  }
+}
+
+// LDS copy of the first tile a wave takes in one tile_loop pass (its block values,
+// [BR*BC][64], then each lane's row and column and the tile's q0, q1, tl), kept
+// across the operator applications of one problem like PairsCache. Two passes per
+// wave (H and H^T, or W): regions 0 and 1.
+template <typename T>
+struct TileCache {
+  static constexpr int kMaxVals = 18;  // BR * BC of the largest blocks (3 x 6)
+  static constexpr int kPassBytes = kMaxVals * 64 * (int)sizeof(T) + (2 * 64 + 8) * 4;
+  static constexpr int kBytes = 2 * kPassBytes;  // per wave
+  char* lds;  // this wave's slice, or nullptr: no copy
+  bool valid[2];
+};
+
+// tile_loop with the first tile's indices and block values read from region `pass`
+// of the wave's TileCache; the first call for a matrix copies them there first.
+template <int BR, int BC, typename T, typename Acc, typename Loader, typename Pre, typename Leader>
+__device__ __forceinline__ void tile_loop_cached(const TileMat<T>& A, int first, int stride, Loader& load, Pre& pre,
+                                                 Leader& lead, Acc* carry, TileCache<T>& tc, int pass) {
+  static_assert(BR * BC <= TileCache<T>::kMaxVals, "tile cache too small");
+  if (first >= A.ntiles) return;
+  if (tc.lds == nullptr) {
+    tile_loop<BR, BC, T, Acc>(A, first, stride, load, pre, lead, carry);
+    return;
+  }
+  char* base = tc.lds + pass * TileCache<T>::kPassBytes;
+  T* cv = reinterpret_cast<T*>(base);
+  int* ci = reinterpret_cast<int*>(base + TileCache<T>::kMaxVals * 64 * sizeof(T));
+  const int l = lane_id();
+  if (!tc.valid[pass]) {
+    const TileIdx ix = tile_indices(A, first);
+    if (l == 0) {
+      ci[128] = ix.q0;
+      ci[129] = ix.q1;
+      ci[130] = ix.tl;
+    }
+    ci[l] = ix.row;
+    ci[64 + l] = ix.col;
+    if (ix.row >= 0) {
+      const T* vq = A.vals + ix.q0 + l;
+#pragma unroll
+      for (int e = 0; e < BR * BC; ++e) cv[e * SB_WAVE + l] = gld(vq + (size_t)e * A.nblocks);
+    }
+    tc.valid[pass] = true;
+  }
+  TileIdx cur;
+  cur.q0 = uni(ci[128]);
+  cur.q1 = uni(ci[129]);
+  cur.tl = uni(ci[130]);
+  cur.row = ci[l];
+  cur.col = ci[64 + l];
+  {
+    const int tn = first + stride;
+    TileIdx nxt = cur;
+    if (tn < A.ntiles) nxt = tile_indices(A, tn);
+    tile_gather<BR, BC, T, Acc, SB_SOLVER_NT>(A, first, cur, load, pre, lead, carry, cv);
+    cur = nxt;
+  }
+  for (int t = first + stride; t < A.ntiles; t += stride) {
     const int tn = t + stride;
     TileIdx nxt = cur;
     if (tn < A.ntiles) nxt = tile_indices(A, tn);
