@@ -63,7 +63,7 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
   NoPre nopre;
   auto leadA = [&](int k, T v[6], NoPre::type) { body(k, v); };
   if constexpr (kHtTilePairs<T> && Team::kPairsCache && sizeof(T) == 4)
-    tile_loop_pairs<6, 3, T, T, true>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry, &tm.pcache);
+    tile_loop_pairs<6, 3, T, T, true, Team::kTileCache>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry, &tm.pcache);
   else if constexpr (kHtTilePairs<T>)
     tile_loop_pairs<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry);
   else if constexpr (Team::kTileCache && sizeof(T) == 8)

@@ -884,7 +884,11 @@ using GridTeamSel =
 #define SB_GRID_CACHE_ATTACH(MFV, T, tm)                                                      \
   if constexpr ((MFV) && sizeof(T) == 4 && GridTeamSel::kPairsCache) {                          \
     __shared__ PairsCache<float>::T2 s_pc[kGridCacheWaves][PairsCache<float>::kWords];          \
-    if (wave_in_block() < kGridCacheWaves) tm.pcache.lds = s_pc[wave_in_block()];               \
+    __shared__ float s_ht[kGridCacheWaves][18 * SB_WAVE];                                       \
+    if (wave_in_block() < kGridCacheWaves) {                                                    \
+      tm.pcache.lds = s_pc[wave_in_block()];                                                    \
+      tm.pcache.htv = s_ht[wave_in_block()];                                                    \
+    }                                                                                           \
   } else if constexpr (sizeof(T) == 8 && GridTeamSel::kTileCache) {                             \
     __shared__ double s_tc[kGridCacheWaves][TileCache<double>::kBytes / sizeof(double)];        \
     if (wave_in_block() < kGridCacheWaves) tm.tcache.lds = reinterpret_cast<char*>(s_tc[wave_in_block()]); \

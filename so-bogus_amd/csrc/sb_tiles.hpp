@@ -164,17 +164,23 @@ struct SplitLoader<L, std::void_t<decltype(L::kSplit)>> {
 // matrix-free operator): both tiles' block values and column vectors are requested
 // before either tile's products, scan and leaders, so a wave has two tiles' loads in
 // flight. An index set with row -1 in every lane (past the last tile) does nothing.
+// cva != nullptr: tile ta's block values come from the wave's LDS copy cva[e * 64 + lane].
 template <int BR, int BC, typename T, typename Acc, typename Loader, typename Leader>
 __device__ __forceinline__ void tile_pair(const TileMat<T>& A, int ta, const TileIdx& ia, int tb, const TileIdx& ib,
-                                          Loader& load, Leader& lead, Acc* carry) {
+                                          Loader& load, Leader& lead, Acc* carry, const T* cva = nullptr) {
   const int l = lane_id();
   const bool aa = ia.row >= 0, ab = ib.row >= 0;
   const size_t nb = (size_t)A.nblocks;
   T ba[BR * BC], bb[BR * BC];
   if (aa) {
-    const T* vq = A.vals + ia.q0 + l;
+    if (cva != nullptr) {
 #pragma unroll
-    for (int e = 0; e < BR * BC; ++e) ba[e] = gld(vq + (size_t)e * nb);
+      for (int e = 0; e < BR * BC; ++e) ba[e] = cva[e * SB_WAVE + l];
+    } else {
+      const T* vq = A.vals + ia.q0 + l;
+#pragma unroll
+      for (int e = 0; e < BR * BC; ++e) ba[e] = gld(vq + (size_t)e * nb);
+    }
   }
   if (ab) {
     const T* vq = A.vals + ib.q0 + l;
@@ -233,13 +239,17 @@ struct PairsCache {
   T2* lds;
   bool valid;   // lds holds the current H's unit
   bool tvalid;  // lds holds the current H^T's first pair of tile indices
+  // optional: the block values of the first H^T tile ([18][64]; grid teams only)
+  T* htv = nullptr;
+  bool htvalid = false;
 };
 
 // Tiles first, first + stride, ... of A two at a time (tile_pair), with the indices
 // of the next two loaded before the current two are processed. Cached: the first
 // pair's indices come from (or, on the first call for a matrix, go to) the wave's
 // PairsCache pc (pc->lds == nullptr: no copy).
-template <int BR, int BC, typename T, typename Acc, bool Cached = false, typename Loader, typename Leader>
+template <int BR, int BC, typename T, typename Acc, bool Cached = false, bool CacheVals = false, typename Loader,
+          typename Leader>
 __device__ __forceinline__ void tile_loop_pairs(const TileMat<T>& A, int first, int stride, Loader& load,
                                                 Leader& lead, Acc* carry, PairsCache<T>* pc = nullptr) {
   if (first >= A.ntiles) return;
@@ -290,7 +300,32 @@ __device__ __forceinline__ void tile_loop_pairs(const TileMat<T>& A, int first, 
     ca = idx(first);
     cb = idx(first + stride);
   }
-  for (int t = first; t < A.ntiles; t += 2 * stride) {
+  int t0 = first;
+  if constexpr (Cached && CacheVals) {
+    // CacheVals: the first tile's block values from LDS (copied there by the first call)
+    T* cv = pc->htv;
+    if (cv != nullptr) {
+      if (!pc->htvalid) {
+        if (ca.row >= 0) {
+          const T* vq = A.vals + ca.q0 + lane_id();
+#pragma unroll
+          for (int e = 0; e < BR * BC; ++e) cv[e * SB_WAVE + lane_id()] = gld(vq + (size_t)e * A.nblocks);
+        }
+        pc->htvalid = true;
+      }
+      const int tn = first + 2 * stride;
+      TileIdx na = ca, nb = cb;
+      if (tn < A.ntiles) {
+        na = idx(tn);
+        nb = idx(tn + stride);
+      }
+      tile_pair<BR, BC, T, Acc>(A, first, ca, first + stride, cb, load, lead, carry, cv);
+      ca = na;
+      cb = nb;
+      t0 = tn;
+    }
+  }
+  for (int t = t0; t < A.ntiles; t += 2 * stride) {
     const int tn = t + 2 * stride;
     TileIdx na = ca, nb = cb;
     if (tn < A.ntiles) {
