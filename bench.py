@@ -404,6 +404,9 @@ def main() -> int:
     ap.add_argument("--profile-grid", action="store_true",
                     help="short run for rocprofv3: the single-problem solver on zipf-1m, 100 APGD iterations in "
                          "FP64 and in mixed precision (FP32 stage, then the FP64 stage), twice each, no JSON")
+    ap.add_argument("--profile-50k", action="store_true",
+                    help="short run for rocprofv3: the 50k-contact problem to FB 1e-8, FP64 APGD (matrix-free) and "
+                         "Gauss-Seidel (SOC law, assembled W, greedy colors), twice each, no JSON")
     ap.add_argument("--cpu-baseline-child", type=str, default="", help=argparse.SUPPRESS)
     ap.add_argument("--profile-config", type=str, default="",
                     help="with --profile-run: solve with this launch shape (threads,blocks_per_cu,contacts_per_block) "
@@ -456,6 +459,30 @@ def main() -> int:
                     f"{1e3 * s1.device_ms / s1.spmvs:.0f} us per application")
         ctx.bench_copy(COPY_BYTES, reps=2)
         del r1m, dp1m
+        ctx.close()
+        return 0
+    if args.profile_50k:
+        p50 = G.workload("contacts-50k")
+        dp50 = R.DualFrictionProblem(ctx, p50, assemble=True)
+        r50 = R.DeviceArray(ctx, 3 * p50.nc)
+        o = R.pg_options(variant=R.PG_APGD, law=R.LAW_COULOMB, op=R.OP_MATRIX_FREE, scaling=R.SCALING_DIAGONAL,
+                         tol=1e-8, max_iters=20000, precision=R.PRECISION_FP64)
+        for _ in range(2):
+            r50.zero()
+            s = dp50.solve_pg(r50, o, on_device=True)
+            log(f"profile 50k APGD FP64: {s.device_ms:.2f} ms, {s.iterations} iterations, {s.spmvs} operator "
+                f"applications, {1e3 * s.device_ms / s.spmvs:.1f} us per application, residual {s.residual:.2e}")
+        perm, colors = dp50.coloring(R.COLORING_GREEDY)
+        dp50.apply_permutation(perm)
+        for _ in range(2):
+            r50.zero()
+            s = dp50.solve_gs(r50, law=R.LAW_SOC, tol=1e-8, max_iters=3000, colors=colors, on_device=True,
+                              precision=R.PRECISION_FP64)
+            log(f"profile 50k GS (SOC, assembled W, {len(colors) - 1} colors): {s.device_ms:.1f} ms, "
+                f"{s.iterations} sweeps, {1e3 * s.device_ms / max(s.iterations, 1):.0f} us per sweep, "
+                f"residual {s.residual:.2e}")
+        ctx.bench_copy(COPY_BYTES, reps=2)
+        del r50, dp50
         ctx.close()
         return 0
     t0 = time.perf_counter()
