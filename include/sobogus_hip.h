@@ -51,7 +51,8 @@ enum sb_status {
   SB_ERR_UNSUPPORTED = -5,    /* option not available on this path */
   SB_ERR_BARRIER_TIMEOUT = -6,/* a persistent kernel's grid barrier timed out */
   SB_ERR_NAN = -7,            /* non-finite residual */
-  SB_ERR_LIMIT = -8           /* a fixed limit of the entry point is exceeded (sb_status_string says which) */
+  SB_ERR_LIMIT = -8,          /* a fixed limit of the entry point is exceeded (sb_status_string says which) */
+  SB_ERR_IO = -9              /* a file could not be opened, read or written, or is not a local-problem file */
 };
 
 /* Solver variants (Core/BlockSolvers.fwd.hpp:29-46, projected_gradient::Variant) */
@@ -234,6 +235,19 @@ int sb_problem_layout(sb_ctx* ctx, const sb_problem* P, int which, int64_t* size
 /* Supplies an assembled W (lower triangle, 3x3 row-major blocks) instead of assembling it. */
 int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* w_rowptr, const int32_t* w_colidx,
                      const double* w_vals);
+/* A dual problem given by W itself, with no H or Minv: the path of a local problem read from a file
+   (apps/FCLibLoader.cpp:64-73: bogus::convert(W, dual.W, 3, 3); dual.W.prune(tol); b = q; mu).
+   W is the scalar CSR of the 3n x 3n matrix (columns ascending within a row, as compressed row
+   storage). It is converted as bogus::convert does into a symmetric block matrix
+   (Core/Eigen/SparseConversions.hpp:26-95): 3x3 blocks of the lower triangle, each scalar row read up
+   to its first entry right of its diagonal block, diagonal blocks replaced by (D + D^T) / 2; then
+   blocks whose entries are all <= prune_tol in magnitude are removed (SparseBlockMatrixBase::prune,
+   Core/Block/SparseBlockMatrixBase.impl.hpp:286-305). Only the assembled operator exists for such a
+   problem: the matrix-free operator, sb_pgs_solve, sb_adjoint and Minv updates return
+   SB_ERR_UNSUPPORTED or SB_ERR_INVALID. */
+int sb_problem_create_from_w(sb_ctx* ctx, sb_problem** P, int n_contacts, const int32_t* w_rowptr,
+                             const int32_t* w_colidx, const double* w_vals, const double* b, const double* mu,
+                             double prune_tol);
 int sb_problem_destroy(sb_problem* P);
 int sb_problem_sizes(const sb_problem* P, int* n_contacts, int* n_bodies, int64_t* h_blocks, int64_t* w_blocks_lower);
 /* Copies out the assembled lower-triangle W (arrays sized by sb_problem_sizes). */
@@ -256,6 +270,22 @@ int sb_problem_apply_w(sb_ctx* ctx, const sb_problem* P, int op, const double* x
    sum_i |FB(s_i r_i, u_i)|^2 / (n+1), or the max with inf_norm. */
 int sb_fb_residual(sb_ctx* ctx, const sb_problem* P, int law, int op, int scaling, int inf_norm, const double* r,
                    double* residual, int on_device);
+/* FCLib's merit1 of r for the dual problem (apps/FCLibLoader.cpp:40-61, the error the FCLib solver
+   reports): u = W r + b with the assembled (SB_OP_ASSEMBLED) or matrix-free operator;
+   merit = sqrt(sum_i |P_K(r_i - u_i - (mu_i |u_iT|, 0, 0)) - r_i|^2) / (1 + |b|), P_K the projection on
+   the cone of aperture mu_i (SOCLaw::projectOnConstraint). */
+int sb_merit1(sb_ctx* ctx, const sb_problem* P, int op, const double* r, int on_device, double* merit);
+/* A local problem in this library's own file layout (FCLib's HDF5 container is not part of this
+   build; the fields are fclib_local's for a pure 3D Coulomb problem, apps/FCLibLoader.cpp:152-218:
+   W as the scalar CSR of the 3n x 3n matrix, q, mu) with an optional initial r0 (NULL: none).
+   Little-endian: magic "SBLOCAL1", int32 spacedim (3), int32 n, int64 nnz, int32 flags (bit 0: r0),
+   int32 0, int32 rowptr[3n+1], int32 colidx[nnz], double vals[nnz], double q[3n], double mu[n],
+   [double r0[3n]]. sb_local_problem_read checks the header, the file length, the row pointer and the
+   columns (SB_ERR_IO otherwise); with every array argument NULL it returns the sizes only. */
+int sb_local_problem_write(const char* path, int n_contacts, const int32_t* w_rowptr, const int32_t* w_colidx,
+                           const double* w_vals, const double* q, const double* mu, const double* r0);
+int sb_local_problem_read(const char* path, int* n_contacts, int64_t* nnz, int* has_r0, int32_t* w_rowptr,
+                          int32_t* w_colidx, double* w_vals, double* q, double* mu, double* r0);
 
 /* ProjectedGradient::solve on the problem; r is warm start and result (host or device). */
 int sb_pg_solve(sb_ctx* ctx, const sb_problem* P, const sb_pg_options* opts, double* r, int on_device,

@@ -346,3 +346,64 @@ def cpu_count_physical() -> int:
         return len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         return os.cpu_count() or 1
+
+
+# ----------------------------------------------------------------- FCLib local problems
+# Pure-Python restatements (small inputs only) of the dual set-up and error of the
+# reference's FCLib application.
+
+def convert_symmetric_3x3(n_contacts: int, rowptr, colidx, vals):
+    """bogus::convert of a scalar row-major sparse matrix into a symmetric block matrix of
+    3x3 blocks (Core/Eigen/SparseConversions.hpp:26-95): block row I reads scalar rows
+    3I..3I+2, each up to its first entry right of the diagonal block (the `break` at :60
+    and :77); blocks are ordered by column (the std::map of :55); a repeated entry
+    overwrites (:82); the diagonal block becomes (D + D^T) / 2 (:87-93). Returns the
+    lower-triangle block CSR (rowptr, colidx, vals[nblocks, 3, 3])."""
+    rowptr, colidx, vals = np.asarray(rowptr), np.asarray(colidx), np.asarray(vals, dtype=np.float64)
+    brp, bci, bv = [0], [], []
+    for I in range(n_contacts):
+        blocks = {}
+        for i in range(3):
+            for p in range(rowptr[3 * I + i], rowptr[3 * I + i + 1]):
+                J = int(colidx[p]) // 3
+                if J > I:
+                    break
+                blocks.setdefault(J, np.zeros((3, 3)))[i, int(colidx[p]) - 3 * J] = vals[p]
+        if I in blocks:
+            D = blocks[I].copy()
+            blocks[I] = 0.5 * (D + D.T)
+        for J in sorted(blocks):
+            bci.append(J)
+            bv.append(blocks[J])
+        brp.append(len(bci))
+    return (np.array(brp, np.int32), np.array(bci, np.int32),
+            np.array(bv).reshape(-1, 3, 3) if bv else np.zeros((0, 3, 3)))
+
+
+def prune_blocks(rowptr, colidx, vals, precision: float):
+    """SparseBlockMatrixBase::prune (Core/Block/SparseBlockMatrixBase.impl.hpp:286-305): drops the
+    blocks for which is_zero holds, i.e. Eigen's isZero(precision): every |entry| <= precision
+    (Core/Eigen/BlockBindings.hpp:40-43)."""
+    rp, out_ci, out_v = [0], [], []
+    for i in range(len(rowptr) - 1):
+        for p in range(rowptr[i], rowptr[i + 1]):
+            if not np.all(np.abs(vals[p]) <= precision):
+                out_ci.append(int(colidx[p]))
+                out_v.append(vals[p])
+        rp.append(len(out_ci))
+    return (np.array(rp, np.int32), np.array(out_ci, np.int32),
+            np.array(out_v).reshape(-1, 3, 3) if out_v else np.zeros((0, 3, 3)))
+
+
+def merit1(r, u, b, mu) -> float:
+    """FCLib's merit1 as the reference application computes it (apps/FCLibLoader.cpp:40-61):
+    sqrt(sum_i |P_K(r_i - u_i - mu_i |u_iT| e_N) - r_i|^2) / (1 + |b|), P_K the projection on the
+    cone of aperture mu_i (SOCLaw::projectOnConstraint, restated by project above)."""
+    r, u, b, mu = f64(r), f64(u), f64(b), f64(mu)
+    err = 0.0
+    for i in range(mu.size):
+        x = r[3 * i:3 * i + 3] - u[3 * i:3 * i + 3]
+        x[0] -= mu[i] * np.linalg.norm(u[3 * i + 1:3 * i + 3])
+        x = project(mu[i], x)
+        err += float(np.sum((x - r[3 * i:3 * i + 3]) ** 2))
+    return float(np.sqrt(err) / (1.0 + np.linalg.norm(b)))

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <functional>
 #include <queue>
 #include <cmath>
@@ -13,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -309,6 +311,7 @@ struct sb_problem {
   // triangle of W and its tiles' source blocks stay on the device
   bool hostPattern = true;
   bool hostSetup = false;  // created with SB_PROBLEM_HOST_SETUP: pattern changes stay on the host
+  bool dualOnly = false;   // created from W itself (sb_problem_create_from_w): no H, no Minv
   int nnz = 0;        // blocks of H
   int64_t nwl = 0;    // blocks of W's lower triangle (0 without W)
   DBuf<int> d_wrp, d_wci, d_wsrc;
@@ -667,6 +670,12 @@ int check_pg_options(const sb_pg_options* o, bool hasW, bool batch = false) {
     return SB_ERR_INVALID;
   if (o->precision == SB_PRECISION_MIXED && o->mixed_stall_iters < 0) return SB_ERR_INVALID;
   return SB_OK;
+}
+
+// A problem created from W alone has no matrix-free operator.
+int check_pg_problem(const sb_pg_options* o, const sb_problem* P) {
+  if (o->op == SB_OP_MATRIX_FREE && P->dualOnly) return SB_ERR_UNSUPPORTED;
+  return check_pg_options(o, P->hasW);
 }
 
 sb::PGParams make_params(const sb_pg_options* o, bool mf) {
@@ -1053,6 +1062,7 @@ const char* sb_status_string(int status) {
     case SB_ERR_NAN: return "non-finite residual";
     case SB_ERR_LIMIT: return last_error_string().empty() ? "a fixed limit of the entry point is exceeded"
                                                           : last_error_string().c_str();
+    case SB_ERR_IO: return "file could not be opened, read or written, or is not a local-problem file";
     default: return "unknown status";
   }
 }
@@ -1441,6 +1451,98 @@ int sb_problem_set_w(sb_ctx* ctx, sb_problem* P, const int32_t* wrp, const int32
   return upload_problem(P);
 }
 
+int sb_problem_create_from_w(sb_ctx* ctx, sb_problem** out, int nc, const int32_t* rp, const int32_t* ci,
+                             const double* v, const double* b, const double* mu, double prune_tol) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!out || nc <= 0 || !rp || !ci || !v || !b || !mu || !(prune_tol >= 0.0)) return SB_ERR_INVALID;
+  const int n = 3 * nc;
+  if (rp[0] != 0) return SB_ERR_INVALID;
+  for (int i = 0; i < n; ++i)
+    if (rp[i + 1] < rp[i]) return SB_ERR_INVALID;
+  for (int p = 0; p < rp[n]; ++p)
+    if (ci[p] < 0 || ci[p] >= n) return SB_ERR_INVALID;
+  // 3x3 blocks of the lower triangle (bogus::convert into a symmetric block matrix,
+  // SparseConversions.hpp:26-95): block row I reads scalar rows 3I..3I+2, each up to
+  // its first entry right of the diagonal block; the diagonal block is symmetrised;
+  // then blocks whose entries are all <= prune_tol in magnitude are dropped
+  // (SparseBlockMatrixBase::prune, SparseBlockMatrixBase.impl.hpp:286-305).
+  std::vector<int> wrp(nc + 1, 0), wci;
+  std::vector<double> wv;
+  std::map<int, std::array<double, 9>> row;
+  for (int I = 0; I < nc; ++I) {
+    row.clear();
+    for (int i = 0; i < 3; ++i)
+      for (int p = rp[3 * I + i]; p < rp[3 * I + i + 1]; ++p) {
+        const int J = ci[p] / 3;
+        if (J > I) break;
+        row.try_emplace(J, std::array<double, 9>{}).first->second[3 * i + (ci[p] - 3 * J)] = v[p];
+      }
+    auto d = row.find(I);
+    if (d != row.end()) {
+      const std::array<double, 9> D = d->second;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) d->second[3 * r + c] = 0.5 * (D[3 * r + c] + D[3 * c + r]);
+    }
+    for (const auto& [J, B] : row) {
+      bool zero = true;
+      for (double x : B) zero = zero && std::fabs(x) <= prune_tol;
+      if (zero) continue;
+      wci.push_back(J);
+      wv.insert(wv.end(), B.begin(), B.end());
+    }
+    wrp[I + 1] = (int)wci.size();
+  }
+  std::unique_ptr<sb_problem> P(new (std::nothrow) sb_problem());
+  if (!P) return SB_ERR_OOM;
+  // one placeholder body without contacts: H is empty and only the assembled W is used
+  P->ctx = ctx;
+  P->nc = nc;
+  P->nb = 1;
+  P->hostSetup = true;
+  P->dualOnly = true;
+  P->hrp.assign((size_t)nc + 1, 0);
+  P->minv.assign(36, 0.0);
+  for (int k = 0; k < 6; ++k) P->minv[7 * k] = 1.0;
+  P->b.assign(b, b + 3 * (size_t)nc);
+  P->mu.assign(mu, mu + nc);
+  P->hasW = true;
+  P->wFromH = false;
+  P->wrp.swap(wrp);
+  P->wci.swap(wci);
+  P->wv.swap(wv);
+  if ((rc = upload_problem(P.get()))) return rc;
+  *out = P.release();
+  return SB_OK;
+}
+
+int sb_merit1(sb_ctx* ctx, const sb_problem* Pc, int op, const double* r, int on_device, double* merit) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!Pc || !r || !merit || (op != SB_OP_ASSEMBLED && op != SB_OP_MATRIX_FREE)) return SB_ERR_INVALID;
+  sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  if (op == SB_OP_MATRIX_FREE && P->dualOnly) return SB_ERR_UNSUPPORTED;
+  if ((rc = sync_host(P))) return rc;  // |b| from the host copy
+  const size_t n3 = 3 * (size_t)P->nc;
+  hipStream_t s = ctx->stream;
+  if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
+  const DevProblem<double> V = P->view(true);
+  if (op == SB_OP_ASSEMBLED)
+    SB_HIP(sb::launch_w(V, P->rio.p, P->y.p, P->carry.p, s));
+  else
+    SB_HIP(sb::launch_matfree(V, P->rio.p, P->y.p, P->wbody.p, P->carry.p, s));
+  SB_HIP(sb::launch_merit1_terms(P->nc, P->d_mu.p, P->y.p, P->d_b.p, P->rio.p, P->terms.p, s));
+  std::vector<double> t(P->nc);
+  SB_HIP(hipMemcpyAsync(t.data(), P->terms.p, sizeof(double) * P->nc, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipStreamSynchronize(s));
+  double err = 0.0, nb2 = 0.0;
+  for (double x : t) err += x;
+  for (double x : P->b) nb2 += x * x;
+  *merit = std::sqrt(err) / (1.0 + std::sqrt(nb2));
+  return std::isfinite(*merit) ? SB_OK : SB_ERR_NAN;
+}
+
 int sb_problem_destroy(sb_problem* P) {
   if (!P) return SB_ERR_INVALID;
   (void)hipSetDevice(P->ctx->device);
@@ -1691,6 +1793,7 @@ int sb_problem_apply_w(sb_ctx* ctx, const sb_problem* Pc, int op, const double* 
   if (!Pc || !x || !y) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  if (op != SB_OP_ASSEMBLED && P->dualOnly) return SB_ERR_UNSUPPORTED;
   const size_t n3 = 3 * (size_t)P->nc;
   hipStream_t s = ctx->stream;
   const double* dx = x;
@@ -1717,6 +1820,7 @@ int sb_fb_residual(sb_ctx* ctx, const sb_problem* Pc, int law, int op, int scali
   if (!Pc || !r || !residual) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
+  if (op != SB_OP_ASSEMBLED && P->dualOnly) return SB_ERR_UNSUPPORTED;
   const size_t n3 = 3 * (size_t)P->nc;
   hipStream_t s = ctx->stream;
   if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
@@ -1887,7 +1991,7 @@ int sb_pg_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o, doubl
   if (rc) return rc;
   if (!Pc || !o || !r) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
-  if ((rc = check_pg_options(o, P->hasW))) return rc;
+  if ((rc = check_pg_problem(o, P))) return rc;
   if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   const size_t n3 = 3 * (size_t)P->nc;
   if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
@@ -1908,7 +2012,7 @@ int sb_pg_solve_noalloc(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* 
   if (!Pc || !o || !r_dev) return SB_ERR_INVALID;
   if (st && st->history && st->history_capacity > 0) return SB_ERR_INVALID;  // no history buffer here
   sb_problem* P = const_cast<sb_problem*>(Pc);
-  if ((rc = check_pg_options(o, P->hasW))) return rc;
+  if ((rc = check_pg_problem(o, P))) return rc;
   if (!ctx->coop && ctx->coopLaunch) return SB_ERR_UNSUPPORTED;
   if (o->op == SB_OP_MATRIX_FREE && P->fVersion != P->version) {
     // the factored tiles are refreshed in place; their storage comes from an earlier
@@ -2148,6 +2252,7 @@ int sb_pgs_solve(sb_ctx* ctx, const sb_problem* Pc, const sb_gs_options* o, cons
   if (rc) return rc;
   if (!Pc || !o || !r) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (P->dualOnly) return SB_ERR_UNSUPPORTED;  // the product form needs H
   if ((rc = sync_pattern(P))) return rc;
   // Contacts of one color update the body vector in parallel, so they must share no
   // body. Given colors are checked; without colors the natural order is cut into
@@ -2356,7 +2461,7 @@ int sb_pg_solve_cadoux(sb_ctx* ctx, const sb_problem* Pc, const sb_pg_options* o
   if (rc) return rc;
   if (!Pc || !o || !x) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
-  if ((rc = check_pg_options(o, P->hasW))) return rc;
+  if ((rc = check_pg_problem(o, P))) return rc;
   return cadoux_run(ctx, P, o, nullptr, nullptr, 0, iterations, tol_tighten, velocity, x, on_device, residual, st);
 }
 
@@ -2376,6 +2481,7 @@ int sb_problem_update(sb_ctx* ctx, sb_problem* P, const double* b, const double*
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!P) return SB_ERR_INVALID;
+  if (minv && P->dualOnly) return SB_ERR_INVALID;  // no bodies
   if ((rc = sync_host(P))) return rc;
   if (b) P->b.assign(b, b + 3 * (size_t)P->nc);
   if (mu) P->mu.assign(mu, mu + P->nc);
@@ -2411,6 +2517,7 @@ int sb_adjoint(sb_ctx* ctx, const sb_problem* Pc, const sb_adjoint_options* o, c
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!Pc || !o || !r0) return SB_ERR_INVALID;
+  if (Pc->dualOnly) return SB_ERR_UNSUPPORTED;  // derivatives in Minv and the operator need H
   if (o->mode != SB_ADJOINT_UNROLL && o->mode != SB_ADJOINT_IMPLICIT) return SB_ERR_INVALID;
   if (o->mode == SB_ADJOINT_IMPLICIT && !g) return SB_ERR_INVALID;
   if (o->mode == SB_ADJOINT_UNROLL && o->iterations < 1) return SB_ERR_INVALID;
@@ -2708,6 +2815,7 @@ int sb_bench_spmv(sb_ctx* ctx, const sb_problem* Pc, int kind, int reps, double*
   kind &= ~SB_BENCH_FLUSH_CACHES;
   if (!Pc || !ms || reps < 1 || kind < 0 || kind > 3) return SB_ERR_INVALID;
   sb_problem* P = const_cast<sb_problem*>(Pc);
+  if (P->dualOnly && kind != 0) return SB_ERR_UNSUPPORTED;
   if (kind == 0 && !P->hasW) return SB_ERR_INVALID;
   hipStream_t s = ctx->stream;
   const DevProblem<double> V = P->view(true);

@@ -312,6 +312,29 @@ hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* 
   return hipGetLastError();
 }
 
+// FCLib's merit1 terms (apps/FCLibLoader.cpp:40-61): with u_i = (W r)_i + b_i,
+// x = r_i - u_i, x_N -= mu_i |u_iT|, x projected on the cone of aperture mu_i
+// (SOCLaw::projectOnConstraint); the term is |x - r_i|^2.
+__global__ void __launch_bounds__(256) k_merit1_terms(int nc, const double* __restrict__ mu,
+                                                      const double* __restrict__ wr, const double* __restrict__ b,
+                                                      const double* __restrict__ r, double* __restrict__ terms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const double u0 = wr[3 * i] + b[3 * i], u1 = wr[3 * i + 1] + b[3 * i + 1], u2 = wr[3 * i + 2] + b[3 * i + 2];
+  const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+  double x0 = r0 - u0, x1 = r1 - u1, x2 = r2 - u2;
+  x0 -= mu[i] * sqrt(u1 * u1 + u2 * u2);
+  project<double>(mu[i], x0, x1, x2);
+  const double d0 = x0 - r0, d1 = x1 - r1, d2 = x2 - r2;
+  terms[i] = d0 * d0 + d1 * d1 + d2 * d2;
+}
+hipError_t launch_merit1_terms(int nc, const double* mu, const double* wr, const double* b, const double* r,
+                               double* terms, hipStream_t s) {
+  (void)hipGetLastError();
+  if (nc > 0) k_merit1_terms<<<(nc + 255) / 256, 256, 0, s>>>(nc, mu, wr, b, r, terms);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ Cadoux fixed point
 // bout = b0 + s(Wx + b0): the affine term of the next SOC problem, s the De Saxce
 // correction (mu |u_T|, 0, 0) of u = W x + b0 (Cadoux.hpp:72-76).

@@ -33,6 +33,8 @@ hipError_t launch_cadoux_rhs(int nc, const double* mu, const double* wx, const d
 hipError_t launch_vel_shift(int nc, const double* mu, const double* v, double* sv, hipStream_t s);
 hipError_t launch_lincomb(size_t n, double a, const double* x, double c, const double* y, double* out,
                           hipStream_t s);
+hipError_t launch_merit1_terms(int nc, const double* mu, const double* wr, const double* b, const double* r,
+                               double* terms, hipStream_t s);
 hipError_t launch_fb_terms(int nc, int coulomb, const double* mu, const double* scal, const double* u,
                            const double* b, const double* r, double* terms, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -50,7 +50,8 @@ EXPORTED = [
     "sb_bench_spmv", "sb_bench_copy", "sb_last_phase_times", "sb_batch_create", "sb_batch_destroy",
     "sb_batch_sizes", "sb_batch_solve", "sb_batch_autotune", "sb_batch_set_iteration_hints",
     "sb_batch_get_iteration_hints", "sb_memset_device", "sb_adjoint_options_default",
-    "sb_adjoint", "sb_problem_update",
+    "sb_adjoint", "sb_problem_update", "sb_problem_create_from_w", "sb_merit1", "sb_local_problem_write",
+    "sb_local_problem_read",
 ]
 
 
@@ -176,6 +177,11 @@ def lib():
         "sb_adjoint": (C.c_int, [_vp, _vp, C.POINTER(AdjointOptions), _dp, _dp, _dp, _dp, _dp, _dp, _vp,
                                  C.POINTER(AdjointStats)]),
         "sb_problem_update": (C.c_int, [_vp, _vp, _dp, _dp, _dp]),
+        "sb_problem_create_from_w": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]),
+        "sb_merit1": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _dp]),
+        "sb_local_problem_write": (C.c_int, [C.c_char_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]),
+        "sb_local_problem_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_int), _ip, _ip, _dp, _dp, _dp, _dp]),
         "sb_bench_spmv": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp]),
         "sb_last_phase_times": (C.c_int, [_vp, _dp]),
         "sb_bench_copy": (C.c_int, [_vp, C.c_size_t, C.c_int, _dp]),
@@ -383,6 +389,39 @@ class SolveResult:
     mixed_device_ms: float = 0.0
 
 
+def write_local_problem(path, rowptr, colidx, vals, q, mu, r0=None) -> None:
+    """A local problem (W as the scalar CSR of a 3n x 3n matrix, q, mu, optional r0) in the
+    library's file layout (sb_local_problem_write)."""
+    rp, ci, v, qq, m = _i32(rowptr), _i32(colidx), _f64(vals), _f64(q), _f64(mu)
+    n = m.size
+    if rp.size != 3 * n + 1 or qq.size != 3 * n or ci.size < rp[-1] or v.size < rp[-1]:
+        raise ValueError("write_local_problem: rowptr 3n + 1, q 3n, colidx and vals rowptr[-1] entries")
+    r0p = None
+    if r0 is not None:
+        r0a = _f64(r0)
+        if r0a.size != 3 * n:
+            raise ValueError("write_local_problem: r0 has 3n entries")
+        r0p = _d(r0a)
+    _check(lib().sb_local_problem_write(str(path).encode(), n, _i(rp), _i(ci), _d(v), _d(qq), _d(m), r0p),
+           "sb_local_problem_write")
+
+
+def read_local_problem(path) -> dict:
+    """A local problem written by write_local_problem (sb_local_problem_read): dict with n,
+    rowptr, colidx, vals, q, mu and r0 (None when the file has none)."""
+    n, nnz, has = C.c_int(), C.c_int64(), C.c_int()
+    p = str(path).encode()
+    _check(lib().sb_local_problem_read(p, C.byref(n), C.byref(nnz), C.byref(has), None, None, None, None, None,
+                                       None), "sb_local_problem_read")
+    nc, nz = n.value, nnz.value
+    out = dict(n=nc, rowptr=np.zeros(3 * nc + 1, np.int32), colidx=np.zeros(nz, np.int32), vals=np.zeros(nz),
+               q=np.zeros(3 * nc), mu=np.zeros(nc), r0=np.zeros(3 * nc) if has.value else None)
+    _check(lib().sb_local_problem_read(p, C.byref(n), C.byref(nnz), C.byref(has), _i(out["rowptr"]),
+                                       _i(out["colidx"]), _d(out["vals"]), _d(out["q"]), _d(out["mu"]),
+                                       _d(out["r0"]) if has.value else None), "sb_local_problem_read")
+    return out
+
+
 class DualFrictionProblem:
     """u = W r + b with W = H Minv H^T on the device (sb_problem)."""
 
@@ -412,6 +451,32 @@ class DualFrictionProblem:
         self.h = h
         self.assembled = assemble
         return self
+
+    @classmethod
+    def from_w(cls, ctx: Context, nc: int, rowptr, colidx, vals, b, mu,
+               prune_tol: float = 0.0) -> "DualFrictionProblem":
+        """A problem given by W itself (sb_problem_create_from_w): the scalar CSR of the
+        3 nc x 3 nc matrix (int32 row pointer and columns, float64 values), converted to
+        the 3x3 blocks of its lower triangle and pruned as the reference's FCLib path does;
+        only the assembled operator exists for it."""
+        self = cls.__new__(cls)
+        self.ctx, self.nc, self.nb = ctx, int(nc), 0
+        rp, ci, v, bb, m = _i32(rowptr), _i32(colidx), _f64(vals), _f64(b), _f64(mu)
+        if rp.size != 3 * self.nc + 1 or bb.size != 3 * self.nc or m.size != self.nc:
+            raise ValueError("from_w: rowptr has 3 nc + 1 entries, b 3 nc, mu nc")
+        h = _vp()
+        _check(lib().sb_problem_create_from_w(ctx.h, C.byref(h), self.nc, _i(rp), _i(ci), _d(v), _d(bb), _d(m),
+                                              float(prune_tol)), "sb_problem_create_from_w")
+        self.h = h
+        self.assembled = True
+        return self
+
+    def merit1(self, r, op=OP_ASSEMBLED) -> float:
+        """FCLib's merit1 of r (sb_merit1)."""
+        r = _f64(r)
+        out = C.c_double()
+        _check(lib().sb_merit1(self.ctx.h, self.h, op, r.ctypes.data_as(_vp), 0, C.byref(out)), "sb_merit1")
+        return out.value
 
     def layout(self, which: int) -> dict:
         """The tile layout of H (0), H^T (1) or W (2) as the solvers use it."""
