@@ -398,6 +398,33 @@ struct DualFrictionProblem<3> {
     m_inv.clear();
   }
 
+  // The dual problem of a local problem file, as the reference's FCLib application builds it
+  // (apps/FCLibLoader.cpp:64-73: bogus::convert(W, dual.W, 3, 3); dual.W.prune(pruneTol);
+  // dual.b = q; dual.mu = mu): W is the scalar CSR of the 3n x 3n matrix. Only the
+  // assembled operator exists for such a problem (no H): solve with GaussSeidel or with an
+  // assembled ProjectedGradient.
+  void fromLocalProblem(int n, const int* rowptr, const int* colidx, const double* vals, const double* q,
+                        const double* muIn, double pruneTol = 0.0) {
+    b.assign(q, q + 3 * (size_t)n);
+    mu.assign(muIn, muIn + n);
+    sb_problem* P = nullptr;
+    sb_check(sb_problem_create_from_w(m_ctx.get(), &P, n, rowptr, colidx, vals, b.data(), mu.data(), pruneTol),
+             "sb_problem_create_from_w");
+    m_P.reset(P, [](sb_problem* p) { sb_problem_destroy(p); });
+    m_perm.clear();
+    m_inv.clear();
+  }
+
+  // The FCLib application's error of r (apps/FCLibLoader.cpp:40-61): merit1 with u = W r + b.
+  double merit1(const double* r) const {
+    const int nc = (int)mu.size();
+    std::vector<double> rp(r, r + 3 * (size_t)nc);
+    if (permuted()) permuteVector(m_perm, rp);
+    double m = 0;
+    sb_check(sb_merit1(m_ctx.get(), m_P.get(), SB_OP_ASSEMBLED, rp.data(), 0, &m), "sb_merit1");
+    return m;
+  }
+
   // DualFrictionProblem::solveWith(gs, r, staticProblem = false)
   double solveWith(GaussSeidelType& gs, double* r, const bool staticProblem = false) const {
     const int nc = (int)mu.size();

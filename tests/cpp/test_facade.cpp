@@ -223,6 +223,64 @@ static void testBlockMV() {
   }
 }
 
+// The reference's two-contact problem given by its W (apps/FCLibLoader.cpp:64-73, the
+// FCLib application's set-up): W = H MInv H^T and b = w - H MInv f computed here with plain
+// loops, written as a scalar CSR with both triangles, then fromLocalProblem. The known
+// Coulomb solution of tests/GaussSeidel.cpp:19-65 and SOC solves of every PG variant.
+static void testLocalProblem() {
+  SmallFrictionPb pb;
+  const auto& H = pb.primal.H;
+  const std::vector<double>& Mi = pb.primal.MInv;
+  const int nc = 2;
+  // G = H MInv per block, then W = G H^T (dense 6 x 6)
+  double W[6][6] = {};
+  for (int i = 0; i < nc; ++i)
+    for (int p = H.rowPointers()[i]; p < H.rowPointers()[i + 1]; ++p)
+      for (int j = 0; j < nc; ++j)
+        for (int q = H.rowPointers()[j]; q < H.rowPointers()[j + 1]; ++q) {
+          if (H.columnIndices()[p] != H.columnIndices()[q]) continue;
+          const double* A = &H.blockValues()[(size_t)p * 18];
+          const double* B = &H.blockValues()[(size_t)q * 18];
+          const double* M = &Mi[(size_t)H.columnIndices()[p] * 36];
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+              double acc = 0;
+              for (int k = 0; k < 6; ++k)
+                for (int l = 0; l < 6; ++l) acc += A[r * 6 + k] * M[k * 6 + l] * B[c * 6 + l];
+              W[3 * i + r][3 * j + c] += acc;
+            }
+        }
+  std::vector<int> rp(1, 0), ci;
+  std::vector<double> v;
+  for (int r = 0; r < 6; ++r) {
+    for (int c = 0; c < 6; ++c)
+      if (W[r][c] != 0.0) {
+        ci.push_back(c);
+        v.push_back(W[r][c]);
+      }
+    rp.push_back((int)ci.size());
+  }
+  const std::vector<double> q = pb.primal.dualRhs();
+  bogus::DualFrictionProblem<3> dual;
+  dual.fromLocalProblem(nc, rp.data(), ci.data(), v.data(), q.data(), pb.mu.data(), 1e-14);
+  bogus::GaussSeidel gs;
+  std::vector<double> x(6, 1.0);
+  const double res = dual.solveWith(gs, x.data(), false);
+  std::printf("local problem GS: residual %.3e x = %.7f %.7f %.7f %.7f %.7f %.7f merit1 %.3e\n", res, x[0], x[1],
+              x[2], x[3], x[4], x[5], dual.merit1(x.data()));
+  CHECK(res < 1.e-8);
+  CHECK(isApprox(pb.sol, x, 1.e-4));
+  CHECK(dual.merit1(x.data()) < 1.e-6);
+  bogus::ProjectedGradient pg;
+  pg.setTol(1.e-8);
+  for (auto vnt : {bogus::projected_gradient::APGD, bogus::projected_gradient::SPG,
+                   bogus::projected_gradient::Conjugated}) {
+    std::vector<double> y(6, 1.0);
+    pg.setDefaultVariant(vnt);
+    CHECK(dual.solveWith(pg, y.data()) < 1.e-8);
+  }
+}
+
 int main() {
   try {
     testBlockMV();
@@ -230,6 +288,7 @@ int main() {
     testProjectedGradient();
     testProductGaussSeidel();
     testCadoux();
+    testLocalProblem();
   } catch (const std::exception& e) {
     std::printf("EXCEPTION %s\n", e.what());
     return 2;
