@@ -1523,7 +1523,6 @@ int sb_merit1(sb_ctx* ctx, const sb_problem* Pc, int op, const double* r, int on
   sb_problem* P = const_cast<sb_problem*>(Pc);
   if (op == SB_OP_ASSEMBLED && !P->hasW) return SB_ERR_INVALID;
   if (op == SB_OP_MATRIX_FREE && P->dualOnly) return SB_ERR_UNSUPPORTED;
-  if ((rc = sync_host(P))) return rc;  // |b| from the host copy
   const size_t n3 = 3 * (size_t)P->nc;
   hipStream_t s = ctx->stream;
   if ((rc = copy_in(ctx, P->rio.p, r, n3, on_device))) return rc;
@@ -1533,12 +1532,13 @@ int sb_merit1(sb_ctx* ctx, const sb_problem* Pc, int op, const double* r, int on
   else
     SB_HIP(sb::launch_matfree(V, P->rio.p, P->y.p, P->wbody.p, P->carry.p, s));
   SB_HIP(sb::launch_merit1_terms(P->nc, P->d_mu.p, P->y.p, P->d_b.p, P->rio.p, P->terms.p, s));
-  std::vector<double> t(P->nc);
+  std::vector<double> t(P->nc), bv(n3);
   SB_HIP(hipMemcpyAsync(t.data(), P->terms.p, sizeof(double) * P->nc, hipMemcpyDeviceToHost, s));
+  SB_HIP(hipMemcpyAsync(bv.data(), P->d_b.p, sizeof(double) * n3, hipMemcpyDeviceToHost, s));  // b is current on the device
   SB_HIP(hipStreamSynchronize(s));
   double err = 0.0, nb2 = 0.0;
   for (double x : t) err += x;
-  for (double x : P->b) nb2 += x * x;
+  for (double x : bv) nb2 += x * x;
   *merit = std::sqrt(err) / (1.0 + std::sqrt(nb2));
   return std::isfinite(*merit) ? SB_OK : SB_ERR_NAN;
 }
