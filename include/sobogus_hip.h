@@ -118,7 +118,9 @@ typedef struct sb_pg_options {
   double ls_pessimistic;  /* 0.5 */
   double ls_armijo;       /* 0.5 (Standard/Conjugated only) */
   int grid_blocks;        /* 0 = automatic; else persistent grid size (clamped) */
-  int block_threads;      /* 0 = automatic (512); else 256 or 512 threads per block */
+  int block_threads;      /* 0 = automatic (512); else 256 or 512 threads per block (the solver keeps
+                             operator data in up to 156 KB of LDS per workgroup, so a grid of
+                             256-thread workgroups holds one per CU, half the waves) */
   int precision;          /* sb_precision, FP64 by default */
   double mixed_tol_factor;/* mixed: the FP32 stage stops at tol * factor (0.95; its residual is evaluated in FP64
                              from the FP32 iterate and product, within 1e-4 relative of the FP64 stage's
