@@ -66,7 +66,7 @@ __device__ __forceinline__ bool apply_operator(const DevProblem<T>& P, const T* 
     tile_loop_pairs<6, 3, T, T, true, Team::kTileCache>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry, &tm.pcache);
   else if constexpr (kHtTilePairs<T>)
     tile_loop_pairs<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, leadA, carry);
-  else if constexpr (Team::kTileCache && sizeof(T) == 8)
+  else if constexpr (Team::kTileCacheHT && sizeof(T) == 8)
     tile_loop_cached<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry, tm.tcache, 1);
   else
     tile_loop<6, 3, T, T>(P.HT, tm.gw, tm.nwaves, ld, nopre, leadA, carry);

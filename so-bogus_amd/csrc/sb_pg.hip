@@ -1038,10 +1038,17 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
   // the first two H^T tiles, of each of the first kPairsCacheWaves waves stay in LDS
   // while the team solves a problem (10.5 KB per wave)
   PairsCache<float> pc{nullptr, false, false};
+  TileCache<double> tc{nullptr, {false, false}};
   if constexpr (sizeof(T) == 4 && BlockTeam::kPairsCache) {
     __shared__ PairsCache<float>::T2 s_pc[kPairsCacheWaves][PairsCache<float>::kWords];
     const int w = wave_in_block();
     if (w < kPairsCacheWaves) pc.lds = s_pc[w];
+  } else if constexpr (MF && sizeof(T) == 8 && BlockTeam::kTileCache) {
+    // FP64: the first H-pass tile of each of the first kPairsCacheWaves waves (region 0 of
+    // a TileCache only: 9.8 KB per wave)
+    __shared__ double s_tc[kPairsCacheWaves][TileCache<double>::kPassBytes / sizeof(double)];
+    const int w = wave_in_block();
+    if (w < kPairsCacheWaves) tc.lds = reinterpret_cast<char*>(s_tc[w]);
   }
   // this block's team: the plan's (class, team, rank), later a helper team of class 0
   int c = code >> 20;
@@ -1060,6 +1067,7 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
         const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + q]);
         BlockTeam tm;
         tm.pcache = pc;
+        tm.tcache = tc;
         uint64_t tsum[4] = {0, 0, 0, 0};
         pg_body<MF, T, BlockTeam, kFuse>(a.prm, probs[p], tm, tsum);
         __syncthreads();
@@ -1089,6 +1097,7 @@ pg_batch_kernel(PGBatchArgs<T> a, const PGProblem<T>* __restrict__ probs) {
         if (q >= nq) break;
         const int p = __builtin_amdgcn_readfirstlane(a.order[cl.q0 + q]);
         tm.pcache = pc;
+        tm.tcache = tc;
         uint64_t tsum[4] = {0, 0, 0, 0};
         pg_body<MF, T, BatchTeamSel, kFuse>(a.prm, probs[p], tm, tsum);
         if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;

@@ -45,7 +45,10 @@ struct GridTeamT {
   static constexpr bool kCoherentReads = !Acquire || SB_COHERENT_GATHER;
   static constexpr bool kScalarState = Scalar;
   static constexpr bool kPairsCache = (Scalar || Cache) && SB_PAIRS_CACHE != 0;
-  static constexpr bool kTileCache = Cache && SB_PAIRS_CACHE != 0;
+  // FP64 tile copies: the H (or W) pass for grid and batch teams, the H^T pass for grid
+  // teams only (a batch workgroup's LDS holds one region per wave)
+  static constexpr bool kTileCache = (Scalar || Cache) && SB_PAIRS_CACHE != 0;
+  static constexpr bool kTileCacheHT = Cache && SB_PAIRS_CACHE != 0;
   PairsCache<float> pcache{nullptr, false, false};
   TileCache<double> tcache{nullptr, {false, false}};
   Barrier bar;
@@ -134,8 +137,10 @@ struct BlockTeam {
   static constexpr bool kCoherentReads = false;
   static constexpr bool kScalarState = true;  // used by the batch kernel only
   static constexpr bool kPairsCache = SB_PAIRS_CACHE != 0;
-  static constexpr bool kTileCache = false;
+  static constexpr bool kTileCache = SB_PAIRS_CACHE != 0;
+  static constexpr bool kTileCacheHT = false;
   PairsCache<float> pcache{nullptr, false, false};
+  TileCache<double> tcache{nullptr, {false, false}};
   int gw, nwaves, tid, nthreads;
   double red[kMaxPartials];
   __device__ BlockTeam() {
