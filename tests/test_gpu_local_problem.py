@@ -128,3 +128,25 @@ def test_local_problem_file_to_solution(R, O, ctx, pair, tmp_path):
                     max_iters=5000)
     print(f"case: file -> SPG 1k: {s.iterations} it res {s.residual:.2e} merit1 {dl.merit1(s.r):.2e}", flush=True)
     assert s.converged and O.OracleProblem(prob).eval(s.r) <= 1.05e-8
+
+
+def test_from_w_rejects_bad_input(R, ctx):
+    n = 2
+    W = np.eye(3 * n) * 2.0
+    rp, ci, v = dense_to_csr(W)
+    b, mu = np.zeros(3 * n), np.full(n, 0.5)
+    R.DualFrictionProblem.from_w(ctx, n, rp, ci, v, b, mu)  # the valid case
+    bad_col = ci.copy()
+    bad_col[2] = 3 * n
+    bad_rp = rp.copy()
+    bad_rp[3] = bad_rp[2] - 1
+    for args, tol in (((rp, bad_col, v), 0.0), ((bad_rp, ci, v), 0.0), ((rp, ci, v), -1.0)):
+        with pytest.raises(R.SbError):
+            R.DualFrictionProblem.from_w(ctx, n, *args, b, mu, prune_tol=tol)
+    with pytest.raises(ValueError):
+        R.DualFrictionProblem.from_w(ctx, n, rp[:-1], ci, v, b, mu)
+    # every block pruned: W = 0 (each empty row holds one zero block in the tiles), and with
+    # b = 0 the residual of r = 0 is 0
+    dz = R.DualFrictionProblem.from_w(ctx, n, rp, ci, v * 1e-20, b, mu, prune_tol=1e-12)
+    assert dz.w_lower()[0][-1] == 0
+    assert dz.eval_with(np.zeros(3 * n)) == 0.0
